@@ -1,0 +1,259 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures in tests/golden/ from the reference Stim CLI.
+
+The CLI is the reference's own sampler, built from the sources under /root/reference by
+oracle/Makefile into oracle/_ref/stim. This script runs only in the build container (it refuses
+to run when /root/reference or the built CLI is absent). Everything it writes is data:
+circuit text produced by `stim gen`, fault lists with the detection events `stim detect` printed
+for them, per-detector rates derived from `stim analyze_errors`, seeded firing counts, and
+`stim detect` output files in every result format.
+
+    make -C oracle -j8 ref && python tests/golden/make_fixtures.py
+"""
+
+import json
+import os
+import random
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.dirname(HERE))
+
+from oracle import frame_oracle as fo  # noqa: E402
+from oracle import stim_text as st  # noqa: E402
+import zoo  # noqa: E402
+
+STIM = os.path.join(ROOT, 'oracle', '_ref', 'stim')
+
+
+def run(args, stdin=None):
+    return subprocess.run([STIM] + args, input=stdin, capture_output=True, check=True, text=True).stdout
+
+
+def gen(code, task, d, r, p, extra=()):
+    args = ['gen', '--code', code, '--task', task, '--distance', str(d), '--rounds', str(r)]
+    if p:
+        args += ['--after_clifford_depolarization', str(p), '--after_reset_flip_probability', str(p),
+                 '--before_measure_flip_probability', str(p)]
+    args += list(extra)
+    return run(args)
+
+
+def fmt_args(a):
+    return '(' + ', '.join(repr(float(x)) for x in a) + ')' if a else ''
+
+
+PAULI_OF = {1: 'X', 2: 'Z', 3: 'Y'}
+
+
+def materialize(flat, faults):
+    """Flat circuit text with all noise stripped and the given faults made deterministic.
+
+    A fault (k, j, code) becomes X/Y/Z_ERROR(1), E(1) on the pair, a p=1 noisy result on that one
+    result, a p=1 link of a correlated chain, or a p=1 heralded Pauli channel outcome."""
+    fk = {}
+    for (k, j, c) in faults:
+        fk.setdefault(k, {})
+        assert j not in fk[k], 'one fault per site'
+        fk[k][j] = c
+    out = []
+    for k, ins in enumerate(flat):
+        n, T, a = ins.name, ins.targets, ins.args
+        F = fk.get(k, {})
+        tstr = lambda ts: ' '.join(repr(t) for t in ts)  # noqa: E731
+        if n in ('X_ERROR', 'Y_ERROR', 'Z_ERROR', 'DEPOLARIZE1', 'PAULI_CHANNEL_1', 'I_ERROR'):
+            for j, c in sorted(F.items()):
+                out.append(f'{PAULI_OF[c & 3]}_ERROR(1) {T[j].value}')
+        elif n in ('DEPOLARIZE2', 'PAULI_CHANNEL_2', 'II_ERROR'):
+            for j, c in sorted(F.items()):
+                terms = []
+                if c & 3:
+                    terms.append(f'{PAULI_OF[c & 3]}{T[2 * j].value}')
+                if (c >> 2) & 3:
+                    terms.append(f'{PAULI_OF[(c >> 2) & 3]}{T[2 * j + 1].value}')
+                out.append('E(1) ' + ' '.join(terms))
+        elif n in st.MEASURE_1Q or n == 'MPAD':
+            for j, t in enumerate(T):
+                out.append(f'{n}{"(1)" if j in F else ""} {t!r}')
+        elif n in st.PAIR_MEASURE:
+            for j in range(len(T) // 2):
+                out.append(f'{n}{"(1)" if j in F else ""} {T[2 * j]!r} {T[2 * j + 1]!r}')
+        elif n == 'MPP':
+            for j, g in enumerate(st.product_groups(T)):
+                out.append(f'MPP{"(1)" if j in F else ""} ' + '*'.join(repr(t) for t in g))
+        elif n in ('E', 'ELSE_CORRELATED_ERROR'):
+            out.append(f'{n}({1 if 0 in F else 0}) {tstr(T)}')
+        elif n in ('HERALDED_ERASE', 'HERALDED_PAULI_CHANNEL_1'):
+            for j, t in enumerate(T):
+                if j in F:
+                    c = F[j]
+                    pv = [0, 0, 0, 0]
+                    pv[{0: 0, 1: 1, 3: 2, 2: 3}[c & 3]] = 1
+                    out.append(f'HERALDED_PAULI_CHANNEL_1({pv[0]}, {pv[1]}, {pv[2]}, {pv[3]}) {t!r}')
+                else:
+                    out.append(f'HERALDED_ERASE(0) {t!r}')
+        else:
+            out.append(f'{n}{fmt_args(a)} {tstr(T)}'.rstrip())
+    return '\n'.join(out) + '\n'
+
+
+def detect_01(text, shots=1):
+    with tempfile.NamedTemporaryFile('w', suffix='.stim', delete=False) as f:
+        f.write(text)
+        path = f.name
+    try:
+        return run(['detect', '--shots', str(shots), '--in', path, '--out_format', '01', '--append_observables'])
+    finally:
+        os.unlink(path)
+
+
+def random_faults(flat, rng, count):
+    chans = fo.noise_sites(flat, allow_chains=True)
+    picks = rng.sample(chans, k=min(count, len(chans)))
+    return [(k, j, rng.choice(outs)[0]) for (k, j, outs) in picks]
+
+
+def fault_fixture(name, text, n_sets, rng, max_faults=4):
+    flat = st.parse_flat(text)
+    stats = st.stats(flat)
+    noiseless = materialize(flat, [])
+    base = detect_01(noiseless, shots=256).split('\n')
+    assert all(set(line) <= {'0'} for line in base if line), f'{name}: nondeterministic detectors'
+    sets = []
+    for i in range(n_sets):
+        nf = 1 if i < n_sets // 2 else rng.randint(2, max_faults)
+        faults = random_faults(flat, rng, nf)
+        line = detect_01(materialize(flat, faults)).strip()
+        sets.append({'faults': [list(f) for f in faults], 'expected': line})
+    return {'name': name, 'circuit': text, 'stats': stats, 'sets': sets}
+
+
+def dem_rates(text):
+    """Per-detector/observable firing probability implied by the reference's detector error model."""
+    with tempfile.NamedTemporaryFile('w', suffix='.stim', delete=False) as f:
+        f.write(text)
+        path = f.name
+    try:
+        dem = run(['analyze_errors', '--in', path, '--approximate_disjoint_errors'])
+    finally:
+        os.unlink(path)
+    stats = st.stats(st.parse_flat(text))
+    nd, no = stats['num_detectors'], stats['num_observables']
+    import math
+    logs = [0.0] * (nd + no)
+    off = 0
+    for line in dem.split('\n'):
+        line = line.strip()
+        if line.startswith('shift_detectors'):
+            off += int(line.split()[-1])
+        elif line.startswith('error('):
+            p = float(line[6:line.index(')')])
+            for tok in line[line.index(')') + 1:].split():
+                if tok.startswith('D'):
+                    logs[off + int(tok[1:])] += math.log(1 - 2 * p)
+                elif tok.startswith('L'):
+                    logs[nd + int(tok[1:])] += math.log(1 - 2 * p)
+        elif line.startswith('repeat'):
+            raise RuntimeError('unexpected folded DEM')
+    return [(1 - math.exp(v)) / 2 for v in logs]
+
+
+def seeded_counts(path, shots, seed):
+    out = subprocess.run([STIM, 'detect', '--shots', str(shots), '--in', path, '--out_format', 'b8',
+                          '--append_observables', '--seed', str(seed)], capture_output=True, check=True).stdout
+    import numpy as np
+    stats = st.stats(st.parse_flat(open(path).read()))
+    n = stats['num_detectors'] + stats['num_observables']
+    arr = np.frombuffer(out, dtype=np.uint8).reshape(shots, (n + 7) // 8)
+    bits = np.unpackbits(arr, axis=1, bitorder='little')[:, :n]
+    return bits.sum(axis=0).astype(int).tolist()
+
+
+def main():
+    if not os.path.isdir('/root/reference') or not os.path.exists(STIM):
+        sys.exit('make_fixtures.py needs /root/reference and oracle/_ref/stim (make -C oracle ref)')
+    rng = random.Random(20261202)
+    cdir = os.path.join(HERE, 'circuits')
+    os.makedirs(cdir, exist_ok=True)
+
+    circuits = {
+        'surface_d3_r3': gen('surface_code', 'rotated_memory_z', 3, 3, 1e-3),
+        'surface_d5_r5': gen('surface_code', 'rotated_memory_z', 5, 5, 1e-3),
+        'surface_d25_r25': gen('surface_code', 'rotated_memory_z', 25, 25, 1e-3),
+        'surface_x_d5_r3': gen('surface_code', 'rotated_memory_x', 5, 3, 2e-3,
+                               ['--before_round_data_depolarization', '0.001']),
+        'unrotated_d3_r2': gen('surface_code', 'unrotated_memory_z', 3, 2, 1e-3),
+        'rep_d7_r5': gen('repetition_code', 'memory', 7, 5, 1e-3),
+        'color_d5_r3': gen('color_code', 'memory_xyz', 5, 3, 1e-3),
+    }
+    for name, text in circuits.items():
+        with open(os.path.join(cdir, name + '.stim'), 'w') as f:
+            f.write(text)
+
+    # Generator pins: stim gen text for the families the build's own generator reproduces.
+    gen_pins = []
+    for (code, task, d, r, p) in [('surface_code', 'rotated_memory_z', 3, 3, 1e-3),
+                                  ('surface_code', 'rotated_memory_z', 7, 4, 0.002),
+                                  ('surface_code', 'rotated_memory_z', 11, 11, 1e-4),
+                                  ('surface_code', 'rotated_memory_x', 5, 5, 0.003),
+                                  ('repetition_code', 'memory', 3, 3, 1e-3),
+                                  ('repetition_code', 'memory', 11, 7, 0.005)]:
+        gen_pins.append({'code': code, 'task': task, 'distance': d, 'rounds': r, 'p': p,
+                         'text': gen(code, task, d, r, p)})
+    json.dump(gen_pins, open(os.path.join(HERE, 'gen_pins.json'), 'w'), indent=0)
+
+    # Fixed-fault vectors from the reference sampler.
+    fixtures = []
+    for name in ['surface_d3_r3', 'surface_d5_r5', 'surface_x_d5_r3', 'unrotated_d3_r2', 'rep_d7_r5', 'color_d5_r3']:
+        fixtures.append(fault_fixture(name, circuits[name], 60, rng))
+    for seed in range(8):
+        text = zoo.gate_zoo(seed, n_qubits=6 + seed % 4, layers=8 + seed)
+        fixtures.append(fault_fixture(f'zoo_{seed}', text, 40, rng))
+    json.dump(fixtures, open(os.path.join(HERE, 'fault_vectors.json'), 'w'))
+
+    # Exact per-detector rates from the reference's detector error model.
+    rates = {name: dem_rates(circuits[name]) for name in ['surface_d3_r3', 'surface_d5_r5', 'surface_x_d5_r3',
+                                                          'rep_d7_r5', 'color_d5_r3']}
+    json.dump(rates, open(os.path.join(HERE, 'dem_rates.json'), 'w'))
+
+    # Seeded firing counts from the reference sampler (statistical comparisons).
+    counts = {}
+    for name, shots in [('surface_d5_r5', 1 << 20), ('surface_d3_r3', 1 << 20)]:
+        counts[name] = {'shots': shots, 'seed': 5,
+                        'counts': seeded_counts(os.path.join(cdir, name + '.stim'), shots, 5)}
+    json.dump(counts, open(os.path.join(HERE, 'seeded_counts.json'), 'w'))
+
+    # Result-format files for a deterministic circuit (every shot identical).
+    det_circuit = ('X_ERROR(1) 0 3 4 9\nM 0 1 2 3 4 5 6 7 8 9 10\n'
+                   + ''.join(f'DETECTOR rec[-{11 - i}]\n' for i in range(11))
+                   + 'OBSERVABLE_INCLUDE(0) rec[-11]\nOBSERVABLE_INCLUDE(2) rec[-8] rec[-2]\n')
+    formats = {'circuit': det_circuit, 'shots': 5, 'outputs': {}}
+    with tempfile.NamedTemporaryFile('w', suffix='.stim', delete=False) as f:
+        f.write(det_circuit)
+        path = f.name
+    for fmt in ['01', 'b8', 'r8', 'hits', 'dets', 'ptb64']:
+        shots = 64 if fmt == 'ptb64' else 5
+        for mode in ['plain', 'append', 'prepend']:
+            extra = {'plain': [], 'append': ['--append_observables'], 'prepend': ['--prepend_observables']}[mode]
+            b = subprocess.run([STIM, 'detect', '--shots', str(shots), '--in', path, '--out_format', fmt] + extra,
+                               capture_output=True, check=True).stdout
+            formats['outputs'][f'{fmt}/{mode}/{shots}'] = b.hex()
+        with tempfile.NamedTemporaryFile(suffix='.obs', delete=False) as g:
+            obs_path = g.name
+        r = subprocess.run([STIM, 'detect', '--shots', str(shots), '--in', path, '--out_format', fmt,
+                            '--obs_out', obs_path, '--obs_out_format', fmt], capture_output=True)
+        if r.returncode == 0:  # the reference aborts on some obs_out formats (e.g. dets); skip those
+            formats['outputs'][f'{fmt}/obs_out_dets/{shots}'] = r.stdout.hex()
+            formats['outputs'][f'{fmt}/obs_out_obs/{shots}'] = open(obs_path, 'rb').read().hex()
+        os.unlink(obs_path)
+    os.unlink(path)
+    json.dump(formats, open(os.path.join(HERE, 'formats.json'), 'w'), indent=0)
+    print('fixtures written')
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,141 @@
+"""Seeded generator of small circuits that exercise every gate the frame path handles.
+
+Every detector and observable it writes is deterministic in the noiseless circuit (measurement
+gadgets compare repeated measurements, resets followed by measurements, feedback resets, heralds),
+so a reference sampler that randomizes its frames still produces the same bits as a fixed-fault
+run. Used by tests/golden/make_fixtures.py (with the reference CLI) and by the GPU parity tests.
+"""
+
+import random
+
+ONE_Q = ['H', 'H_XY', 'H_YZ', 'H_NXY', 'H_NXZ', 'H_NYZ', 'I', 'X', 'Y', 'Z',
+         'C_XYZ', 'C_NXYZ', 'C_XNYZ', 'C_XYNZ', 'C_ZYX', 'C_ZYNX', 'C_ZNYX', 'C_NZYX',
+         'SQRT_X', 'SQRT_X_DAG', 'SQRT_Y', 'SQRT_Y_DAG', 'S', 'S_DAG', 'SQRT_Z', 'SQRT_Z_DAG', 'H_XZ']
+TWO_Q = ['XCX', 'XCY', 'XCZ', 'YCX', 'YCY', 'YCZ', 'CX', 'CY', 'CZ', 'CNOT', 'ZCX', 'ZCY', 'ZCZ', 'II',
+         'SQRT_XX', 'SQRT_XX_DAG', 'SQRT_YY', 'SQRT_YY_DAG', 'SQRT_ZZ', 'SQRT_ZZ_DAG',
+         'SWAP', 'ISWAP', 'ISWAP_DAG', 'CXSWAP', 'SWAPCX', 'CZSWAP', 'SWAPCZ']
+
+
+def _p(rng):
+    return round(rng.uniform(0.001, 0.02), 4)
+
+
+def _noise(rng, qs):
+    kind = rng.randrange(9)
+    a = rng.sample(qs, k=min(len(qs), rng.randint(1, 4)))
+    if kind == 0:
+        return f'X_ERROR({_p(rng)}) ' + ' '.join(map(str, a))
+    if kind == 1:
+        return f'Y_ERROR({_p(rng)}) ' + ' '.join(map(str, a))
+    if kind == 2:
+        return f'Z_ERROR({_p(rng)}) ' + ' '.join(map(str, a))
+    if kind == 3:
+        return f'DEPOLARIZE1({_p(rng)}) ' + ' '.join(map(str, a))
+    if kind == 4:
+        return f'PAULI_CHANNEL_1({_p(rng)}, {_p(rng)}, {_p(rng)}) ' + ' '.join(map(str, a))
+    if kind in (5, 6):
+        b = rng.sample(qs, k=2 * (len(qs) // 2))
+        b = b[:2 * rng.randint(1, max(1, len(b) // 2))]
+        if kind == 5:
+            return f'DEPOLARIZE2({_p(rng)}) ' + ' '.join(map(str, b))
+        args = ', '.join(str(round(rng.uniform(0.0005, 0.003), 5)) for _ in range(15))
+        return f'PAULI_CHANNEL_2({args}) ' + ' '.join(map(str, b))
+    # correlated chain
+    lines = []
+    for link in range(rng.randint(1, 3)):
+        name = 'E' if link == 0 else 'ELSE_CORRELATED_ERROR'
+        ps = rng.sample(qs, k=rng.randint(1, 3))
+        lines.append(f'{name}({_p(rng)}) ' + ' '.join(rng.choice('XYZ') + str(q) for q in ps))
+    return '\n'.join(lines)
+
+
+def _product(rng, qs, w):
+    ps = rng.sample(qs, k=w)
+    return '*'.join(rng.choice('XYZ') + str(q) for q in ps)
+
+
+def _gadget(rng, qs, nobs):
+    """Returns (lines, observable_lines) producing deterministic detectors."""
+    k = rng.randrange(12)
+    q = rng.choice(qs)
+    if k == 0:
+        w = rng.randint(1, min(4, len(qs)))
+        P = _product(rng, qs, w)
+        inv = '!' if rng.random() < 0.3 else ''
+        lines = [f'MPP({_p(rng)}) {inv}{P}', f'MPP {P}', 'DETECTOR rec[-1] rec[-2]']
+        if nobs and rng.random() < 0.5:
+            lines.append(f'OBSERVABLE_INCLUDE({rng.randrange(nobs)}) rec[-1] rec[-2]')
+        return lines
+    if k == 1:
+        m = rng.choice(['M', 'MX', 'MY', 'MZ'])
+        return [f'{m}({_p(rng)}) {q}', f'{m} {q}', 'DETECTOR rec[-1] rec[-2]']
+    if k == 2:
+        mr, m = rng.choice([('MR', 'M'), ('MRX', 'MX'), ('MRY', 'MY'), ('MRZ', 'MZ')])
+        return [f'{mr}({_p(rng)}) {q}', f'{m} {q}', 'DETECTOR rec[-1]']
+    if k == 3:
+        r, m = rng.choice([('R', 'M'), ('RX', 'MX'), ('RY', 'MY'), ('RZ', 'MZ')])
+        return [f'{r} {q}', f'{m} {q}', 'DETECTOR rec[-1]']
+    if k == 4:
+        g = rng.choice(['CX', 'CNOT', 'ZCX'])
+        return [f'M {q}', f'{g} rec[-1] {q}', f'M {q}', 'DETECTOR rec[-1]']
+    if k == 5:
+        # MX then CZ rec feedback resets the qubit to |+>.
+        g = rng.choice(['CZ', 'ZCZ'])
+        order = f'{g} rec[-1] {q}' if rng.random() < 0.5 else f'{g} {q} rec[-1]'
+        return [f'MX {q}', order, f'MX {q}', 'DETECTOR rec[-1]']
+    if k == 6:
+        # MY then CY rec feedback... Y-basis feedback uses a Y flip after a Z measurement.
+        return [f'M {q}', f'CY rec[-1] {q}', f'M {q}', 'DETECTOR rec[-1]']
+    if k == 7 and len(qs) >= 2:
+        a, b = rng.sample(qs, 2)
+        g = rng.choice(['MXX', 'MYY', 'MZZ'])
+        lines = [f'{g}({_p(rng)}) {a} {b}', f'{g} {a} {b}', 'DETECTOR rec[-1] rec[-2]']
+        if nobs and rng.random() < 0.5:
+            lines.append(f'OBSERVABLE_INCLUDE({rng.randrange(nobs)}) rec[-2] rec[-1]')
+        return lines
+    if k == 8:
+        return [f'HERALDED_ERASE({_p(rng)}) {q}', 'DETECTOR rec[-1]']
+    if k == 9:
+        return [f'HERALDED_PAULI_CHANNEL_1({_p(rng)}, {_p(rng)}, {_p(rng)}, {_p(rng)}) {q}', 'DETECTOR rec[-1]']
+    if k == 10:
+        return [f'MPAD({_p(rng)}) 0 1', 'DETECTOR rec[-1]', 'DETECTOR rec[-2]']
+    # sweep-controlled gates are no-ops without a sweep table
+    return [f'CX sweep[{rng.randrange(4)}] {q}', f'CZ {q} sweep[0]']
+
+
+def gate_zoo(seed, n_qubits=8, layers=10, nobs=2, repeat=True):
+    """A random circuit touching every frame rule, noise channel and measurement gadget."""
+    rng = random.Random(seed)
+    qs = list(range(n_qubits))
+    out = ['R ' + ' '.join(map(str, qs))]
+
+    def layer():
+        lines = []
+        k1 = rng.sample(qs, k=rng.randint(1, n_qubits))
+        for q in k1:
+            lines.append(f'{rng.choice(ONE_Q)} {q}')
+        perm = rng.sample(qs, k=n_qubits)
+        pairs = [(perm[2 * i], perm[2 * i + 1]) for i in range(n_qubits // 2)]
+        g = rng.choice(TWO_Q)
+        lines.append(g + ' ' + ' '.join(f'{a} {b}' for a, b in pairs[:rng.randint(1, len(pairs))]))
+        if rng.random() < 0.3:
+            w = rng.randint(1, 3)
+            lines.append(('SPP ' if rng.random() < 0.5 else 'SPP_DAG ') + _product(rng, qs, w))
+        lines.append(_noise(rng, qs))
+        lines.extend(_gadget(rng, qs, nobs))
+        lines.append('TICK')
+        return lines
+
+    for i in range(layers):
+        out.extend(layer())
+        if repeat and i == layers // 2:
+            out.append(f'REPEAT {rng.randint(2, 3)} {{')
+            for _ in range(2):
+                out.extend('    ' + line for line in '\n'.join(layer()).split('\n'))
+            out.append('}')
+    # Frame-dependent observable: measure everything in Z after undoing nothing is not deterministic,
+    # so Pauli-target observables are only added on freshly reset qubits.
+    q = rng.choice(qs)
+    out.append(f'RX {q}')
+    out.append(f'OBSERVABLE_INCLUDE({nobs - 1 if nobs else 0}) X{q}')
+    return '\n'.join(out) + '\n'
