@@ -1,0 +1,99 @@
+/* stim_amd: MI355X-native detection-event sampler, C ABI.
+ *
+ * Replaces, for the bulk Pauli-frame detection-event path, the reference's
+ *   stim.Circuit.compile_detector_sampler(seed=...)
+ *       src/stim/circuit/circuit.pybind.cc:827-830, src/stim/py/compiled_detector_sampler.pybind.cc:151-154
+ *   CompiledDetectorSampler.sample(shots, prepend/append/separate_observables, bit_packed, dets_out, obs_out)
+ *       src/stim/py/compiled_detector_sampler.pybind.cc:34-86, 216-322
+ *   CompiledDetectorSampler.sample_write(shots, filepath, format, ..., obs_out_filepath, obs_out_format)
+ *       src/stim/py/compiled_detector_sampler.pybind.cc:88-141, 357-415
+ *   CompiledDetectorSampler.sample_bit_packed(...)   :324-355 (legacy alias)
+ * Plain pointers and sizes only; every call returns 0 on success or a non-zero error kind, with the
+ * message available from mfs_last_error() (thread-local). Error kinds map to the exceptions the
+ * reference raises through pybind11: MFS_ERR_INVALID_ARGUMENT -> ValueError (std::invalid_argument),
+ * MFS_ERR_OUT_OF_RANGE -> IndexError (std::out_of_range).
+ *
+ * A sampler handle is single-threaded (like the reference object); use one handle per thread.
+ */
+#ifndef STIM_AMD_H
+#define STIM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mfs_sampler mfs_sampler;
+
+enum {
+    MFS_OK = 0,
+    MFS_ERR_INVALID_ARGUMENT = 1, /* ValueError */
+    MFS_ERR_OUT_OF_RANGE = 2,     /* IndexError */
+    MFS_ERR_RUNTIME = 3,          /* RuntimeError (HIP failures, no GPU, ...) */
+};
+
+/* Flags for mfs_sample / mfs_sample_with_faults. */
+enum {
+    MFS_BIT_PACKED = 1u << 0,        /* (S, ceil(n/8)) uint8, bit m of shot s at byte m/8 bit m%8; else (S, n) bytes 0/1 */
+    MFS_PREPEND_OBSERVABLES = 1u << 1,
+    MFS_APPEND_OBSERVABLES = 1u << 2,
+    MFS_SEPARATE_OBSERVABLES = 1u << 3, /* write observables to obs (reference: separate_observables or obs_out) */
+    MFS_DEVICE_OUTPUT = 1u << 4,     /* dets/obs are device pointers (results stay in HBM) */
+};
+
+/* Compiles .stim circuit text. seed is used when has_seed != 0 (reference: seed=None draws
+ * entropy from the OS, probability_util.cc:53-64). */
+int mfs_compile(const char *stim_text, size_t len, uint64_t seed, int has_seed, mfs_sampler **out);
+
+void mfs_destroy(mfs_sampler *s);
+
+/* out[0..5] = num_qubits, num_measurements, num_detectors, num_observables, max_lookback. */
+int mfs_circuit_stats(const mfs_sampler *s, uint64_t out[5]);
+
+/* Parses and compiles circuit text on the host only (no GPU needed) and reports
+ * out[0..9] = num_qubits, num_measurements, num_detectors, num_observables, max_lookback,
+ * device ops, barriers, noise ops, record ring rows, LDS bytes for a 64-shot tile.
+ * Reference: Circuit.compute_stats / num_detectors etc. (src/stim/circuit/circuit_instruction.cc:44-100). */
+int mfs_parse_stats(const char *stim_text, size_t len, uint64_t out[10]);
+
+/* Samples detection events. Row strides are in bytes. dets must hold shots rows of
+ * (packed ? ceil(n/8) : n) bytes where n = D + O*(prepend+append); obs (when
+ * MFS_SEPARATE_OBSERVABLES) holds shots rows of (packed ? ceil(O/8) : O) bytes.
+ * Reference: CompiledDetectorSampler::sample_to_numpy (compiled_detector_sampler.pybind.cc:34-86). */
+int mfs_sample(mfs_sampler *s, uint64_t shots, uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs,
+               size_t obs_stride);
+
+/* Samples and writes to a file in format "01", "b8", "r8", "ptb64", "hits" or "dets".
+ * obs_path may be NULL. prepend + append + (obs_path != NULL) > 1 -> MFS_ERR_OUT_OF_RANGE.
+ * Reference: sample_batch_detection_events_writing_results_to_disk (frame_simulator_util.inl:114-187, 206-288). */
+int mfs_sample_write(mfs_sampler *s, uint64_t shots, const char *path, const char *format, int prepend_observables,
+                     int append_observables, const char *obs_path, const char *obs_format);
+
+/* Fixed-fault mode (no reference counterpart; the reference's equivalent is FlipSimulator with
+ * disable_stabilizer_randomization=True plus broadcast_pauli_errors, frame_simulator.pybind.cc:310-329,
+ * 1365-1421): no noise is drawn; shot s gets exactly the faults faults[3*i .. 3*i+2] for
+ * i in [shot_offsets[s], shot_offsets[s+1]), each (instruction index in the REPEAT-unrolled circuit,
+ * site index inside that instruction, outcome code). Codes: single-qubit channels 1=X 2=Z 3=Y;
+ * two-qubit channels bits (x1, z1, x2, z2); noisy results 1=flip; E/ELSE 1=fires; heralded
+ * channels bit2 = herald, bits 0/1 = X/Z. */
+int mfs_sample_with_faults(mfs_sampler *s, uint64_t shots, const uint64_t *shot_offsets, const uint32_t *faults,
+                           uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs, size_t obs_stride);
+
+/* Benchmark helper: runs warmup + iters samples of `shots` into device-resident packed outputs
+ * (separate observables) and reports the average per-call kernel time in milliseconds measured
+ * with HIP events on the sampler's stream (out_ms[0] = frame kernel, out_ms[1] = whole call). */
+int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, double out_ms[2]);
+
+/* Configuration knobs: words per tile (1, 2 or 4; 0 = automatic). Returns the value in use. */
+int mfs_set_tile_words(mfs_sampler *s, int words);
+int mfs_get_info(const mfs_sampler *s, uint64_t out[8]); /* tile_words, lds_bytes, ops, syncs, noise_ops, randomize, ring, dense_qubits */
+
+const char *mfs_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,71 @@
+"""ctypes binding of libstim_amd.so (the C ABI declared in include/stim_amd.h).
+
+The library is built in-tree by stim_amd/csrc/build.sh. There is no fallback: if the library is
+missing, importing the sampler raises ImportError, and sampling without a GPU raises RuntimeError.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libstim_amd.so')
+
+MFS_OK = 0
+MFS_ERR_INVALID_ARGUMENT = 1
+MFS_ERR_OUT_OF_RANGE = 2
+MFS_ERR_RUNTIME = 3
+
+MFS_BIT_PACKED = 1 << 0
+MFS_PREPEND_OBSERVABLES = 1 << 1
+MFS_APPEND_OBSERVABLES = 1 << 2
+MFS_SEPARATE_OBSERVABLES = 1 << 3
+MFS_DEVICE_OUTPUT = 1 << 4
+
+EXPORTED = [
+    'mfs_compile', 'mfs_destroy', 'mfs_circuit_stats', 'mfs_parse_stats', 'mfs_sample', 'mfs_sample_write',
+    'mfs_sample_with_faults', 'mfs_bench_device', 'mfs_set_tile_words', 'mfs_get_info', 'mfs_last_error',
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f'{LIB_PATH} is not built; run stim_amd/csrc/build.sh (or __graft_entry__.build())')
+    L = ctypes.CDLL(LIB_PATH)
+    u64, u32, sz, p = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_size_t, ctypes.c_void_p
+    L.mfs_compile.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.POINTER(p)]
+    L.mfs_compile.restype = ctypes.c_int
+    L.mfs_destroy.argtypes = [p]
+    L.mfs_destroy.restype = None
+    L.mfs_circuit_stats.argtypes = [p, ctypes.POINTER(u64)]
+    L.mfs_parse_stats.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(u64)]
+    L.mfs_sample.argtypes = [p, u64, u32, p, sz, p, sz]
+    L.mfs_sample_write.argtypes = [p, u64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                   ctypes.c_char_p, ctypes.c_char_p]
+    L.mfs_sample_with_faults.argtypes = [p, u64, p, p, u32, p, sz, p, sz]
+    L.mfs_bench_device.argtypes = [p, u64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.mfs_set_tile_words.argtypes = [p, ctypes.c_int]
+    L.mfs_get_info.argtypes = [p, ctypes.POINTER(u64)]
+    L.mfs_last_error.argtypes = []
+    L.mfs_last_error.restype = ctypes.c_char_p
+    for name in EXPORTED:
+        if name not in ('mfs_destroy', 'mfs_last_error', 'mfs_compile'):
+            getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def check(rc):
+    """Raises the Python exception the reference raises for the same failure."""
+    if rc == MFS_OK:
+        return
+    msg = lib().mfs_last_error().decode('utf-8', 'replace')
+    if rc == MFS_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if rc == MFS_ERR_OUT_OF_RANGE:
+        raise IndexError(msg)
+    raise RuntimeError(msg)

@@ -1,0 +1,663 @@
+// C-ABI entry points of the stim_amd sampler: compile, sample, sample_write, fixed-fault sampling.
+//
+// Mirrors the reference's CompiledDetectorSampler (src/stim/py/compiled_detector_sampler.pybind.{h,cc})
+// and its drivers (src/stim/simulators/frame_simulator_util.inl): the circuit is compiled once into a
+// device program; each call simulates batches of shot tiles on the GPU, writing shot-major rows
+// directly (no separate transpose pass), and copies them to the caller.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/stim_amd.h"
+#include "circuit.h"
+#include "compiler.h"
+#include "program.h"
+
+namespace sa {
+hipError_t launch_frame_kernel(int T, const LaunchParams &P, uint32_t num_tiles, size_t lds, hipStream_t stream);
+hipError_t launch_pack_rows(const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride, uint64_t rows,
+                            uint32_t ncols, hipStream_t stream);
+}  // namespace sa
+
+using namespace sa;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        throw HipError(std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    void ensure(size_t n) {
+        if (n <= cap) return;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+        check(hipMalloc(&p, std::max<size_t>(n, 256)), "hipMalloc");
+        cap = std::max<size_t>(n, 256);
+    }
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class X>
+    X *as() {
+        return reinterpret_cast<X *>(p);
+    }
+};
+
+uint64_t splitmix64(uint64_t x) {
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+constexpr size_t MAX_LDS = 160 * 1024;
+constexpr size_t HOST_BATCH_BYTES = (size_t)1 << 30;
+
+}  // namespace
+
+struct mfs_sampler {
+    std::string text;
+    Block block;
+    std::vector<const Instr *> flat;
+    Program prog;
+    uint64_t key = 0;
+    uint64_t tiles_used = 0;
+    int T = 2;
+    size_t lds = 0;
+    bool randomize = false;
+    hipStream_t stream = nullptr;
+    DevBuf d_ops, d_tgt, d_otab, d_faults, d_fault_off, d_out_det, d_out_obs, d_tmp;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ~mfs_sampler() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    uint32_t W() const { return 32u * (uint32_t)T; }
+
+    void pick_tile_words(int want) {
+        if (want) {
+            size_t need = lds_bytes(prog, want);
+            if (need > MAX_LDS) throw std::invalid_argument("tile does not fit in LDS for this circuit");
+            T = want;
+        } else if (lds_bytes(prog, 2) <= 64 * 1024) {
+            T = 2;
+        } else if (lds_bytes(prog, 1) <= MAX_LDS) {
+            T = 1;
+        } else {
+            throw std::invalid_argument(
+                "circuit's frame does not fit in LDS (" + std::to_string(prog.num_qubits) + " qubits, record ring " +
+                std::to_string(prog.ring) + " rows)");
+        }
+        lds = lds_bytes(prog, T);
+    }
+
+    LaunchParams base_params() {
+        LaunchParams P{};
+        P.ops = d_ops.as<DevOp>();
+        P.tgt = d_tgt.as<uint32_t>();
+        P.otab = d_otab.as<uint32_t>();
+        P.seed = key;
+        P.n_ops = (uint32_t)prog.ops.size();
+        P.num_qubits = prog.num_qubits;
+        P.ring = prog.ring;
+        P.num_obs = prog.num_observables;
+        return P;
+    }
+
+    void launch(const LaunchParams &P, uint64_t tiles) {
+        if (tiles >= (1ull << 31)) throw std::invalid_argument("too many shots in one launch");
+        check(launch_frame_kernel(T, P, (uint32_t)tiles, lds, stream), "frame kernel launch");
+    }
+
+    // Output layout of one sample call.
+    struct Layout {
+        uint32_t ncols;      // entries per dets row
+        size_t det_row;      // bytes per dets row
+        size_t obs_row;      // bytes per obs row (separate)
+        bool packed;
+        bool prepend, append, separate;
+        bool via_unpacked;   // packed with a non-byte-aligned dets/obs boundary
+    };
+
+    Layout layout(uint32_t flags) const {
+        Layout L{};
+        L.packed = flags & MFS_BIT_PACKED;
+        L.prepend = flags & MFS_PREPEND_OBSERVABLES;
+        L.append = flags & MFS_APPEND_OBSERVABLES;
+        L.separate = flags & MFS_SEPARATE_OBSERVABLES;
+        const uint32_t D = prog.num_detectors, O = prog.num_observables;
+        L.ncols = D + O * ((L.prepend ? 1 : 0) + (L.append ? 1 : 0));
+        L.det_row = L.packed ? (L.ncols + 7) / 8 : L.ncols;
+        L.obs_row = L.packed ? (O + 7) / 8 : O;
+        L.via_unpacked = L.packed && ((L.prepend && (O % 8)) || (L.append && (D % 8)));
+        return L;
+    }
+
+    // Fills output descriptors for a batch written into device rows.
+    void outputs(const Layout &L, uint8_t *det_dev, size_t det_stride, uint8_t *obs_dev, size_t obs_stride,
+                 LaunchParams &P, char base_char = 0) {
+        const uint32_t D = prog.num_detectors, O = prog.num_observables;
+        const bool packed = L.packed && !L.via_unpacked;
+        P.dets = DevOut{det_dev, det_stride, 0, packed ? 1u : 0u, (uint32_t)(uint8_t)base_char, 1};
+        if (L.prepend) P.dets.col0 = O;
+        if (L.prepend || L.append) {
+            // Observables share the dets row. A second combined row copy goes to obs when separate.
+            P.obs = DevOut{det_dev, det_stride, L.append ? D : 0u, packed ? 1u : 0u, (uint32_t)(uint8_t)base_char, 1};
+        } else if (L.separate && obs_dev) {
+            P.obs = DevOut{obs_dev, obs_stride, 0, L.packed ? 1u : 0u, (uint32_t)(uint8_t)base_char, 1};
+        } else {
+            P.obs = DevOut{nullptr, 0, 0, 0, 0, 0};
+        }
+    }
+
+    void upload_program() {
+        d_ops.ensure(prog.ops.size() * sizeof(DevOp));
+        d_tgt.ensure(std::max<size_t>(prog.tgt.size(), 1) * 4);
+        d_otab.ensure(std::max<size_t>(prog.otab.size(), 1) * 4);
+        check(hipMemcpy(d_ops.p, prog.ops.data(), prog.ops.size() * sizeof(DevOp), hipMemcpyHostToDevice), "upload ops");
+        if (!prog.tgt.empty())
+            check(hipMemcpy(d_tgt.p, prog.tgt.data(), prog.tgt.size() * 4, hipMemcpyHostToDevice), "upload targets");
+        if (!prog.otab.empty())
+            check(hipMemcpy(d_otab.p, prog.otab.data(), prog.otab.size() * 4, hipMemcpyHostToDevice), "upload table");
+    }
+
+    // A noiseless run with frame randomization: if every detector and observable stays zero over
+    // many random frames, randomization cannot change any output (the outputs are linear in the
+    // random bits), so sampling can skip it. Misses a nondeterministic output with probability 2^-512.
+    void probe_randomization() {
+        const uint64_t tiles = std::max<uint64_t>(1, 512 / W());
+        const uint64_t shots = tiles * W();
+        Layout L = layout(MFS_BIT_PACKED | MFS_SEPARATE_OBSERVABLES);
+        d_out_det.ensure(shots * std::max<size_t>(L.det_row, 1));
+        d_out_obs.ensure(shots * std::max<size_t>(L.obs_row, 1));
+        check(hipMemsetAsync(d_out_det.p, 0, d_out_det.cap, stream), "memset");
+        check(hipMemsetAsync(d_out_obs.p, 0, d_out_obs.cap, stream), "memset");
+        LaunchParams P = base_params();
+        outputs(L, d_out_det.as<uint8_t>(), L.det_row, d_out_obs.as<uint8_t>(), L.obs_row, P);
+        P.seed = splitmix64(key ^ 0x5EEDull);
+        P.shots = shots;
+        P.tile0 = 0;
+        P.mode = MODE_NOISELESS;
+        P.randomize = 1;
+        launch(P, tiles);
+        std::vector<uint8_t> a(shots * L.det_row), b(shots * L.obs_row);
+        if (!a.empty()) check(hipMemcpyAsync(a.data(), d_out_det.p, a.size(), hipMemcpyDeviceToHost, stream), "copy");
+        if (!b.empty()) check(hipMemcpyAsync(b.data(), d_out_obs.p, b.size(), hipMemcpyDeviceToHost, stream), "copy");
+        check(hipStreamSynchronize(stream), "probe");
+        randomize = false;
+        for (uint8_t v : a) randomize |= v != 0;
+        for (uint8_t v : b) randomize |= v != 0;
+    }
+
+    // Runs one sample call into host or device rows.
+    void sample(uint64_t shots, uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs, size_t obs_stride,
+                const uint64_t *fault_offsets = nullptr, const uint32_t *faults = nullptr, char base_char = 0,
+                bool newline = false) {
+        Layout L = layout(flags);
+        if (L.separate && (L.prepend || L.append)) {
+            throw std::invalid_argument(
+                "Can't specify separate_observables=True with append_observables=True or prepend_observables=True");
+        }
+        const bool device_out = flags & MFS_DEVICE_OUTPUT;
+        if (shots == 0) return;
+        const uint64_t W64 = W();
+        const size_t row_bytes = L.det_row + (L.separate ? L.obs_row : 0) + (L.via_unpacked ? L.ncols : 0);
+        uint64_t batch = device_out && !L.via_unpacked ? shots
+                                                        : std::max<uint64_t>(W64, (HOST_BATCH_BYTES / std::max<size_t>(row_bytes, 1)) / W64 * W64);
+        batch = std::min(batch, shots);
+        const bool fixed = faults != nullptr;
+        const uint64_t call_tile0 = tiles_used;
+        for (uint64_t s0 = 0; s0 < shots; s0 += batch) {
+            const uint64_t n = std::min(batch, shots - s0);
+            const uint64_t tiles = (n + W64 - 1) / W64;
+            LaunchParams P = base_params();
+            P.shots = n;
+            P.tile0 = call_tile0 + s0 / W64;
+            P.mode = fixed ? MODE_FIXED : MODE_SAMPLED;
+            P.randomize = (!fixed && randomize) ? 1 : 0;
+            if (fixed) stage_faults(s0, n, tiles, fault_offsets, faults, P);
+            uint8_t *det_dst, *obs_dst;
+            size_t det_stride_dev, obs_stride_dev;
+            if (L.via_unpacked) {
+                d_tmp.ensure(n * L.ncols);
+                Layout U = L;
+                U.packed = false;
+                U.via_unpacked = false;
+                outputs(U, d_tmp.as<uint8_t>(), L.ncols, nullptr, 0, P);
+                if (device_out) {
+                    det_dst = dets + s0 * dets_stride;
+                    det_stride_dev = dets_stride;
+                } else {
+                    d_out_det.ensure(n * L.det_row);
+                    det_dst = d_out_det.as<uint8_t>();
+                    det_stride_dev = L.det_row;
+                }
+                launch(P, tiles);
+                check(launch_pack_rows(d_tmp.as<uint8_t>(), L.ncols, det_dst, det_stride_dev, n, L.ncols, stream),
+                      "pack rows");
+                obs_dst = nullptr;
+                obs_stride_dev = 0;
+            } else {
+                if (device_out) {
+                    det_dst = dets + s0 * dets_stride;
+                    det_stride_dev = dets_stride;
+                    obs_dst = (L.separate && obs) ? obs + s0 * obs_stride : nullptr;
+                    obs_stride_dev = obs_stride;
+                } else {
+                    d_out_det.ensure(n * std::max<size_t>(L.det_row, 1));
+                    det_dst = d_out_det.as<uint8_t>();
+                    det_stride_dev = L.det_row;
+                    d_out_obs.ensure(n * std::max<size_t>(L.obs_row, 1));
+                    obs_dst = L.separate ? d_out_obs.as<uint8_t>() : nullptr;
+                    obs_stride_dev = L.obs_row;
+                }
+                if (L.packed && (L.det_row * 8 != L.ncols)) {
+                    // Padding bits of the last byte are written as zero by the flush of the final partial
+                    // block; nothing to do. Rows with zero columns are skipped below.
+                }
+                outputs(L, det_dst, det_stride_dev, obs_dst, obs_stride_dev, P, base_char);
+                if (L.ncols == 0 && !(L.separate && obs_dst)) {
+                    // Nothing to write; still advance the RNG position.
+                } else {
+                    launch(P, tiles);
+                }
+            }
+            if (!device_out) {
+                if (L.det_row && dets) {
+                    check(hipMemcpy2DAsync(dets + s0 * dets_stride, dets_stride, det_dst, det_stride_dev, L.det_row, n,
+                                           hipMemcpyDeviceToHost, stream),
+                          "copy dets");
+                }
+                if (L.separate && obs && L.obs_row && obs_dst) {
+                    check(hipMemcpy2DAsync(obs + s0 * obs_stride, obs_stride, obs_dst, obs_stride_dev, L.obs_row, n,
+                                           hipMemcpyDeviceToHost, stream),
+                          "copy obs");
+                }
+                check(hipStreamSynchronize(stream), "sample");
+                if (newline) {
+                    for (uint64_t i = 0; i < n; i++) dets[(s0 + i) * dets_stride + L.ncols] = '\n';
+                }
+            }
+        }
+        if (device_out) check(hipStreamSynchronize(stream), "sample");
+        tiles_used += (shots + W64 - 1) / W64;
+    }
+
+    void stage_faults(uint64_t s0, uint64_t n, uint64_t tiles, const uint64_t *off, const uint32_t *faults,
+                      LaunchParams &P) {
+        std::vector<DevFault> fs;
+        std::vector<uint32_t> fo(tiles + 1, 0);
+        const uint32_t W32 = W();
+        for (uint64_t t = 0; t < tiles; t++) {
+            fo[t] = (uint32_t)fs.size();
+            const size_t start = fs.size();
+            for (uint64_t s = s0 + t * W32; s < std::min(s0 + n, s0 + (t + 1) * W32); s++) {
+                for (uint64_t i = off[s]; i < off[s + 1]; i++) {
+                    uint32_t op, site;
+                    resolve_fault(prog, faults[3 * i], faults[3 * i + 1], &op, &site);
+                    fs.push_back(DevFault{op, site, (uint32_t)(s - s0 - t * W32), faults[3 * i + 2]});
+                }
+            }
+            std::stable_sort(fs.begin() + (long)start, fs.end(),
+                             [](const DevFault &a, const DevFault &b) { return a.op < b.op; });
+        }
+        fo[tiles] = (uint32_t)fs.size();
+        d_faults.ensure(std::max<size_t>(fs.size(), 1) * sizeof(DevFault));
+        d_fault_off.ensure(fo.size() * 4);
+        if (!fs.empty()) check(hipMemcpy(d_faults.p, fs.data(), fs.size() * sizeof(DevFault), hipMemcpyHostToDevice), "faults");
+        check(hipMemcpy(d_fault_off.p, fo.data(), fo.size() * 4, hipMemcpyHostToDevice), "fault offsets");
+        P.faults = d_faults.as<DevFault>();
+        P.fault_off = d_fault_off.as<uint32_t>();
+    }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Result-format writers (src/stim/io/measure_record_writer.cc:23-211, measure_record_writer.h:111-167).
+
+namespace {
+
+enum class Fmt { F01, B8, R8, HITS, DETS, PTB64 };
+
+Fmt parse_format(const char *f) {
+    std::string s = f ? f : "01";
+    if (s == "01") return Fmt::F01;
+    if (s == "b8") return Fmt::B8;
+    if (s == "r8") return Fmt::R8;
+    if (s == "hits") return Fmt::HITS;
+    if (s == "dets") return Fmt::DETS;
+    if (s == "ptb64") return Fmt::PTB64;
+    throw std::invalid_argument("Unrecognized sample format: '" + s + "'. Expected one of: 01, b8, r8, ptb64, hits, dets");
+}
+
+struct FileCloser {
+    void operator()(FILE *f) const {
+        if (f) fclose(f);
+    }
+};
+
+// Writes packed shot-major rows (ncols bits each) in a per-shot text or byte format.
+// prefix1 applies to columns < transition, prefix2 to the rest (dets format only).
+void write_rows(FILE *out, Fmt fmt, const uint8_t *rows, size_t stride, uint64_t nshots, uint32_t ncols, char prefix1,
+                char prefix2, uint32_t transition) {
+    if (transition == 0) {
+        transition = ncols;
+        prefix1 = prefix2;
+    } else if (prefix1 == prefix2 || transition >= ncols) {
+        transition = ncols;
+    }
+    std::string line;
+    for (uint64_t s = 0; s < nshots; s++) {
+        const uint8_t *r = rows + s * stride;
+        auto bit = [&](uint32_t m) { return (r[m >> 3] >> (m & 7)) & 1; };
+        switch (fmt) {
+            case Fmt::B8:
+                fwrite(r, 1, (ncols + 7) / 8, out);
+                break;
+            case Fmt::F01:
+                line.assign(ncols + 1, '\n');
+                for (uint32_t m = 0; m < ncols; m++) line[m] = (char)('0' + bit(m));
+                fwrite(line.data(), 1, line.size(), out);
+                break;
+            case Fmt::HITS: {
+                bool first = true;
+                for (uint32_t m = 0; m < ncols; m++) {
+                    if ((m & 7) == 0 && m + 8 <= ncols && r[m >> 3] == 0) {
+                        m += 7;
+                        continue;
+                    }
+                    if (bit(m)) {
+                        if (!first) putc(',', out);
+                        first = false;
+                        fprintf(out, "%llu", (unsigned long long)m);
+                    }
+                }
+                putc('\n', out);
+                break;
+            }
+            case Fmt::DETS: {
+                fputs("shot", out);
+                for (uint32_t m = 0; m < ncols; m++) {
+                    if ((m & 7) == 0 && m + 8 <= ncols && r[m >> 3] == 0) {
+                        m += 7;
+                        continue;
+                    }
+                    if (bit(m)) {
+                        if (m < transition) fprintf(out, " %c%u", prefix1, m);
+                        else fprintf(out, " %c%u", prefix2, m - transition);
+                    }
+                }
+                putc('\n', out);
+                break;
+            }
+            case Fmt::R8: {
+                uint32_t run = 0;
+                for (uint32_t m = 0; m < ncols; m++) {
+                    if (bit(m)) {
+                        putc((int)run, out);
+                        run = 0;
+                    } else {
+                        run++;
+                        if (run == 255) {
+                            putc(255, out);
+                            run = 0;
+                        }
+                    }
+                }
+                putc((int)run, out);
+                break;
+            }
+            case Fmt::PTB64:
+                break;
+        }
+    }
+}
+
+// ptb64: for each group of 64 shots, for each column, 8 bytes holding that column's bit for the 64 shots.
+void write_ptb64(FILE *out, const uint8_t *rows, size_t stride, uint64_t nshots, uint32_t ncols) {
+    for (uint64_t g = 0; g < nshots / 64; g++) {
+        for (uint32_t m = 0; m < ncols; m++) {
+            uint64_t v = 0;
+            for (uint32_t k = 0; k < 64; k++) {
+                const uint8_t *r = rows + (g * 64 + k) * stride;
+                v |= (uint64_t)((r[m >> 3] >> (m & 7)) & 1) << k;
+            }
+            uint8_t b[8];
+            for (int i = 0; i < 8; i++) b[i] = (uint8_t)(v >> (8 * i));
+            fwrite(b, 1, 8, out);
+        }
+    }
+}
+
+int fail(const std::exception &e, int kind) {
+    g_last_error = e.what();
+    return kind;
+}
+
+template <class F>
+int guarded(F f) {
+    try {
+        f();
+        return MFS_OK;
+    } catch (const std::out_of_range &e) {
+        return fail(e, MFS_ERR_OUT_OF_RANGE);
+    } catch (const std::invalid_argument &e) {
+        return fail(e, MFS_ERR_INVALID_ARGUMENT);
+    } catch (const std::exception &e) {
+        return fail(e, MFS_ERR_RUNTIME);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mfs_last_error(void) {
+    return g_last_error.c_str();
+}
+
+int mfs_compile(const char *stim_text, size_t len, uint64_t seed, int has_seed, mfs_sampler **out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out is null");
+        *out = nullptr;
+        auto s = std::make_unique<mfs_sampler>();
+        s->text.assign(stim_text ? stim_text : "", stim_text ? len : 0);
+        s->block = parse_circuit(s->text.data(), s->text.size());
+        s->flat = flatten(s->block);
+        s->prog = compile_program(s->flat);
+        if (has_seed) {
+            s->key = splitmix64(seed ^ 0xDEADBEEF124Cull);
+        } else {
+            std::random_device rd;
+            s->key = ((uint64_t)rd() << 32) ^ rd();
+        }
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            throw HipError("no HIP device available: the stim_amd sampler runs only on the GPU");
+        }
+        check(hipStreamCreate(&s->stream), "stream");
+        s->pick_tile_words(0);
+        s->upload_program();
+        s->probe_randomization();
+        *out = s.release();
+    });
+}
+
+void mfs_destroy(mfs_sampler *s) {
+    delete s;
+}
+
+int mfs_parse_stats(const char *stim_text, size_t len, uint64_t out[10]) {
+    return guarded([&] {
+        std::string text(stim_text ? stim_text : "", stim_text ? len : 0);
+        Block b = parse_circuit(text.data(), text.size());
+        std::vector<const Instr *> flat = flatten(b);
+        Program prog = compile_program(flat);
+        out[0] = prog.stats.num_qubits;
+        out[1] = prog.stats.num_measurements;
+        out[2] = prog.stats.num_detectors;
+        out[3] = prog.stats.num_observables;
+        out[4] = prog.stats.max_lookback;
+        out[5] = prog.ops.size();
+        out[6] = prog.num_syncs;
+        out[7] = prog.num_noise_ops;
+        out[8] = prog.ring;
+        out[9] = lds_bytes(prog, 2);
+    });
+}
+
+int mfs_circuit_stats(const mfs_sampler *s, uint64_t out[5]) {
+    return guarded([&] {
+        const Stats &st = s->prog.stats;
+        out[0] = st.num_qubits;
+        out[1] = st.num_measurements;
+        out[2] = st.num_detectors;
+        out[3] = st.num_observables;
+        out[4] = st.max_lookback;
+    });
+}
+
+int mfs_sample(mfs_sampler *s, uint64_t shots, uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs,
+               size_t obs_stride) {
+    return guarded([&] { s->sample(shots, flags, dets, dets_stride, obs, obs_stride); });
+}
+
+int mfs_sample_with_faults(mfs_sampler *s, uint64_t shots, const uint64_t *shot_offsets, const uint32_t *faults,
+                           uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs, size_t obs_stride) {
+    return guarded([&] {
+        if (!shot_offsets) throw std::invalid_argument("shot_offsets is null");
+        static const uint32_t none[3] = {0, 0, 0};
+        s->sample(shots, flags, dets, dets_stride, obs, obs_stride, shot_offsets, faults ? faults : none);
+    });
+}
+
+int mfs_sample_write(mfs_sampler *s, uint64_t shots, const char *path, const char *format, int prepend_observables,
+                     int append_observables, const char *obs_path, const char *obs_format) {
+    return guarded([&] {
+        const Fmt fmt = parse_format(format);
+        const bool has_obs = obs_path && obs_path[0];
+        const Fmt ofmt = has_obs ? parse_format(obs_format) : Fmt::F01;
+        if ((prepend_observables ? 1 : 0) + (append_observables ? 1 : 0) + (has_obs ? 1 : 0) > 1) {
+            throw std::out_of_range("Can't combine --prepend_observables, --append_observables, or --obs_out");
+        }
+        if ((fmt == Fmt::PTB64 || (has_obs && ofmt == Fmt::PTB64)) && shots % 64) {
+            throw std::invalid_argument("shots must be a multiple of 64 to use ptb64 format.");
+        }
+        std::unique_ptr<FILE, FileCloser> out(fopen(path, "wb"));
+        if (!out) throw std::invalid_argument(std::string("Failed to open '") + path + "' to write.");
+        std::unique_ptr<FILE, FileCloser> oout;
+        if (has_obs) {
+            oout.reset(fopen(obs_path, "wb"));
+            if (!oout) throw std::invalid_argument(std::string("Failed to open '") + obs_path + "' to write.");
+        }
+        const uint32_t D = s->prog.num_detectors, O = s->prog.num_observables;
+        uint32_t flags = MFS_BIT_PACKED | (prepend_observables ? MFS_PREPEND_OBSERVABLES : 0) |
+                         (append_observables ? MFS_APPEND_OBSERVABLES : 0) | (has_obs ? MFS_SEPARATE_OBSERVABLES : 0);
+        const uint32_t ncols = D + O * ((prepend_observables ? 1 : 0) + (append_observables ? 1 : 0));
+        const size_t drow = (ncols + 7) / 8, orow = (O + 7) / 8;
+        const uint64_t chunk = std::max<uint64_t>(64, ((64ull << 20) / std::max<size_t>(drow + orow, 1)) / 64 * 64);
+        std::vector<uint8_t> dbuf, obuf;
+        char p1 = 'D', p2 = 'L';
+        uint32_t transition = D;
+        if (prepend_observables) {
+            p1 = 'L';
+            p2 = 'D';
+            transition = O;
+        }
+        for (uint64_t s0 = 0; s0 < shots; s0 += chunk) {
+            const uint64_t n = std::min(chunk, shots - s0);
+            dbuf.assign(n * std::max<size_t>(drow, 1), 0);
+            obuf.assign(n * std::max<size_t>(orow, 1), 0);
+            s->sample(n, flags, dbuf.data(), std::max<size_t>(drow, 1), obuf.data(), std::max<size_t>(orow, 1));
+            if (fmt == Fmt::PTB64) write_ptb64(out.get(), dbuf.data(), std::max<size_t>(drow, 1), n, ncols);
+            else write_rows(out.get(), fmt, dbuf.data(), std::max<size_t>(drow, 1), n, ncols, p1, p2, transition);
+            if (has_obs) {
+                if (ofmt == Fmt::PTB64) write_ptb64(oout.get(), obuf.data(), std::max<size_t>(orow, 1), n, O);
+                else write_rows(oout.get(), ofmt, obuf.data(), std::max<size_t>(orow, 1), n, O, 'L', 'L', O);
+            }
+        }
+    });
+}
+
+int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, double out_ms[2]) {
+    return guarded([&] {
+        mfs_sampler::Layout L = s->layout(MFS_BIT_PACKED | MFS_SEPARATE_OBSERVABLES);
+        DevBuf dd, od;
+        dd.ensure(shots * std::max<size_t>(L.det_row, 1));
+        od.ensure(shots * std::max<size_t>(L.obs_row, 1));
+        if (!s->ev0) {
+            check(hipEventCreate(&s->ev0), "event");
+            check(hipEventCreate(&s->ev1), "event");
+        }
+        const uint64_t tiles = (shots + s->W() - 1) / s->W();
+        auto one = [&]() {
+            LaunchParams P = s->base_params();
+            s->outputs(L, dd.as<uint8_t>(), L.det_row, od.as<uint8_t>(), L.obs_row, P);
+            P.shots = shots;
+            P.tile0 = s->tiles_used;
+            P.mode = MODE_SAMPLED;
+            P.randomize = s->randomize ? 1 : 0;
+            s->launch(P, tiles);
+            s->tiles_used += tiles;
+        };
+        for (int i = 0; i < warmup; i++) one();
+        check(hipStreamSynchronize(s->stream), "warmup");
+        double kernel_total = 0;
+        for (int i = 0; i < iters; i++) {
+            check(hipEventRecord(s->ev0, s->stream), "event");
+            one();
+            check(hipEventRecord(s->ev1, s->stream), "event");
+            check(hipEventSynchronize(s->ev1), "event");
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
+            kernel_total += ms;
+        }
+        out_ms[0] = iters ? kernel_total / iters : 0;
+        out_ms[1] = out_ms[0];
+    });
+}
+
+int mfs_set_tile_words(mfs_sampler *s, int words) {
+    int rc = guarded([&] {
+        if (words != 0 && words != 1 && words != 2 && words != 4) throw std::invalid_argument("tile words must be 0, 1, 2 or 4");
+        s->pick_tile_words(words);
+    });
+    if (rc) return -rc;
+    return s->T;
+}
+
+int mfs_get_info(const mfs_sampler *s, uint64_t out[8]) {
+    return guarded([&] {
+        out[0] = (uint64_t)s->T;
+        out[1] = s->lds;
+        out[2] = s->prog.ops.size();
+        out[3] = s->prog.num_syncs;
+        out[4] = s->prog.num_noise_ops;
+        out[5] = s->randomize ? 1 : 0;
+        out[6] = s->prog.ring;
+        out[7] = s->prog.num_qubits;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,286 @@
+"""GPU parity: the HIP frame kernel (through the C ABI) against the reference's own outputs and the oracle.
+
+Fixed-fault mode must be bit-exact; seeded mode must pass binomial tests against exact rates.
+"""
+
+import os
+import random
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import frame_oracle as fo
+from oracle import stim_text as st
+import zoo
+
+pytestmark = pytest.mark.gpu
+
+
+def _sampler(text, seed=1):
+    import stim_amd
+    return stim_amd.Circuit(text).compile_detector_sampler(seed=seed)
+
+
+def _rows(dets, obs):
+    return [''.join('1' if b else '0' for b in np.concatenate([dets[i], obs[i]])) for i in range(len(dets))]
+
+
+# ---------------------------------------------------------------- fixed-fault parity (bit-exact)
+
+def test_fixed_faults_match_reference_vectors(golden):
+    fixtures = golden.load('fault_vectors.json')
+    for f in fixtures:
+        print('circuit', f['name'], flush=True)
+        s = _sampler(f['circuit'])
+        faults = [[tuple(x) for x in st_['faults']] for st_ in f['sets']]
+        dets, obs = s.sample_with_faults(faults, separate_observables=True)
+        got = _rows(dets, obs)
+        exp = [st_['expected'] for st_ in f['sets']]
+        bad = [i for i in range(len(exp)) if got[i] != exp[i]]
+        assert not bad, (f['name'], bad[:5], [f['sets'][i]['faults'] for i in bad[:3]])
+
+
+def _random_fault_sets(flat, n_shots, rng, max_faults=6):
+    chans = fo.noise_sites(flat, allow_chains=True)
+    out = []
+    for _ in range(n_shots):
+        k = rng.randint(0, max_faults)
+        picks = rng.sample(chans, k=min(k, len(chans)))
+        out.append([(a, b, rng.choice(outs)[0]) for (a, b, outs) in picks])
+    return out
+
+
+@pytest.mark.parametrize('seed', range(6))
+def test_fixed_faults_match_oracle_on_gate_zoo(seed):
+    text = zoo.gate_zoo(100 + seed, n_qubits=12 + 4 * seed, layers=14)
+    flat = st.parse_flat(text)
+    rng = random.Random(seed)
+    faults = _random_fault_sets(flat, 300, rng)
+    print('zoo seed', seed, flush=True)
+    s = _sampler(text)
+    dets, obs = s.sample_with_faults(faults, separate_observables=True)
+    edets, eobs = fo.FrameOracle(flat).run(len(faults), faults=faults)
+    np.testing.assert_array_equal(dets, edets)
+    np.testing.assert_array_equal(obs, eobs)
+
+
+@pytest.mark.parametrize('name', ['surface_d5_r5', 'color_d5_r3', 'surface_x_d5_r3'])
+def test_fixed_faults_bit_packed_match_oracle(golden, name):
+    text = golden.circuit(name)
+    flat = st.parse_flat(text)
+    faults = _random_fault_sets(flat, 700, random.Random(11), max_faults=12)
+    s = _sampler(text)
+    dets, obs = s.sample_with_faults(faults, separate_observables=True, bit_packed=True)
+    edets, eobs = fo.FrameOracle(flat).run(len(faults), faults=faults)
+    np.testing.assert_array_equal(dets, fo.pack_bits(edets))
+    np.testing.assert_array_equal(obs, fo.pack_bits(eobs))
+
+
+def test_fixed_faults_d25_full_size_match_oracle(golden):
+    text = golden.circuit('surface_d25_r25')
+    flat = st.parse_flat(text)
+    faults = _random_fault_sets(flat, 64, random.Random(5), max_faults=40)
+    s = _sampler(text)
+    dets, obs = s.sample_with_faults(faults, separate_observables=True, bit_packed=True)
+    edets, eobs = fo.FrameOracle(flat).run(len(faults), faults=faults)
+    np.testing.assert_array_equal(dets, fo.pack_bits(edets))
+    np.testing.assert_array_equal(obs, fo.pack_bits(eobs))
+
+
+def test_xor_linearity_of_fault_injection(golden):
+    """Detection events of a fault set equal the XOR of its single faults' events."""
+    text = golden.circuit('surface_d5_r5')
+    flat = st.parse_flat(text)
+    rng = random.Random(3)
+    sets = _random_fault_sets(flat, 50, rng, max_faults=5)
+    singles = [[f] for fs in sets for f in fs]
+    s = _sampler(text)
+    d_sets = s.sample_with_faults(sets, bit_packed=True, append_observables=True)
+    d_single = s.sample_with_faults(singles, bit_packed=True, append_observables=True)
+    pos = 0
+    for i, fs in enumerate(sets):
+        acc = np.zeros(d_sets.shape[1], dtype=np.uint8)
+        for _ in fs:
+            acc ^= d_single[pos]
+            pos += 1
+        np.testing.assert_array_equal(d_sets[i], acc)
+
+
+def test_noiseless_circuits_give_zero_detectors(golden):
+    for name in ['surface_d5_r5', 'surface_d25_r25', 'color_d5_r3', 'rep_d7_r5']:
+        text = golden.circuit(name)
+        for p in ['0.001', '0.002']:
+            text = text.replace(f'({p})', '(0)')
+        s = _sampler(text)
+        d, o = s.sample(5000, bit_packed=True, separate_observables=True)
+        assert not d.any() and not o.any(), name
+
+
+# ---------------------------------------------------------------- reference API vectors
+
+KAT = """
+X_ERROR(1) 0
+M 0 1 2
+DETECTOR rec[-3] rec[-2]
+DETECTOR rec[-3] rec[-1]
+DETECTOR rec[-1] rec[-2]
+OBSERVABLE_INCLUDE(0) rec[-3]
+OBSERVABLE_INCLUDE(3) rec[-2] rec[-1]
+OBSERVABLE_INCLUDE(0) rec[-2]
+"""
+
+
+def test_reference_sample_vectors():
+    """Vectors from the reference's compiled_detector_sampler_pybind_test.py:37-160."""
+    s = _sampler(KAT)
+    np.testing.assert_array_equal(s.sample(5), np.array([[1, 1, 0]] * 5, dtype=np.bool_))
+    np.testing.assert_array_equal(s.sample(5, prepend_observables=True),
+                                  np.array([[1, 0, 0, 0, 1, 1, 0]] * 5, dtype=np.bool_))
+    np.testing.assert_array_equal(s.sample(5, append_observables=True),
+                                  np.array([[1, 1, 0, 1, 0, 0, 0]] * 5, dtype=np.bool_))
+    dets, obs = s.sample(5, separate_observables=True)
+    np.testing.assert_array_equal(dets, np.array([[1, 1, 0]] * 5, dtype=np.bool_))
+    np.testing.assert_array_equal(obs, np.array([[1, 0, 0, 0]] * 5, dtype=np.bool_))
+    np.testing.assert_array_equal(s.sample(5, bit_packed=True), np.array([[0b011]] * 5, dtype=np.uint8))
+    np.testing.assert_array_equal(s.sample(5, prepend_observables=True, bit_packed=True),
+                                  np.array([[0b0110001]] * 5, dtype=np.uint8))
+    np.testing.assert_array_equal(s.sample(5, append_observables=True, bit_packed=True),
+                                  np.array([[0b0001011]] * 5, dtype=np.uint8))
+    dets, obs = s.sample(5, separate_observables=True, bit_packed=True)
+    np.testing.assert_array_equal(dets, np.array([[0b011]] * 5, dtype=np.uint8))
+    np.testing.assert_array_equal(obs, np.array([[0b0001]] * 5, dtype=np.uint8))
+    t = _sampler('X_ERROR(1) 0\nM 0\nDETECTOR rec[-1]\n')
+    np.testing.assert_array_equal(t.sample(shots=2), np.array([[1], [1]], dtype=np.bool_))
+
+
+def test_sample_argument_errors():
+    s = _sampler(KAT)
+    with pytest.raises(ValueError):
+        s.sample(3, separate_observables=True, append_observables=True)
+    with pytest.raises(ValueError):
+        s.sample(3, dets_out=np.zeros((3, 3), dtype=np.uint8))
+    with pytest.raises(ValueError):
+        s.sample(3, bit_packed=True, dets_out=np.zeros((3, 2), dtype=np.uint8))
+    with pytest.raises(ValueError):
+        s.sample(3, bit_packed=True, dets_out=np.zeros((3,), dtype=np.uint8))
+
+
+def test_manual_output_buffers():
+    s = _sampler(KAT)
+    dets = np.zeros((11, 1), dtype=np.uint8)
+    obs = np.zeros((11, 1), dtype=np.uint8)
+    d2, o2 = s.sample(11, separate_observables=True, bit_packed=True, dets_out=dets, obs_out=obs)
+    assert d2 is dets and o2 is obs
+    assert (dets == 0b011).all() and (obs == 0b0001).all()
+    dets = np.zeros((11, 3), dtype=np.bool_)
+    s.sample(11, dets_out=dets)
+    assert dets[:, :2].all() and not dets[:, 2].any()
+
+
+def test_sample_write_formats_match_reference_bytes(golden):
+    fx = golden.load('formats.json')
+    s = _sampler(fx['circuit'])
+    with tempfile.TemporaryDirectory() as d:
+        for key, hexdata in sorted(fx['outputs'].items()):
+            fmt, mode, shots = key.split('/')
+            shots = int(shots)
+            path = os.path.join(d, 'out')
+            obs_path = os.path.join(d, 'obs')
+            if mode in ('plain', 'append', 'prepend'):
+                s.sample_write(shots, filepath=path, format=fmt, prepend_observables=mode == 'prepend',
+                               append_observables=mode == 'append')
+                got = open(path, 'rb').read()
+            else:
+                s.sample_write(shots, filepath=path, format=fmt, obs_out_filepath=obs_path, obs_out_format=fmt)
+                got = open(path if mode == 'obs_out_dets' else obs_path, 'rb').read()
+            assert got.hex() == hexdata, key
+
+
+def test_sample_write_conflicting_observable_flags():
+    s = _sampler(KAT)
+    with tempfile.TemporaryDirectory() as d:
+        with pytest.raises(IndexError):
+            s.sample_write(3, filepath=os.path.join(d, 'a'), format='b8', prepend_observables=True,
+                           append_observables=True)
+        with pytest.raises(ValueError):
+            s.sample_write(3, filepath=os.path.join(d, 'a'), format='nope')
+
+
+# ---------------------------------------------------------------- seeded statistics
+
+def _z(counts, shots, p):
+    p = np.clip(np.asarray(p, dtype=float), 1e-300, 1)
+    return (np.asarray(counts, dtype=float) - shots * p) / np.sqrt(shots * p * (1 - p))
+
+
+def _zcrit(n, alpha=1e-4):
+    from scipy.stats import norm
+    return abs(norm.ppf(alpha / (2 * n)))
+
+
+@pytest.mark.parametrize('name', ['surface_d5_r5', 'surface_d3_r3'])
+def test_seeded_rates_match_exact_rates(golden, name):
+    """Per-detector firing rates pass a Bonferroni binomial test (alpha=1e-4) against exact rates."""
+    rates = np.array(golden.load('dem_rates.json')[name])
+    s = _sampler(golden.circuit(name), seed=12345)
+    shots = 1 << 21
+    d = s.sample(shots, bit_packed=True, append_observables=True)
+    n = len(rates)
+    bits = np.unpackbits(d, axis=1, bitorder='little')[:, :n]
+    counts = bits.sum(axis=0)
+    z = _z(counts, shots, rates)
+    assert np.max(np.abs(z)) < _zcrit(n), np.argsort(-np.abs(z))[:5]
+    # Global chi-square over all detectors.
+    from scipy.stats import chi2
+    stat = float((z ** 2).sum())
+    assert stat < chi2.ppf(1 - 1e-4, df=n)
+
+
+def test_seeded_pair_correlations_match_exact(golden):
+    """Per-mechanism check: co-firing rate of detector pairs joined by single faults."""
+    name = 'surface_d3_r3'
+    text = golden.circuit(name).replace('0.001', '0.01')
+    flat = st.parse_flat(text)
+    chans = fo.noise_sites(flat)
+    orc = fo.FrameOracle(flat)
+    items = [(k, j, c, p) for (k, j, outs) in chans for (c, p) in outs]
+    dets, obs = orc.run(len(items), faults=[[(k, j, c)] for (k, j, c, _) in items])
+    sym = np.concatenate([dets, obs], axis=1)
+    # Exact P(d1 and d2 both fire) by enumerating channels is expensive; instead compare the XOR
+    # rate of pairs: P(d1 ^ d2) is an exact single-bit parity rate computable like a detector rate.
+    n = sym.shape[1]
+    pairs = [(a, b) for a in range(n) for b in range(a + 1, n) if (sym[:, a] & sym[:, b]).any()]
+    assert pairs
+    logs = np.zeros(len(pairs))
+    pos = 0
+    for (k, j, outs) in chans:
+        q = np.zeros(len(pairs))
+        for (c, p) in outs:
+            row = sym[pos]
+            q += p * np.array([row[a] ^ row[b] for a, b in pairs])
+            pos += 1
+        logs += np.log(np.abs(1 - 2 * q))
+    exact = (1 - np.exp(logs)) / 2
+    s = _sampler(text, seed=99)
+    shots = 1 << 20
+    d = s.sample(shots, append_observables=True)
+    counts = np.array([(d[:, a] ^ d[:, b]).sum() for a, b in pairs])
+    z = _z(counts, shots, exact)
+    assert np.max(np.abs(z)) < _zcrit(len(pairs))
+
+
+def test_d25_full_size_rates_and_shape(golden):
+    text = golden.circuit('surface_d25_r25')
+    s = _sampler(text, seed=2)
+    shots = 1 << 20
+    dets, obs = s.sample(shots, bit_packed=True, separate_observables=True)
+    assert dets.shape == (shots, 1950) and obs.shape == (shots, 1)
+    bits = np.unpackbits(dets[:1 << 16], axis=1, bitorder='little')[:, :15600]
+    rate = bits.mean()
+    # Mean detector firing rate of the d=25 circuit at p=1e-3 lies near 1.7% (exact mean from the
+    # reference's error model is checked in the statistics tests at smaller distance).
+    assert 0.005 < rate < 0.05
+    # Different calls draw different randomness; same seed reproduces.
+    again = _sampler(text, seed=2).sample(1024, bit_packed=True, separate_observables=True)[0]
+    np.testing.assert_array_equal(again, dets[:1024])
