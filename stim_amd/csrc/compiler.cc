@@ -9,6 +9,13 @@
 // PAULI_CHANNEL_1/2 outcomes follow the disjoint order of perform_pauli_errors_via_correlated_errors
 // (src/stim/simulators/tableau_simulator.h:291-324); DEPOLARIZE1/2 codes follow do_DEPOLARIZE1/2
 // (frame_simulator.inl:632-663).
+//
+// Layout rules the kernel relies on:
+//   * no two items of one frame op touch the same qubit (instructions are cut into segments);
+//   * record slots written by one op are contiguous (ops are cut at the ring wrap);
+//   * a frame op's words fit one LDS staging buffer (ops are capped at STAGE_WORDS words) and an
+//     epoch (ops between barriers) fits one staging buffer;
+//   * CSR headers hold offsets relative to the op's first word.
 #include "compiler.h"
 
 #include <algorithm>
@@ -17,7 +24,6 @@
 #include <functional>
 #include <stdexcept>
 #include <string>
-#include <unordered_map>
 
 namespace sa {
 
@@ -39,20 +45,17 @@ enum Kind : char { KR = 'r', KW = 'w', KA = 'a' };
 
 struct Builder {
     Program P;
+    std::vector<uint32_t> op_tlen;  // staged words per op (0 for noise / wordless ops)
     std::vector<uint32_t> qmap;
     uint64_t mcount = 0;
-    uint32_t dcount = 0;
     uint32_t noise_count = 0;
     uint32_t epoch = 1;
     Tracker tq, tr, tm;  // qubits, record slots, misc (staging rows 0..255, obs 256, chain 257)
     std::vector<std::pair<uint32_t, char>> aq, ar, am;
-    // segment splitting
     std::vector<uint32_t> seg_mark;
     uint32_t seg_id = 1;
-    // pending detectors
     std::vector<std::vector<uint32_t>> det_terms;
     uint32_t det_base = 0;
-    // pending observable items
     std::vector<std::vector<uint32_t>> obs_items;
 
     static constexpr uint32_t MISC_OBS = STAGE_ROWS;
@@ -93,11 +96,17 @@ struct Builder {
             else t.A[a.first] = epoch;
         }
     }
-    void push(DevOp op) {
-        if (!P.ops.empty() && (conflict(tq, aq) || conflict(tr, ar) || conflict(tm, am))) {
+    void barrier_before_next() {
+        if (!P.ops.empty() && !(P.ops.back().code & OPF_SYNC)) {
             P.ops.back().code |= OPF_SYNC;
             P.num_syncs++;
-            epoch++;
+        }
+        epoch++;
+    }
+    // Appends an op. Frame ops must have appended their words to P.tgt since make().
+    void push(DevOp op, bool frame_words = true) {
+        if (!P.ops.empty() && (conflict(tq, aq) || conflict(tr, ar) || conflict(tm, am))) {
+            barrier_before_next();
         }
         mark(tq, aq);
         mark(tr, ar);
@@ -106,6 +115,7 @@ struct Builder {
         ar.clear();
         am.clear();
         P.ops.push_back(op);
+        op_tlen.push_back(frame_words ? (uint32_t)(P.tgt.size() - op.toff) : 0u);
     }
     DevOp make(uint32_t code, uint32_t n) {
         DevOp op{};
@@ -116,7 +126,6 @@ struct Builder {
     }
 
     // ------------------------------------------------------------------ segments
-    // Greedily cuts item lists so no qubit appears twice in one op.
     bool seg_conflict(std::initializer_list<uint32_t> qs) {
         for (uint32_t x : qs) {
             if (seg_mark[x] == seg_id) return true;
@@ -134,7 +143,7 @@ struct Builder {
         while (i < qs.size()) {
             seg_reset();
             DevOp op = make(code | flags, 0);
-            while (i < qs.size() && !seg_conflict({qs[i]})) {
+            while (i < qs.size() && op.n < STAGE_WORDS && !seg_conflict({qs[i]})) {
                 seg_add({qs[i]});
                 P.tgt.push_back(qs[i]);
                 touch_q(qs[i], KW);
@@ -149,9 +158,8 @@ struct Builder {
         size_t i = 0;
         while (i < ps.size()) {
             seg_reset();
-            if (P.tgt.size() & 1) P.tgt.push_back(0);  // pairs are read as 8-byte aligned words
             DevOp op = make(code, 0);
-            while (i < ps.size() && !seg_conflict({ps[i].first, ps[i].second})) {
+            while (i < ps.size() && 2 * op.n + 2 <= STAGE_WORDS && !seg_conflict({ps[i].first, ps[i].second})) {
                 seg_add({ps[i].first, ps[i].second});
                 P.tgt.push_back(ps[i].first);
                 P.tgt.push_back(ps[i].second);
@@ -170,7 +178,7 @@ struct Builder {
         while (i < items.size()) {
             seg_reset();
             DevOp op = make(OP_FEEDBACK, 0);
-            while (i < items.size() && !seg_conflict({items[i][1]})) {
+            while (i < items.size() && 3 * op.n + 3 <= STAGE_WORDS && !seg_conflict({items[i][1]})) {
                 seg_add({items[i][1]});
                 for (int k = 0; k < 3; k++) P.tgt.push_back(items[i][k]);
                 touch_q(items[i][1], KW);
@@ -182,16 +190,16 @@ struct Builder {
         }
     }
 
-    // Noise op builder; returns op index.
-    uint32_t emit_noise(uint32_t code, uint32_t n_sites, double p, const std::vector<std::pair<double, uint32_t>> &outs,
-                        uint32_t flat_k, uint32_t first_site, uint32_t flags = 0) {
+    // Noise op builder; its site words must already be in P.ntgt starting at ntoff. Returns op index.
+    uint32_t emit_noise(uint32_t code, uint32_t n_sites, uint32_t ntoff, double p,
+                        const std::vector<std::pair<double, uint32_t>> &outs, uint32_t flat_k, uint32_t first_site,
+                        uint32_t flags = 0) {
         DevOp op{};
         op.code = code | flags;
         op.n = n_sites;
-        op.toff = (uint32_t)P.tgt.size();
+        op.toff = ntoff;
         op.p = (float)p;
         op.inv_log = (p > 0 && p < 1) ? (float)(1.0 / std::log2(1.0 - p)) : 0.0f;
-        op.fid = noise_count++;
         // Outcome table: cumulative thresholds over uint32 in proportion to outcome weights.
         op.woff = (uint32_t)P.otab.size();
         double tot = 0;
@@ -220,36 +228,73 @@ struct Builder {
         }
         if (p > 0.3) op.code |= OPF_DENSE;
         op.nout = nout | (chunk_log2 << 16);
+        noise_count++;
         if (flat_k != NONE) {
             P.sites[flat_k].push_back({(uint32_t)P.ops.size(), first_site, n_sites});
         }
-        push(op);
+        push(op, false);
         return (uint32_t)P.ops.size() - 1;
+    }
+
+    // ------------------------------------------------------------------ CSR ops
+    // Emits ops of CSR items (header of n+1 offsets relative to the op's first word, then terms),
+    // cut so each op's words fit one staging buffer. item_words[i] lists item i's words;
+    // touch(i) records its accesses. 'prologue' runs before each op (e.g. record base).
+    template <class TouchF, class OpF>
+    void emit_csr(uint32_t code, const std::vector<std::vector<uint32_t>> &items, TouchF touch, OpF configure,
+                  const std::function<bool(size_t, size_t)> &can_join = nullptr) {
+        size_t i = 0;
+        while (i < items.size()) {
+            seg_reset();
+            size_t e = i;
+            size_t words = 1;
+            while (e < items.size()) {
+                size_t add = 1 + items[e].size();
+                if (words + add > STAGE_WORDS) break;
+                if (can_join && !can_join(i, e)) break;
+                words += add;
+                e++;
+            }
+            if (e == i) {
+                throw std::invalid_argument("a detector, observable or Pauli product has too many terms for the GPU "
+                                            "sampler (limit " + std::to_string(STAGE_WORDS - 2) + ")");
+            }
+            DevOp op = make(code, (uint32_t)(e - i));
+            configure(op, i);
+            const size_t cnt = e - i;
+            const size_t hdr = P.tgt.size();
+            P.tgt.resize(hdr + cnt + 1);
+            for (size_t d = 0; d < cnt; d++) {
+                P.tgt[hdr + d] = (uint32_t)(P.tgt.size() - op.toff);
+                for (uint32_t v : items[i + d]) P.tgt.push_back(v);
+                touch(i + d);
+            }
+            P.tgt[hdr + cnt] = (uint32_t)(P.tgt.size() - op.toff);
+            push(op);
+            i = e;
+        }
     }
 
     // ------------------------------------------------------------------ detectors / observables
     void flush_pending_dets() {
         size_t i = 0;
         while (i < det_terms.size()) {
-            DevOp op = make(OP_DET, 0);
-            op.base = det_base;
             uint32_t room = STAGE_ROWS - (det_base % STAGE_ROWS);
-            uint32_t cnt = (uint32_t)std::min<size_t>(room, det_terms.size() - i);
-            // CSR header: cnt + 1 absolute offsets into tgt, then terms.
-            size_t hdr = P.tgt.size();
-            P.tgt.resize(hdr + cnt + 1);
-            for (uint32_t d = 0; d < cnt; d++) {
-                P.tgt[hdr + d] = (uint32_t)P.tgt.size();
-                for (uint32_t s : det_terms[i + d]) {
-                    P.tgt.push_back(s);
-                    touch_r(s, KR);
-                }
-                touch_m((det_base + d) % STAGE_ROWS, KW);
-            }
-            P.tgt[hdr + cnt] = (uint32_t)P.tgt.size();
-            op.n = cnt;
-            push(op);
-            det_base += cnt;
+            size_t cnt = std::min<size_t>(room, det_terms.size() - i);
+            std::vector<std::vector<uint32_t>> chunk(det_terms.begin() + (long)i, det_terms.begin() + (long)(i + cnt));
+            size_t done = 0;
+            emit_csr(
+                OP_DET, chunk,
+                [&](size_t d) {
+                    for (uint32_t s : chunk[d]) touch_r(s, KR);
+                    touch_m((det_base + (uint32_t)d) % STAGE_ROWS, KW);
+                },
+                [&](DevOp &op, size_t first) {
+                    op.base = det_base + (uint32_t)first;
+                    done = first;
+                });
+            (void)done;
+            det_base += (uint32_t)cnt;
             i += cnt;
             if (det_base % STAGE_ROWS == 0) {
                 emit_flush(det_base - STAGE_ROWS, STAGE_ROWS);
@@ -265,21 +310,17 @@ struct Builder {
     }
     void flush_pending_obs() {
         if (obs_items.empty()) return;
-        DevOp op = make(OP_OBS, (uint32_t)obs_items.size());
-        size_t hdr = P.tgt.size();
-        P.tgt.resize(hdr + obs_items.size() + 1);
-        for (size_t i = 0; i < obs_items.size(); i++) {
-            P.tgt[hdr + i] = (uint32_t)P.tgt.size();
-            for (uint32_t v : obs_items[i]) P.tgt.push_back(v);
-            for (size_t k = 1; k < obs_items[i].size(); k++) {
-                uint32_t t = obs_items[i][k];
-                if (t & TERM_PAULI) touch_q(t & TERM_QMASK, KR);
-                else touch_r(t, KR);
-            }
-        }
-        P.tgt[hdr + obs_items.size()] = (uint32_t)P.tgt.size();
-        touch_m(MISC_OBS, KA);
-        push(op);
+        emit_csr(
+            OP_OBS, obs_items,
+            [&](size_t i) {
+                for (size_t k = 1; k < obs_items[i].size(); k++) {
+                    uint32_t t = obs_items[i][k];
+                    if (t & TERM_PAULI) touch_q(t & TERM_QMASK, KR);
+                    else touch_r(t, KR);
+                }
+                touch_m(MISC_OBS, KA);
+            },
+            [&](DevOp &, size_t) {});
         obs_items.clear();
     }
     void flush_pending() {
@@ -288,21 +329,20 @@ struct Builder {
     }
 
     // ------------------------------------------------------------------ measurements
-    // Emits a record-producing op over items, splitting at ring wrap so slots stay contiguous.
-    // 'body' appends one item's targets and touches; returns void.
+    // Record-producing op over items, cut at ring wraps so slots stay contiguous, at qubit
+    // repeats, and at the staging size.
     template <class F>
-    void emit_records(uint32_t code, size_t n, F body, bool split_on_qubits,
-                      const std::function<std::vector<uint32_t>(size_t)> &qubits_of, uint32_t flags,
-                      std::vector<std::pair<uint32_t, uint32_t>> *ranges) {
+    void emit_records(uint32_t code, size_t n, F body, const std::function<std::vector<uint32_t>(size_t)> &qubits_of,
+                      uint32_t flags, std::vector<std::pair<uint32_t, uint32_t>> *ranges) {
         size_t i = 0;
         while (i < n) {
             seg_reset();
             DevOp op = make(code | flags, 0);
             op.base = slot_of(mcount);
             size_t first = i;
-            while (i < n) {
+            while (i < n && op.n < STAGE_WORDS) {
                 if (slot_of(mcount) == 0 && i > first) break;  // ring wrap
-                if (split_on_qubits) {
+                if (qubits_of) {
                     std::vector<uint32_t> qs = qubits_of(i);
                     bool c = false;
                     for (uint32_t x : qs) c |= seg_mark[x] == seg_id;
@@ -324,10 +364,8 @@ struct Builder {
     void emit_mflips(double p, uint64_t m_first, const std::vector<std::pair<uint32_t, uint32_t>> &ranges, uint32_t k) {
         for (const auto &rg : ranges) {
             uint64_t m0 = m_first + rg.first;
-            DevOp dummy{};
-            (void)dummy;
             for (uint32_t j = 0; j < rg.second; j++) touch_r(slot_of(m0 + j), KA);
-            uint32_t opi = emit_noise(OP_MFLIP, rg.second, p, {{1.0, 1u}}, k, rg.first);
+            uint32_t opi = emit_noise(OP_MFLIP, rg.second, (uint32_t)P.ntgt.size(), p, {{1.0, 1u}}, k, rg.first);
             P.ops[opi].base = slot_of(m0);
         }
     }
@@ -344,7 +382,7 @@ struct Builder {
                 P.tgt.push_back(qs[i]);
                 touch_q(qs[i], KW);
             },
-            true, [&](size_t i) { return std::vector<uint32_t>{qs[i]}; }, OPF_RANDOMIZE, &ranges);
+            [&](size_t i) { return std::vector<uint32_t>{qs[i]}; }, OPF_RANDOMIZE, &ranges);
         P.has_randomizing_ops = true;
         if (!in.args.empty()) emit_mflips(in.args[0], m_first, ranges, k);
     }
@@ -353,51 +391,56 @@ struct Builder {
         flush_pending();
         uint64_t m_first = mcount;
         std::vector<std::pair<uint32_t, uint32_t>> ranges;
-        // Items: product i -> CSR header entry + terms. Build per-op CSR inside body.
-        size_t i = 0;
-        size_t n = products.size();
-        while (i < n) {
+        // Products are cut where they overlap an earlier product of the same op (they are measured
+        // one after another), at ring wraps, and at the staging size.
+        std::vector<uint32_t> seg_of(products.size());
+        {
             seg_reset();
-            DevOp op = make(OP_MPP | OPF_RANDOMIZE, 0);
-            op.base = slot_of(mcount);
-            size_t first = i;
-            // Determine segment end first.
-            size_t e = i;
-            uint64_t mc = mcount;
-            while (e < n) {
-                if (slot_of(mc) == 0 && e > first) break;
+            uint32_t cur = seg_id;
+            for (size_t i = 0; i < products.size(); i++) {
                 bool c = false;
-                for (uint32_t t : products[e]) c |= seg_mark[t & TERM_QMASK] == seg_id;
-                if (c) break;
-                for (uint32_t t : products[e]) seg_mark[t & TERM_QMASK] = seg_id;
-                e++;
-                mc++;
-            }
-            size_t cnt = e - i;
-            size_t hdr = P.tgt.size();
-            P.tgt.resize(hdr + cnt + 1);
-            for (size_t d = 0; d < cnt; d++) {
-                P.tgt[hdr + d] = (uint32_t)P.tgt.size();
-                for (uint32_t t : products[i + d]) {
-                    P.tgt.push_back(t);
-                    touch_q(t & TERM_QMASK, KW);
+                for (uint32_t t : products[i]) c |= seg_mark[t & TERM_QMASK] == seg_id;
+                if (c) {
+                    seg_reset();
+                    cur = seg_id;
                 }
+                for (uint32_t t : products[i]) seg_mark[t & TERM_QMASK] = seg_id;
+                seg_of[i] = cur;
+            }
+        }
+        size_t placed = 0;
+        emit_csr(
+            OP_MPP | OPF_RANDOMIZE, products,
+            [&](size_t i) {
+                for (uint32_t t : products[i]) touch_q(t & TERM_QMASK, KW);
                 touch_r(slot_of(mcount), KW);
                 mcount++;
-            }
-            P.tgt[hdr + cnt] = (uint32_t)P.tgt.size();
-            op.n = (uint32_t)cnt;
-            push(op);
-            ranges.push_back({(uint32_t)first, (uint32_t)cnt});
-            i = e;
+                placed = i + 1;
+            },
+            [&](DevOp &op, size_t first) {
+                op.base = slot_of(mcount);
+                if (!ranges.empty() || first) {
+                }
+                ranges.push_back({(uint32_t)first, 0});
+            },
+            [&](size_t first, size_t e) {
+                if (e == first) return true;
+                if (seg_of[e] != seg_of[first]) return false;
+                return slot_of(m_first + e) != 0;  // ring wrap
+            });
+        // Fill range counts from consecutive starts.
+        for (size_t r = 0; r < ranges.size(); r++) {
+            size_t end = r + 1 < ranges.size() ? ranges[r + 1].first : products.size();
+            ranges[r].second = (uint32_t)(end - ranges[r].first);
         }
+        (void)placed;
         P.has_randomizing_ops = true;
         if (!in.args.empty()) emit_mflips(in.args[0], m_first, ranges, k);
     }
 
-    // Combines '*'-joined Pauli terms into a product: map qubit -> (x, z); throws on anti-Hermitian.
+    // Combines '*'-joined Pauli terms into a product; throws on anti-Hermitian products.
     std::vector<uint32_t> combine_product(const Instr &in, size_t b, size_t e, std::vector<uint32_t> *bits) {
-        std::vector<std::pair<uint32_t, uint8_t>> acc;  // (dense qubit, pauli bits) in first-seen order
+        std::vector<std::pair<uint32_t, uint8_t>> acc;
         int imag = 0;
         for (size_t i = b; i < e; i++) {
             const Target &t = in.targets[i];
@@ -433,7 +476,6 @@ struct Builder {
     // ------------------------------------------------------------------ main dispatch
     void run(const std::vector<const Instr *> &flat) {
         P.stats = compute_stats(flat);
-        // Dense qubit map over qubits that touch frames.
         std::vector<uint8_t> used(P.stats.num_qubits + 1, 0);
         for (const Instr *in : flat) {
             if (in->gate == Gate::MPAD || in->gate == Gate::QUBIT_COORDS) continue;
@@ -472,9 +514,51 @@ struct Builder {
             touch_m(MISC_OBS, KR);
             push(op);
         }
-        if (!P.ops.empty()) P.ops.back().code |= OPF_SYNC;
+        if (!P.ops.empty() && !(P.ops.back().code & OPF_SYNC)) {
+            P.ops.back().code |= OPF_SYNC;
+            P.num_syncs++;
+        }
         P.num_noise_ops = noise_count;
-        if (P.tgt.size() >= (1ull << 32)) throw std::invalid_argument("circuit program too large");
+        if (P.tgt.size() >= (1ull << 32) || P.ntgt.size() >= (1ull << 32)) {
+            throw std::invalid_argument("circuit program too large");
+        }
+        build_epochs();
+    }
+
+    // Groups ops into barrier-free runs whose staged words fit one staging buffer.
+    void build_epochs() {
+        uint32_t b = 0;
+        uint64_t tbegin = 0, tend = 0;
+        bool any = false;
+        auto close = [&](uint32_t e) {
+            DevEpoch ep{};
+            ep.op_begin = b;
+            ep.op_end = e;
+            ep.tbegin = any ? (uint32_t)tbegin : 0;
+            ep.tlen = any ? (uint32_t)(tend - tbegin) : 0;
+            P.epochs.push_back(ep);
+            b = e;
+            any = false;
+        };
+        for (uint32_t i = 0; i < (uint32_t)P.ops.size(); i++) {
+            const uint32_t len = op_tlen[i];
+            if (len) {
+                const uint64_t s = P.ops[i].toff, e = s + len;
+                if (any && e - tbegin > STAGE_WORDS) {
+                    // Would overflow the staging buffer: barrier before this op.
+                    P.ops[i - 1].code |= OPF_SYNC;
+                    P.num_syncs++;
+                    close(i);
+                }
+                if (!any) {
+                    tbegin = s;
+                    any = true;
+                }
+                tend = e;
+            }
+            if (P.ops[i].code & OPF_SYNC) close(i + 1);
+        }
+        if (b < P.ops.size()) close((uint32_t)P.ops.size());
     }
 
     void step(const Instr &in, uint32_t k) {
@@ -489,7 +573,6 @@ struct Builder {
                 std::vector<uint32_t> terms;
                 for (const Target &t : T) terms.push_back(lookback_slot(t.value));
                 det_terms.push_back(std::move(terms));
-                dcount++;
                 return;
             }
             case Gate::OBSERVABLE_INCLUDE: {
@@ -551,7 +634,7 @@ struct Builder {
             case Gate::MPAD: {
                 uint64_t m_first = mcount;
                 std::vector<std::pair<uint32_t, uint32_t>> ranges;
-                emit_records(OP_RECZERO, T.size(), [&](size_t) {}, false, nullptr, 0, &ranges);
+                emit_records(OP_RECZERO, T.size(), [&](size_t) {}, nullptr, 0, &ranges);
                 if (!in.args.empty()) emit_mflips(in.args[0], m_first, ranges, k);
                 return;
             }
@@ -576,19 +659,16 @@ struct Builder {
                     std::vector<uint32_t> bits;
                     std::vector<uint32_t> terms = combine_product(in, g.first, g.second, &bits);
                     if (terms.empty()) continue;
-                    DevOp op = make(OP_SPP, 1);
-                    P.tgt.push_back(op.toff + 2);
-                    P.tgt.push_back(0);
-                    for (uint32_t t : terms) {
-                        P.tgt.push_back(t);
-                        touch_q(t & TERM_QMASK, KW);
-                    }
-                    for (uint32_t s : bits) {
-                        P.tgt.push_back(s);
-                        touch_r(s, KR);
-                    }
-                    P.tgt[op.toff + 1] = (uint32_t)P.tgt.size();
-                    push(op);
+                    std::vector<uint32_t> words = terms;
+                    for (uint32_t s : bits) words.push_back(s);
+                    std::vector<std::vector<uint32_t>> one{words};
+                    emit_csr(
+                        OP_SPP, one,
+                        [&](size_t) {
+                            for (uint32_t t : terms) touch_q(t & TERM_QMASK, KW);
+                            for (uint32_t s : bits) touch_r(s, KR);
+                        },
+                        [&](DevOp &, size_t) {});
                 }
                 return;
             }
@@ -668,13 +748,12 @@ struct Builder {
                 }
                 std::vector<uint32_t> qs = qubits();
                 if (qs.empty()) return;
-                uint32_t toff = (uint32_t)P.tgt.size();
+                uint32_t ntoff = (uint32_t)P.ntgt.size();
                 for (uint32_t x : qs) {
-                    P.tgt.push_back(x);
+                    P.ntgt.push_back(x);
                     touch_q(x, KA);
                 }
-                uint32_t opi = emit_noise(OP_NOISE1, (uint32_t)qs.size(), std::min(p, 1.0), outs, k, 0);
-                P.ops[opi].toff = toff;
+                emit_noise(OP_NOISE1, (uint32_t)qs.size(), ntoff, std::min(p, 1.0), outs, k, 0);
                 return;
             }
             case Gate::DEPOLARIZE2: case Gate::PAULI_CHANNEL_2: {
@@ -692,13 +771,12 @@ struct Builder {
                     }
                 }
                 if (T.empty()) return;
-                uint32_t toff = (uint32_t)P.tgt.size();
+                uint32_t ntoff = (uint32_t)P.ntgt.size();
                 for (const Target &t : T) {
-                    P.tgt.push_back(q(t));
+                    P.ntgt.push_back(q(t));
                     touch_q(q(t), KA);
                 }
-                uint32_t opi = emit_noise(OP_NOISE2, (uint32_t)T.size() / 2, std::min(p, 1.0), outs, k, 0);
-                P.ops[opi].toff = toff;
+                emit_noise(OP_NOISE2, (uint32_t)T.size() / 2, ntoff, std::min(p, 1.0), outs, k, 0);
                 return;
             }
             case Gate::E: case Gate::ELSE_CORRELATED_ERROR: {
@@ -707,16 +785,15 @@ struct Builder {
                     touch_m(MISC_CHAIN, KW);
                     push(op);
                 }
-                uint32_t toff = (uint32_t)P.tgt.size();
+                uint32_t ntoff = (uint32_t)P.ntgt.size();
                 for (const Target &t : T) {
                     uint32_t term = TERM_PAULI | q(t) | ((t.pauli & 1) ? TERM_X : 0) | ((t.pauli & 2) ? TERM_Z : 0);
-                    P.tgt.push_back(term);
+                    P.ntgt.push_back(term);
                     touch_q(q(t), KA);
                 }
                 touch_m(MISC_CHAIN, KA);
-                uint32_t opi = emit_noise(OP_CORR, 1, std::min(in.args[0], 1.0), {{1.0, 1u}}, k, 0,
+                uint32_t opi = emit_noise(OP_CORR, 1, ntoff, std::min(in.args[0], 1.0), {{1.0, 1u}}, k, 0,
                                           in.gate == Gate::E ? OPF_CHAIN_START : 0);
-                P.ops[opi].toff = toff;
                 P.ops[opi].base = (uint32_t)T.size();  // term count
                 return;
             }
@@ -724,7 +801,7 @@ struct Builder {
                 std::vector<uint32_t> qs = qubits();
                 uint64_t m_first = mcount;
                 std::vector<std::pair<uint32_t, uint32_t>> ranges;
-                emit_records(OP_RECZERO, qs.size(), [&](size_t) {}, false, nullptr, 0, &ranges);
+                emit_records(OP_RECZERO, qs.size(), [&](size_t) {}, nullptr, 0, &ranges);
                 double p;
                 std::vector<std::pair<double, uint32_t>> outs;
                 if (in.gate == Gate::HERALDED_ERASE) {
@@ -735,14 +812,13 @@ struct Builder {
                     outs = {{in.args[0], 4u}, {in.args[1], 5u}, {in.args[2], 7u}, {in.args[3], 6u}};
                 }
                 for (const auto &rg : ranges) {
-                    uint32_t toff = (uint32_t)P.tgt.size();
+                    uint32_t ntoff = (uint32_t)P.ntgt.size();
                     for (uint32_t j = 0; j < rg.second; j++) {
-                        P.tgt.push_back(qs[rg.first + j]);
+                        P.ntgt.push_back(qs[rg.first + j]);
                         touch_q(qs[rg.first + j], KA);
                         touch_r(slot_of(m_first + rg.first + j), KA);
                     }
-                    uint32_t opi = emit_noise(OP_HERALD, rg.second, std::min(p, 1.0), outs, k, rg.first);
-                    P.ops[opi].toff = toff;
+                    uint32_t opi = emit_noise(OP_HERALD, rg.second, ntoff, std::min(p, 1.0), outs, k, rg.first);
                     P.ops[opi].base = slot_of(m_first + rg.first);
                 }
                 return;
@@ -777,7 +853,7 @@ void resolve_fault(const Program &prog, uint32_t k, uint32_t j, uint32_t *op, ui
 
 size_t lds_bytes(const Program &prog, int T) {
     size_t rows = 2 * (size_t)prog.num_qubits + prog.ring + STAGE_ROWS + prog.num_observables + 1;
-    return rows * 4 * (size_t)T;
+    return rows * 4 * (size_t)T + 2 * STAGE_WORDS * 4;
 }
 
 }  // namespace sa

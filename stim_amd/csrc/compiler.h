@@ -17,15 +17,11 @@ struct SiteRange {
     uint32_t count;     // sites covered
 };
 
-// One noise channel site family, used by the symptom engine and statistics.
-struct NoiseOpInfo {
-    uint32_t op;         // device op index
-    uint32_t flat_index; // instruction index in the flattened circuit
-};
-
 struct Program {
     std::vector<DevOp> ops;
-    std::vector<uint32_t> tgt;
+    std::vector<DevEpoch> epochs;
+    std::vector<uint32_t> tgt;    // frame-op words (staged into LDS per epoch)
+    std::vector<uint32_t> ntgt;   // noise-op words
     std::vector<uint32_t> otab;
     uint32_t num_qubits = 0;      // dense qubit rows
     uint32_t ring = 1;            // record ring rows

@@ -127,8 +127,8 @@ struct Tile {
 
 // Applies one noise outcome to shot s of the tile. site is the site within op.
 template <int T>
-__device__ __forceinline__ void apply_hit(const Tile<T> &t, const DevOp &op, const uint32_t *tgt, uint32_t site,
-                                          uint32_t s, uint32_t code) {
+__device__ __forceinline__ void apply_hit(const Tile<T> &t, const DevOp &op, const uint32_t *__restrict__ tgt,
+                                          uint32_t site, uint32_t s, uint32_t code) {
     const uint32_t w = s >> 5;
     const uint32_t bit = 1u << (s & 31);
     switch (op.code & 0xFF) {
@@ -191,6 +191,20 @@ __device__ __forceinline__ uint32_t transpose32(uint32_t x, uint32_t lane) {
     return x;
 }
 
+// Streams up to 4 bytes of x to p with the widest aligned stores (non-temporal: output rows are
+// written once and must not evict the program from L2).
+__device__ __forceinline__ void store_bytes(uint8_t *p, uint32_t x, uint32_t nbytes) {
+    const uintptr_t a = (uintptr_t)p;
+    if (nbytes == 4 && (a & 3) == 0) {
+        __builtin_nontemporal_store(x, reinterpret_cast<uint32_t *>(p));
+    } else if (nbytes == 4 && (a & 1) == 0) {
+        __builtin_nontemporal_store((uint16_t)x, reinterpret_cast<uint16_t *>(p));
+        __builtin_nontemporal_store((uint16_t)(x >> 16), reinterpret_cast<uint16_t *>(p + 2));
+    } else {
+        for (uint32_t k = 0; k < nbytes; k++) __builtin_nontemporal_store((uint8_t)(x >> (8 * k)), p + k);
+    }
+}
+
 // Writes staged rows [0, n) (entries base .. base + n) of the tile to one output.
 template <int T>
 __device__ void flush_rows(const uint32_t *rows, uint32_t n, uint32_t base, const DevOut &out, uint64_t shot0,
@@ -212,7 +226,7 @@ __device__ void flush_rows(const uint32_t *rows, uint32_t n, uint32_t base, cons
                 uint8_t *p = out.ptr + shot * out.stride + (out.col0 >> 3) + ((base + rb * 32) >> 3);
                 const uint32_t remaining = n - rb * 32;
                 const uint32_t nbytes = remaining >= 32 ? 4 : (remaining + 7) / 8;
-                for (uint32_t k = 0; k < nbytes; k++) p[k] = (uint8_t)(x >> (8 * k));
+                store_bytes(p, x, nbytes);
             }
         }
     } else {
@@ -239,7 +253,10 @@ __device__ __forceinline__ Wd<T> rand_words(const Rng &rng, uint32_t op_i, uint3
 }
 
 template <int T>
-__global__ __launch_bounds__(NT) void frame_sim_kernel(LaunchParams P) {
+__global__ __launch_bounds__(NT) void frame_sim_kernel(const DevOp *__restrict__ ops, const DevEpoch *__restrict__ epochs,
+                                                       const uint32_t *__restrict__ tgt,
+                                                       const uint32_t *__restrict__ ntgt,
+                                                       const uint32_t *__restrict__ otab, LaunchParams P) {
     extern __shared__ uint32_t lds[];
     Tile<T> t;
     t.X = lds;
@@ -248,18 +265,22 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(LaunchParams P) {
     t.STG = t.REC + P.ring * T;
     t.OBS = t.STG + STAGE_ROWS * T;
     t.CHAIN = t.OBS + P.num_obs * T;
+    uint32_t *const STAGE = t.CHAIN + T;
     const uint32_t tid = threadIdx.x;
     const uint32_t W = 32 * T;
     const uint64_t shot0 = (uint64_t)blockIdx.x * W;
     Rng rng{(uint32_t)P.seed, (uint32_t)(P.seed >> 32), (uint32_t)(P.tile0 + blockIdx.x)};
     const bool randomize = P.randomize != 0;
-    const uint32_t *tgt = P.tgt;
 
     for (uint32_t i = tid; i < P.num_qubits; i += NT) {
         row<T>(t.X, i) = wzero<T>();
         row<T>(t.Z, i) = randomize ? rand_words<T>(rng, 0xFFFFFFFFu, i) : wzero<T>();
     }
     for (uint32_t i = tid; i < (P.num_obs + 1) * T; i += NT) t.OBS[i] = 0;
+    if (P.n_epochs) {
+        const DevEpoch e0 = epochs[0];
+        for (uint32_t i = tid; i < e0.tlen; i += NT) STAGE[i] = tgt[e0.tbegin + i];
+    }
     __syncthreads();
 
     uint32_t fptr = 0, fend = 0;
@@ -268,286 +289,315 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(LaunchParams P) {
         fend = P.fault_off[blockIdx.x + 1];
     }
 
-    for (uint32_t oi = 0; oi < P.n_ops; oi++) {
-        const DevOp op = P.ops[oi];
-        const uint32_t code = op.code & 0xFF;
-        const bool rnd = randomize && (op.code & OPF_RANDOMIZE);
-        switch (code) {
-            case OP_H:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    uint32_t q = tgt[op.toff + i];
-                    Wd<T> x = row<T>(t.X, q), z = row<T>(t.Z, q);
-                    row<T>(t.X, q) = z;
-                    row<T>(t.Z, q) = x;
-                }
-                break;
-            case OP_HXY:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    uint32_t q = tgt[op.toff + i];
-                    row<T>(t.Z, q) = wxor(row<T>(t.Z, q), row<T>(t.X, q));
-                }
-                break;
-            case OP_HYZ:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    uint32_t q = tgt[op.toff + i];
-                    row<T>(t.X, q) = wxor(row<T>(t.X, q), row<T>(t.Z, q));
-                }
-                break;
-            case OP_CXYZ:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    uint32_t q = tgt[op.toff + i];
-                    Wd<T> x = row<T>(t.X, q), z = row<T>(t.Z, q);
-                    x = wxor(x, z);
-                    z = wxor(z, x);
-                    row<T>(t.X, q) = x;
-                    row<T>(t.Z, q) = z;
-                }
-                break;
-            case OP_CZYX:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    uint32_t q = tgt[op.toff + i];
-                    Wd<T> x = row<T>(t.X, q), z = row<T>(t.Z, q);
-                    z = wxor(z, x);
-                    x = wxor(x, z);
-                    row<T>(t.X, q) = x;
-                    row<T>(t.Z, q) = z;
-                }
-                break;
-            case OP_CX: case OP_CY: case OP_CZ: case OP_SWAP: case OP_ISWAP: case OP_CXSWAP: case OP_CZSWAP:
-            case OP_SWAPCX: case OP_SQRTXX: case OP_SQRTYY: case OP_SQRTZZ: case OP_XCX: case OP_XCY: case OP_YCX:
-            case OP_YCY:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    const uint2 qq = *reinterpret_cast<const uint2 *>(tgt + op.toff + 2 * i);
-                    Wd<T> x1 = row<T>(t.X, qq.x), z1 = row<T>(t.Z, qq.x);
-                    Wd<T> x2 = row<T>(t.X, qq.y), z2 = row<T>(t.Z, qq.y);
-                    switch (code) {
-                        case OP_CX: z1 = wxor(z1, z2); x2 = wxor(x2, x1); break;
-                        case OP_CY: z1 = wxor(z1, wxor(x2, z2)); z2 = wxor(z2, x1); x2 = wxor(x2, x1); break;
-                        case OP_CZ: z1 = wxor(z1, x2); z2 = wxor(z2, x1); break;
-                        case OP_SWAP: { Wd<T> a = x1; x1 = x2; x2 = a; a = z1; z1 = z2; z2 = a; } break;
-                        case OP_ISWAP: {
-                            Wd<T> dx = wxor(x1, x2);
-                            Wd<T> t1 = wxor(z1, dx), t2 = wxor(z2, dx);
-                            z1 = t2; z2 = t1;
-                            Wd<T> a = x1; x1 = x2; x2 = a;
-                        } break;
-                        case OP_CXSWAP: z2 = wxor(z2, z1); z1 = wxor(z1, z2); x1 = wxor(x1, x2); x2 = wxor(x2, x1); break;
-                        case OP_CZSWAP: {
-                            Wd<T> a = x1; x1 = x2; x2 = a; a = z1; z1 = z2; z2 = a;
-                            z1 = wxor(z1, x2); z2 = wxor(z2, x1);
-                        } break;
-                        case OP_SWAPCX: z1 = wxor(z1, z2); z2 = wxor(z2, z1); x2 = wxor(x2, x1); x1 = wxor(x1, x2); break;
-                        case OP_SQRTXX: { Wd<T> d = wxor(z1, z2); x1 = wxor(x1, d); x2 = wxor(x2, d); } break;
-                        case OP_SQRTYY: {
-                            Wd<T> d = wxor(wxor(x1, z1), wxor(x2, z2));
-                            x1 = wxor(x1, d); z1 = wxor(z1, d); x2 = wxor(x2, d); z2 = wxor(z2, d);
-                        } break;
-                        case OP_SQRTZZ: { Wd<T> d = wxor(x1, x2); z1 = wxor(z1, d); z2 = wxor(z2, d); } break;
-                        case OP_XCX: x1 = wxor(x1, z2); x2 = wxor(x2, z1); break;
-                        case OP_XCY: x1 = wxor(x1, wxor(x2, z2)); x2 = wxor(x2, z1); z2 = wxor(z2, z1); break;
-                        case OP_YCX: x2 = wxor(x2, wxor(x1, z1)); x1 = wxor(x1, z2); z1 = wxor(z1, z2); break;
-                        case OP_YCY: {
-                            Wd<T> y1 = wxor(x1, z1), y2 = wxor(x2, z2);
-                            x1 = wxor(x1, y2); z1 = wxor(z1, y2); x2 = wxor(x2, y1); z2 = wxor(z2, y1);
-                        } break;
-                        default: break;
-                    }
-                    row<T>(t.X, qq.x) = x1;
-                    row<T>(t.Z, qq.x) = z1;
-                    row<T>(t.X, qq.y) = x2;
-                    row<T>(t.Z, qq.y) = z2;
-                }
-                break;
-            case OP_FEEDBACK:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    const uint32_t slot = tgt[op.toff + 3 * i], q = tgt[op.toff + 3 * i + 1], mask = tgt[op.toff + 3 * i + 2];
-                    const Wd<T> r = row<T>(t.REC, slot);
-                    if (mask & 1) row<T>(t.X, q) = wxor(row<T>(t.X, q), r);
-                    if (mask & 2) row<T>(t.Z, q) = wxor(row<T>(t.Z, q), r);
-                }
-                break;
-            case OP_R: case OP_RX: case OP_RY:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    const uint32_t q = tgt[op.toff + i];
-                    const Wd<T> r = rnd ? rand_words<T>(rng, oi, i) : wzero<T>();
-                    if (code == OP_R) {
-                        row<T>(t.X, q) = wzero<T>();
-                        if (rnd) row<T>(t.Z, q) = wxor(row<T>(t.Z, q), r);
-                    } else if (code == OP_RX) {
-                        row<T>(t.Z, q) = wzero<T>();
-                        if (rnd) row<T>(t.X, q) = wxor(row<T>(t.X, q), r);
-                    } else {
-                        Wd<T> z = rnd ? wxor(row<T>(t.Z, q), r) : row<T>(t.Z, q);
-                        row<T>(t.Z, q) = z;
-                        row<T>(t.X, q) = z;
-                    }
-                }
-                break;
-            case OP_M: case OP_MX: case OP_MY: case OP_MR: case OP_MRX: case OP_MRY:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    const uint32_t q = tgt[op.toff + i];
-                    Wd<T> x = row<T>(t.X, q), z = row<T>(t.Z, q);
-                    const Wd<T> r = rnd ? rand_words<T>(rng, oi, i) : wzero<T>();
-                    Wd<T> rec;
-                    if (code == OP_M || code == OP_MR) {
-                        rec = x;
-                        if (code == OP_MR) x = wzero<T>();
-                        z = wxor(z, r);
-                    } else if (code == OP_MX || code == OP_MRX) {
-                        rec = z;
-                        if (code == OP_MRX) z = wzero<T>();
-                        x = wxor(x, r);
-                    } else {
-                        rec = wxor(x, z);
-                        if (code == OP_MY) {
-                            x = wxor(x, r);
-                            z = wxor(z, r);
-                        } else {
-                            z = wxor(z, r);
-                            x = z;
-                        }
-                    }
-                    row<T>(t.X, q) = x;
-                    row<T>(t.Z, q) = z;
-                    row<T>(t.REC, op.base + i) = rec;
-                }
-                break;
-            case OP_MPP:
-            case OP_SPP:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    const uint32_t b = tgt[op.toff + i], e = tgt[op.toff + i + 1];
-                    Wd<T> v = wzero<T>();
-                    for (uint32_t k = b; k < e; k++) {
-                        const uint32_t term = tgt[k];
-                        if (term & TERM_PAULI) {
-                            const uint32_t q = term & TERM_QMASK;
-                            if (term & TERM_X) v = wxor(v, row<T>(t.Z, q));
-                            if (term & TERM_Z) v = wxor(v, row<T>(t.X, q));
-                        } else {
-                            v = wxor(v, row<T>(t.REC, term));
-                        }
-                    }
-                    Wd<T> apply;
-                    if (code == OP_MPP) {
-                        row<T>(t.REC, op.base + i) = v;
-                        if (!rnd) continue;
-                        apply = rand_words<T>(rng, oi, i);
-                    } else {
-                        apply = v;
-                    }
-                    for (uint32_t k = b; k < e; k++) {
-                        const uint32_t term = tgt[k];
-                        if (!(term & TERM_PAULI)) continue;
-                        const uint32_t q = term & TERM_QMASK;
-                        if (term & TERM_X) row<T>(t.X, q) = wxor(row<T>(t.X, q), apply);
-                        if (term & TERM_Z) row<T>(t.Z, q) = wxor(row<T>(t.Z, q), apply);
-                    }
-                }
-                break;
-            case OP_RECZERO:
-                for (uint32_t i = tid; i < op.n; i += NT) row<T>(t.REC, op.base + i) = wzero<T>();
-                break;
-            case OP_DET:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    const uint32_t b = tgt[op.toff + i], e = tgt[op.toff + i + 1];
-                    Wd<T> v = wzero<T>();
-                    for (uint32_t k = b; k < e; k++) v = wxor(v, row<T>(t.REC, tgt[k]));
-                    row<T>(t.STG, (op.base + i) % STAGE_ROWS) = v;
-                }
-                break;
-            case OP_OBS:
-                for (uint32_t i = tid; i < op.n; i += NT) {
-                    const uint32_t b = tgt[op.toff + i], e = tgt[op.toff + i + 1];
-                    const uint32_t idx = tgt[b];
-                    Wd<T> v = wzero<T>();
-                    for (uint32_t k = b + 1; k < e; k++) {
-                        const uint32_t term = tgt[k];
-                        if (term & TERM_PAULI) {
-                            const uint32_t q = term & TERM_QMASK;
-                            if (term & TERM_X) v = wxor(v, row<T>(t.Z, q));
-                            if (term & TERM_Z) v = wxor(v, row<T>(t.X, q));
-                        } else {
-                            v = wxor(v, row<T>(t.REC, term));
-                        }
-                    }
+    uint32_t buf = 0;
+    for (uint32_t e = 0; e < P.n_epochs; e++) {
+        const DevEpoch ep = epochs[e];
+        // Prefetch the next epoch's words into registers; they land in the other staging buffer
+        // after this epoch's ops, so their latency hides behind this epoch's work.
+        uint32_t nlen = 0, nbeg = 0;
+        if (e + 1 < P.n_epochs) {
+            const DevEpoch ne = epochs[e + 1];
+            nlen = ne.tlen;
+            nbeg = ne.tbegin;
+        }
+        uint32_t pre[STAGE_WORDS / NT];
 #pragma unroll
-                    for (int w = 0; w < T; w++) {
-                        if (v.v[w]) atomicXor(&t.OBS[idx * T + w], v.v[w]);
+        for (uint32_t k = 0; k < STAGE_WORDS / NT; k++) {
+            const uint32_t idx = tid + k * NT;
+            pre[k] = idx < nlen ? tgt[nbeg + idx] : 0u;
+        }
+        // st[op.toff + j] reads word j of a frame op from the staging buffer.
+        const uint32_t *st = STAGE + buf * STAGE_WORDS - ep.tbegin;
+
+        for (uint32_t oi = ep.op_begin; oi < ep.op_end; oi++) {
+            const DevOp op = ops[oi];
+            const uint32_t code = op.code & 0xFF;
+            const bool rnd = randomize && (op.code & OPF_RANDOMIZE);
+            const uint32_t *w = st + op.toff;
+            switch (code) {
+                case OP_H:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t q = w[i];
+                        Wd<T> x = row<T>(t.X, q), z = row<T>(t.Z, q);
+                        row<T>(t.X, q) = z;
+                        row<T>(t.Z, q) = x;
                     }
-                }
-                break;
-            case OP_FLUSH:
-                if (P.dets.enabled) flush_rows<T>(t.STG, op.n, op.base, P.dets, shot0, P.shots);
-                break;
-            case OP_FLUSH_OBS:
-                if (P.obs.enabled) flush_rows<T>(t.OBS, op.n, 0, P.obs, shot0, P.shots);
-                break;
-            case OP_CHAINCLR:
-                if (tid < T) t.CHAIN[tid] = 0;
-                break;
-            case OP_NOISE1: case OP_NOISE2: case OP_MFLIP: case OP_CORR: case OP_HERALD: {
-                if (P.mode == MODE_FIXED) {
-                    // Faults of this tile are sorted by op; consume the prefix that belongs to this op.
-                    while (true) {
-                        const uint32_t idx = fptr + tid;
-                        bool mine = false;
-                        if (idx < fend) {
-                            const DevFault f = P.faults[idx];
-                            if (f.op == oi) {
-                                mine = true;
-                                apply_hit<T>(t, op, tgt, f.site, f.shot, f.code);
+                    break;
+                case OP_HXY:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t q = w[i];
+                        row<T>(t.Z, q) = wxor(row<T>(t.Z, q), row<T>(t.X, q));
+                    }
+                    break;
+                case OP_HYZ:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t q = w[i];
+                        row<T>(t.X, q) = wxor(row<T>(t.X, q), row<T>(t.Z, q));
+                    }
+                    break;
+                case OP_CXYZ:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t q = w[i];
+                        Wd<T> x = row<T>(t.X, q), z = row<T>(t.Z, q);
+                        x = wxor(x, z);
+                        z = wxor(z, x);
+                        row<T>(t.X, q) = x;
+                        row<T>(t.Z, q) = z;
+                    }
+                    break;
+                case OP_CZYX:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t q = w[i];
+                        Wd<T> x = row<T>(t.X, q), z = row<T>(t.Z, q);
+                        z = wxor(z, x);
+                        x = wxor(x, z);
+                        row<T>(t.X, q) = x;
+                        row<T>(t.Z, q) = z;
+                    }
+                    break;
+                case OP_CX: case OP_CY: case OP_CZ: case OP_SWAP: case OP_ISWAP: case OP_CXSWAP: case OP_CZSWAP:
+                case OP_SWAPCX: case OP_SQRTXX: case OP_SQRTYY: case OP_SQRTZZ: case OP_XCX: case OP_XCY:
+                case OP_YCX: case OP_YCY:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t qa = w[2 * i], qb = w[2 * i + 1];
+                        Wd<T> x1 = row<T>(t.X, qa), z1 = row<T>(t.Z, qa);
+                        Wd<T> x2 = row<T>(t.X, qb), z2 = row<T>(t.Z, qb);
+                        switch (code) {
+                            case OP_CX: z1 = wxor(z1, z2); x2 = wxor(x2, x1); break;
+                            case OP_CY: z1 = wxor(z1, wxor(x2, z2)); z2 = wxor(z2, x1); x2 = wxor(x2, x1); break;
+                            case OP_CZ: z1 = wxor(z1, x2); z2 = wxor(z2, x1); break;
+                            case OP_SWAP: { Wd<T> a = x1; x1 = x2; x2 = a; a = z1; z1 = z2; z2 = a; } break;
+                            case OP_ISWAP: {
+                                Wd<T> dx = wxor(x1, x2);
+                                Wd<T> t1 = wxor(z1, dx), t2 = wxor(z2, dx);
+                                z1 = t2; z2 = t1;
+                                Wd<T> a = x1; x1 = x2; x2 = a;
+                            } break;
+                            case OP_CXSWAP: z2 = wxor(z2, z1); z1 = wxor(z1, z2); x1 = wxor(x1, x2); x2 = wxor(x2, x1); break;
+                            case OP_CZSWAP: {
+                                Wd<T> a = x1; x1 = x2; x2 = a; a = z1; z1 = z2; z2 = a;
+                                z1 = wxor(z1, x2); z2 = wxor(z2, x1);
+                            } break;
+                            case OP_SWAPCX: z1 = wxor(z1, z2); z2 = wxor(z2, z1); x2 = wxor(x2, x1); x1 = wxor(x1, x2); break;
+                            case OP_SQRTXX: { Wd<T> d = wxor(z1, z2); x1 = wxor(x1, d); x2 = wxor(x2, d); } break;
+                            case OP_SQRTYY: {
+                                Wd<T> d = wxor(wxor(x1, z1), wxor(x2, z2));
+                                x1 = wxor(x1, d); z1 = wxor(z1, d); x2 = wxor(x2, d); z2 = wxor(z2, d);
+                            } break;
+                            case OP_SQRTZZ: { Wd<T> d = wxor(x1, x2); z1 = wxor(z1, d); z2 = wxor(z2, d); } break;
+                            case OP_XCX: x1 = wxor(x1, z2); x2 = wxor(x2, z1); break;
+                            case OP_XCY: x1 = wxor(x1, wxor(x2, z2)); x2 = wxor(x2, z1); z2 = wxor(z2, z1); break;
+                            case OP_YCX: x2 = wxor(x2, wxor(x1, z1)); x1 = wxor(x1, z2); z1 = wxor(z1, z2); break;
+                            case OP_YCY: {
+                                Wd<T> y1 = wxor(x1, z1), y2 = wxor(x2, z2);
+                                x1 = wxor(x1, y2); z1 = wxor(z1, y2); x2 = wxor(x2, y1); z2 = wxor(z2, y1);
+                            } break;
+                            default: break;
+                        }
+                        row<T>(t.X, qa) = x1;
+                        row<T>(t.Z, qa) = z1;
+                        row<T>(t.X, qb) = x2;
+                        row<T>(t.Z, qb) = z2;
+                    }
+                    break;
+                case OP_FEEDBACK:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t slot = w[3 * i], q = w[3 * i + 1], mask = w[3 * i + 2];
+                        const Wd<T> r = row<T>(t.REC, slot);
+                        if (mask & 1) row<T>(t.X, q) = wxor(row<T>(t.X, q), r);
+                        if (mask & 2) row<T>(t.Z, q) = wxor(row<T>(t.Z, q), r);
+                    }
+                    break;
+                case OP_R: case OP_RX: case OP_RY:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t q = w[i];
+                        const Wd<T> r = rnd ? rand_words<T>(rng, oi, i) : wzero<T>();
+                        if (code == OP_R) {
+                            row<T>(t.X, q) = wzero<T>();
+                            if (rnd) row<T>(t.Z, q) = wxor(row<T>(t.Z, q), r);
+                        } else if (code == OP_RX) {
+                            row<T>(t.Z, q) = wzero<T>();
+                            if (rnd) row<T>(t.X, q) = wxor(row<T>(t.X, q), r);
+                        } else {
+                            Wd<T> z = rnd ? wxor(row<T>(t.Z, q), r) : row<T>(t.Z, q);
+                            row<T>(t.Z, q) = z;
+                            row<T>(t.X, q) = z;
+                        }
+                    }
+                    break;
+                case OP_M: case OP_MX: case OP_MY: case OP_MR: case OP_MRX: case OP_MRY:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t q = w[i];
+                        Wd<T> x = row<T>(t.X, q), z = row<T>(t.Z, q);
+                        const Wd<T> r = rnd ? rand_words<T>(rng, oi, i) : wzero<T>();
+                        Wd<T> rec;
+                        if (code == OP_M || code == OP_MR) {
+                            rec = x;
+                            if (code == OP_MR) x = wzero<T>();
+                            z = wxor(z, r);
+                        } else if (code == OP_MX || code == OP_MRX) {
+                            rec = z;
+                            if (code == OP_MRX) z = wzero<T>();
+                            x = wxor(x, r);
+                        } else {
+                            rec = wxor(x, z);
+                            if (code == OP_MY) {
+                                x = wxor(x, r);
+                                z = wxor(z, r);
+                            } else {
+                                z = wxor(z, r);
+                                x = z;
                             }
                         }
-                        const uint32_t cnt = __syncthreads_count(mine);
-                        fptr += cnt;
-                        if (cnt < NT) break;
+                        row<T>(t.X, q) = x;
+                        row<T>(t.Z, q) = z;
+                        row<T>(t.REC, op.base + i) = rec;
                     }
                     break;
-                }
-                if (P.mode != MODE_SAMPLED || !(op.p > 0.0f)) break;
-                const uint32_t nout = op.nout & 0xFFFF;
-                if (op.code & OPF_DENSE) {
-                    // Dense draw: Bernoulli word per (site, word), then one outcome per fired shot.
-                    for (uint32_t i = tid; i < op.n * T; i += NT) {
-                        const uint32_t site = i / T, w = i % T;
-                        uint32_t fire = bernoulli_word(op.p, rng, i, op.fid);
+                case OP_MPP:
+                case OP_SPP:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t b = w[i], e2 = w[i + 1];
+                        Wd<T> v = wzero<T>();
+                        for (uint32_t k = b; k < e2; k++) {
+                            const uint32_t term = w[k];
+                            if (term & TERM_PAULI) {
+                                const uint32_t q = term & TERM_QMASK;
+                                if (term & TERM_X) v = wxor(v, row<T>(t.Z, q));
+                                if (term & TERM_Z) v = wxor(v, row<T>(t.X, q));
+                            } else {
+                                v = wxor(v, row<T>(t.REC, term));
+                            }
+                        }
+                        Wd<T> apply;
+                        if (code == OP_MPP) {
+                            row<T>(t.REC, op.base + i) = v;
+                            if (!rnd) continue;
+                            apply = rand_words<T>(rng, oi, i);
+                        } else {
+                            apply = v;
+                        }
+                        for (uint32_t k = b; k < e2; k++) {
+                            const uint32_t term = w[k];
+                            if (!(term & TERM_PAULI)) continue;
+                            const uint32_t q = term & TERM_QMASK;
+                            if (term & TERM_X) row<T>(t.X, q) = wxor(row<T>(t.X, q), apply);
+                            if (term & TERM_Z) row<T>(t.Z, q) = wxor(row<T>(t.Z, q), apply);
+                        }
+                    }
+                    break;
+                case OP_RECZERO:
+                    for (uint32_t i = tid; i < op.n; i += NT) row<T>(t.REC, op.base + i) = wzero<T>();
+                    break;
+                case OP_DET:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t b = w[i], e2 = w[i + 1];
+                        Wd<T> v = wzero<T>();
+                        for (uint32_t k = b; k < e2; k++) v = wxor(v, row<T>(t.REC, w[k]));
+                        row<T>(t.STG, (op.base + i) % STAGE_ROWS) = v;
+                    }
+                    break;
+                case OP_OBS:
+                    for (uint32_t i = tid; i < op.n; i += NT) {
+                        const uint32_t b = w[i], e2 = w[i + 1];
+                        const uint32_t idx = w[b];
+                        Wd<T> v = wzero<T>();
+                        for (uint32_t k = b + 1; k < e2; k++) {
+                            const uint32_t term = w[k];
+                            if (term & TERM_PAULI) {
+                                const uint32_t q = term & TERM_QMASK;
+                                if (term & TERM_X) v = wxor(v, row<T>(t.Z, q));
+                                if (term & TERM_Z) v = wxor(v, row<T>(t.X, q));
+                            } else {
+                                v = wxor(v, row<T>(t.REC, term));
+                            }
+                        }
+#pragma unroll
+                        for (int k = 0; k < T; k++) {
+                            if (v.v[k]) atomicXor(&t.OBS[idx * T + k], v.v[k]);
+                        }
+                    }
+                    break;
+                case OP_FLUSH:
+                    if (P.dets.enabled) flush_rows<T>(t.STG, op.n, op.base, P.dets, shot0, P.shots);
+                    break;
+                case OP_FLUSH_OBS:
+                    if (P.obs.enabled) flush_rows<T>(t.OBS, op.n, 0, P.obs, shot0, P.shots);
+                    break;
+                case OP_CHAINCLR:
+                    if (tid < T) t.CHAIN[tid] = 0;
+                    break;
+                case OP_NOISE1: case OP_NOISE2: case OP_MFLIP: case OP_CORR: case OP_HERALD: {
+                    if (P.mode == MODE_FIXED) {
+                        // Faults of this tile are sorted by op; consume the prefix that belongs to this op.
+                        while (true) {
+                            const uint32_t idx = fptr + tid;
+                            bool mine = false;
+                            if (idx < fend) {
+                                const DevFault f = P.faults[idx];
+                                if (f.op == oi) {
+                                    mine = true;
+                                    apply_hit<T>(t, op, ntgt, f.site, f.shot, f.code);
+                                }
+                            }
+                            const uint32_t cnt = __syncthreads_count(mine);
+                            fptr += cnt;
+                            if (cnt < NT) break;
+                        }
+                        break;
+                    }
+                    if (P.mode != MODE_SAMPLED || !(op.p > 0.0f)) break;
+                    const uint32_t nout = op.nout & 0xFFFF;
+                    if (op.code & OPF_DENSE) {
+                        // Dense draw: Bernoulli word per (site, word), then one outcome per fired shot.
+                        for (uint32_t i = tid; i < op.n * T; i += NT) {
+                            const uint32_t site = i / T, wd = i % T;
+                            uint32_t fire = bernoulli_word(op.p, rng, i, oi);
+                            uint32_t ctr = 0;
+                            uint4 r = make_uint4(0, 0, 0, 0);
+                            while (fire) {
+                                const uint32_t b = __ffs(fire) - 1;
+                                fire &= fire - 1;
+                                if ((ctr & 3) == 0) r = rng.draw(i, 0x40000000u | ctr, TAG_NOISE | oi);
+                                const uint32_t rs = (ctr & 3) == 0 ? r.x : (ctr & 3) == 1 ? r.y : (ctr & 3) == 2 ? r.z : r.w;
+                                ctr++;
+                                apply_hit<T>(t, op, ntgt, site, wd * 32 + b, pick_outcome(otab, op.woff, nout, rs));
+                            }
+                        }
+                        break;
+                    }
+                    const uint64_t N = (uint64_t)op.n * W;
+                    const uint32_t clog = (op.nout >> 16) & 0xFF;
+                    const uint64_t C = (uint64_t)1 << clog;
+                    const uint64_t nch = (N + C - 1) >> clog;
+                    for (uint64_t ch = tid; ch < nch; ch += NT) {
+                        const uint64_t lo = ch << clog;
+                        const uint64_t hi = lo + C < N ? lo + C : N;
+                        uint64_t pos = lo;
                         uint32_t ctr = 0;
-                        uint4 r = make_uint4(0, 0, 0, 0);
-                        while (fire) {
-                            const uint32_t b = __ffs(fire) - 1;
-                            fire &= fire - 1;
-                            if ((ctr & 3) == 0) r = rng.draw(i, 0x40000000u | ctr, TAG_NOISE | op.fid);
-                            const uint32_t rs = (ctr & 3) == 0 ? r.x : (ctr & 3) == 1 ? r.y : (ctr & 3) == 2 ? r.z : r.w;
-                            ctr++;
-                            apply_hit<T>(t, op, tgt, site, w * 32 + b, pick_outcome(P.otab, op.woff, nout, rs));
+                        while (true) {
+                            const uint4 r = rng.draw(ctr++, (uint32_t)ch, TAG_NOISE | oi);
+                            pos += geometric_gap(r.x, op.inv_log);
+                            if (pos >= hi) break;
+                            const uint32_t site = (uint32_t)(pos / W), s = (uint32_t)(pos % W);
+                            apply_hit<T>(t, op, ntgt, site, s, pick_outcome(otab, op.woff, nout, r.y));
+                            pos++;
                         }
                     }
                     break;
                 }
-                const uint64_t N = (uint64_t)op.n * W;
-                const uint32_t clog = (op.nout >> 16) & 0xFF;
-                const uint64_t C = (uint64_t)1 << clog;
-                const uint64_t nch = (N + C - 1) >> clog;
-                for (uint64_t ch = tid; ch < nch; ch += NT) {
-                    const uint64_t lo = ch << clog;
-                    const uint64_t hi = lo + C < N ? lo + C : N;
-                    uint64_t pos = lo;
-                    uint32_t ctr = 0;
-                    while (true) {
-                        const uint4 r = rng.draw(ctr++, (uint32_t)ch, TAG_NOISE | op.fid);
-                        pos += geometric_gap(r.x, op.inv_log);
-                        if (pos >= hi) break;
-                        const uint32_t site = (uint32_t)(pos / W), s = (uint32_t)(pos % W);
-                        apply_hit<T>(t, op, tgt, site, s, pick_outcome(P.otab, op.woff, nout, r.y));
-                        pos++;
-                    }
-                }
-                break;
+                default:
+                    break;
             }
-            default:
-                break;
         }
-        if (op.code & OPF_SYNC) __syncthreads();
+        uint32_t *nst = STAGE + (buf ^ 1) * STAGE_WORDS;
+#pragma unroll
+        for (uint32_t k = 0; k < STAGE_WORDS / NT; k++) {
+            const uint32_t idx = tid + k * NT;
+            if (idx < nlen) nst[idx] = pre[k];
+        }
+        __syncthreads();
+        buf ^= 1;
     }
 }
 
@@ -572,25 +622,22 @@ __global__ void pack_rows_kernel(const uint8_t *src, size_t src_stride, uint8_t 
 // Host-side launcher (C++ linkage, called from sampler.cc).
 namespace sa {
 
-hipError_t launch_frame_kernel(int T, const LaunchParams &P, uint32_t num_tiles, size_t lds, hipStream_t stream) {
+hipError_t launch_frame_kernel(int T, const DevProgram &prog, const LaunchParams &P, uint32_t num_tiles, size_t lds,
+                               hipStream_t stream) {
     if (num_tiles == 0) return hipSuccess;
     dim3 grid(num_tiles), block(NT);
+#define STIM_AMD_LAUNCH(TW)                                                                                     \
+    (void)hipFuncSetAttribute((const void *)frame_sim_kernel<TW>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
+                              (int)lds);                                                                         \
+    hipLaunchKernelGGL(frame_sim_kernel<TW>, grid, block, lds, stream, prog.ops, prog.epochs, prog.tgt, prog.ntgt, \
+                       prog.otab, P);
     switch (T) {
-        case 1:
-            (void)hipFuncSetAttribute((const void *)frame_sim_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipLaunchKernelGGL(frame_sim_kernel<1>, grid, block, lds, stream, P);
-            break;
-        case 2:
-            (void)hipFuncSetAttribute((const void *)frame_sim_kernel<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipLaunchKernelGGL(frame_sim_kernel<2>, grid, block, lds, stream, P);
-            break;
-        case 4:
-            (void)hipFuncSetAttribute((const void *)frame_sim_kernel<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-            hipLaunchKernelGGL(frame_sim_kernel<4>, grid, block, lds, stream, P);
-            break;
-        default:
-            return hipErrorInvalidValue;
+        case 1: STIM_AMD_LAUNCH(1) break;
+        case 2: STIM_AMD_LAUNCH(2) break;
+        case 4: STIM_AMD_LAUNCH(4) break;
+        default: return hipErrorInvalidValue;
     }
+#undef STIM_AMD_LAUNCH
     return hipGetLastError();
 }
 

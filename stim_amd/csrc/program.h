@@ -63,16 +63,23 @@ constexpr uint32_t TERM_QMASK = (1u << 24) - 1;
 struct DevOp {
     uint32_t code;   // opcode | flags
     uint32_t n;      // items (sites for noise ops)
-    uint32_t toff;   // offset into the u32 target array
-    uint32_t base;   // record slot base / detector base / observable count / site id base
+    uint32_t toff;   // frame ops: offset into tgt (staged per epoch); noise ops: offset into ntgt
+    uint32_t base;   // record slot base / detector base / term count (CORR)
     float p;         // total firing probability (noise)
     float inv_log;   // 1 / log2(1 - p) for geometric gaps (noise)
-    uint32_t woff;   // offset into the outcome table (pairs of (threshold, code)) / term count for CORR
-    uint32_t nout;   // number of outcomes (noise) ; chunk log2 in bits 16..23
-    uint32_t fid;    // noise stream id (index among noise ops)
-    uint32_t pad[3];
+    uint32_t woff;   // offset into the outcome table (pairs of (threshold, code))
+    uint32_t nout;   // number of outcomes (noise) | chunk log2 << 16
 };
-static_assert(sizeof(DevOp) == 48, "DevOp layout");
+static_assert(sizeof(DevOp) == 32, "DevOp layout");
+
+// A run of ops with no barrier inside. Its frame-op words tgt[tbegin, tbegin + tlen) are copied
+// into an LDS staging buffer before the run starts (prefetched during the previous run).
+struct DevEpoch {
+    uint32_t op_begin, op_end;
+    uint32_t tbegin, tlen;
+};
+
+constexpr uint32_t STAGE_WORDS = 1024;  // staged words per epoch (4 KB; double-buffered)
 
 constexpr uint32_t STAGE_ROWS = 256;   // detectors staged per flush
 
@@ -95,9 +102,6 @@ struct DevOut {
 };
 
 struct LaunchParams {
-    const DevOp *ops;
-    const uint32_t *tgt;
-    const uint32_t *otab;      // outcome table (threshold, code) pairs
     const DevFault *faults;    // fixed-fault mode: all faults of this launch
     const uint32_t *fault_off; // per tile start offsets (num_tiles + 1)
     DevOut dets;
@@ -105,12 +109,21 @@ struct LaunchParams {
     uint64_t seed;
     uint64_t shots;            // valid shots in this launch
     uint64_t tile0;            // global tile index of block 0 (RNG stream)
-    uint32_t n_ops;
+    uint32_t n_epochs;
     uint32_t num_qubits;       // dense qubit rows
     uint32_t ring;             // record ring rows
     uint32_t num_obs;
     uint32_t mode;             // 0 = no noise, 1 = sampled, 2 = fixed faults
     uint32_t randomize;        // 1 = frame randomization
+};
+
+// Device-resident program buffers.
+struct DevProgram {
+    const DevOp *ops;
+    const DevEpoch *epochs;
+    const uint32_t *tgt;       // frame-op words (staged)
+    const uint32_t *ntgt;      // noise-op words (read on hits)
+    const uint32_t *otab;      // outcome table (threshold, code) pairs
 };
 
 constexpr uint32_t MODE_NOISELESS = 0;

@@ -21,7 +21,8 @@
 #include "program.h"
 
 namespace sa {
-hipError_t launch_frame_kernel(int T, const LaunchParams &P, uint32_t num_tiles, size_t lds, hipStream_t stream);
+hipError_t launch_frame_kernel(int T, const DevProgram &prog, const LaunchParams &P, uint32_t num_tiles, size_t lds,
+                               hipStream_t stream);
 hipError_t launch_pack_rows(const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride, uint64_t rows,
                             uint32_t ncols, hipStream_t stream);
 }  // namespace sa
@@ -85,7 +86,7 @@ struct mfs_sampler {
     size_t lds = 0;
     bool randomize = false;
     hipStream_t stream = nullptr;
-    DevBuf d_ops, d_tgt, d_otab, d_faults, d_fault_off, d_out_det, d_out_obs, d_tmp;
+    DevBuf d_ops, d_epochs, d_tgt, d_ntgt, d_otab, d_faults, d_fault_off, d_out_det, d_out_obs, d_tmp;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~mfs_sampler() {
@@ -115,11 +116,8 @@ struct mfs_sampler {
 
     LaunchParams base_params() {
         LaunchParams P{};
-        P.ops = d_ops.as<DevOp>();
-        P.tgt = d_tgt.as<uint32_t>();
-        P.otab = d_otab.as<uint32_t>();
         P.seed = key;
-        P.n_ops = (uint32_t)prog.ops.size();
+        P.n_epochs = (uint32_t)prog.epochs.size();
         P.num_qubits = prog.num_qubits;
         P.ring = prog.ring;
         P.num_obs = prog.num_observables;
@@ -128,7 +126,9 @@ struct mfs_sampler {
 
     void launch(const LaunchParams &P, uint64_t tiles) {
         if (tiles >= (1ull << 31)) throw std::invalid_argument("too many shots in one launch");
-        check(launch_frame_kernel(T, P, (uint32_t)tiles, lds, stream), "frame kernel launch");
+        DevProgram dp{d_ops.as<DevOp>(), d_epochs.as<DevEpoch>(), d_tgt.as<uint32_t>(), d_ntgt.as<uint32_t>(),
+                      d_otab.as<uint32_t>()};
+        check(launch_frame_kernel(T, dp, P, (uint32_t)tiles, lds, stream), "frame kernel launch");
     }
 
     // Output layout of one sample call.
@@ -172,15 +172,17 @@ struct mfs_sampler {
         }
     }
 
+    template <class V>
+    static void upload(DevBuf &b, const std::vector<V> &v, const char *what) {
+        b.ensure(std::max<size_t>(v.size(), 1) * sizeof(V));
+        if (!v.empty()) check(hipMemcpy(b.p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice), what);
+    }
     void upload_program() {
-        d_ops.ensure(prog.ops.size() * sizeof(DevOp));
-        d_tgt.ensure(std::max<size_t>(prog.tgt.size(), 1) * 4);
-        d_otab.ensure(std::max<size_t>(prog.otab.size(), 1) * 4);
-        check(hipMemcpy(d_ops.p, prog.ops.data(), prog.ops.size() * sizeof(DevOp), hipMemcpyHostToDevice), "upload ops");
-        if (!prog.tgt.empty())
-            check(hipMemcpy(d_tgt.p, prog.tgt.data(), prog.tgt.size() * 4, hipMemcpyHostToDevice), "upload targets");
-        if (!prog.otab.empty())
-            check(hipMemcpy(d_otab.p, prog.otab.data(), prog.otab.size() * 4, hipMemcpyHostToDevice), "upload table");
+        upload(d_ops, prog.ops, "upload ops");
+        upload(d_epochs, prog.epochs, "upload epochs");
+        upload(d_tgt, prog.tgt, "upload targets");
+        upload(d_ntgt, prog.ntgt, "upload noise targets");
+        upload(d_otab, prog.otab, "upload outcome table");
     }
 
     // A noiseless run with frame randomization: if every detector and observable stays zero over

@@ -188,7 +188,10 @@ def test_sample_write_formats_match_reference_bytes(golden):
             path = os.path.join(d, 'out')
             obs_path = os.path.join(d, 'obs')
             if mode in ('plain', 'append', 'prepend'):
-                s.sample_write(shots, filepath=path, format=fmt, prepend_observables=mode == 'prepend',
+                # The CLI turns on prepend_observables for the dets format unless observables are
+                # appended (src/stim/cmd/command_detect.cc:45-47); sample_write itself does not.
+                prepend = mode == 'prepend' or (fmt == 'dets' and mode == 'plain')
+                s.sample_write(shots, filepath=path, format=fmt, prepend_observables=prepend,
                                append_observables=mode == 'append')
                 got = open(path, 'rb').read()
             else:
