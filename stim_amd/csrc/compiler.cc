@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstring>
 #include <functional>
 #include <stdexcept>
 #include <string>
@@ -143,7 +144,7 @@ struct Builder {
         while (i < qs.size()) {
             seg_reset();
             DevOp op = make(code | flags, 0);
-            while (i < qs.size() && op.n < STAGE_WORDS && !seg_conflict({qs[i]})) {
+            while (i < qs.size() && op.n < OP_WORDS_MAX && !seg_conflict({qs[i]})) {
                 seg_add({qs[i]});
                 P.tgt.push_back(qs[i]);
                 touch_q(qs[i], KW);
@@ -159,7 +160,7 @@ struct Builder {
         while (i < ps.size()) {
             seg_reset();
             DevOp op = make(code, 0);
-            while (i < ps.size() && 2 * op.n + 2 <= STAGE_WORDS && !seg_conflict({ps[i].first, ps[i].second})) {
+            while (i < ps.size() && 2 * op.n + 2 <= OP_WORDS_MAX && !seg_conflict({ps[i].first, ps[i].second})) {
                 seg_add({ps[i].first, ps[i].second});
                 P.tgt.push_back(ps[i].first);
                 P.tgt.push_back(ps[i].second);
@@ -178,7 +179,7 @@ struct Builder {
         while (i < items.size()) {
             seg_reset();
             DevOp op = make(OP_FEEDBACK, 0);
-            while (i < items.size() && 3 * op.n + 3 <= STAGE_WORDS && !seg_conflict({items[i][1]})) {
+            while (i < items.size() && 3 * op.n + 3 <= OP_WORDS_MAX && !seg_conflict({items[i][1]})) {
                 seg_add({items[i][1]});
                 for (int k = 0; k < 3; k++) P.tgt.push_back(items[i][k]);
                 touch_q(items[i][1], KW);
@@ -250,14 +251,14 @@ struct Builder {
             size_t words = 1;
             while (e < items.size()) {
                 size_t add = 1 + items[e].size();
-                if (words + add > STAGE_WORDS) break;
+                if (words + add > OP_WORDS_MAX) break;
                 if (can_join && !can_join(i, e)) break;
                 words += add;
                 e++;
             }
             if (e == i) {
                 throw std::invalid_argument("a detector, observable or Pauli product has too many terms for the GPU "
-                                            "sampler (limit " + std::to_string(STAGE_WORDS - 2) + ")");
+                                            "sampler (limit " + std::to_string(OP_WORDS_MAX - 2) + ")");
             }
             DevOp op = make(code, (uint32_t)(e - i));
             configure(op, i);
@@ -340,7 +341,7 @@ struct Builder {
             DevOp op = make(code | flags, 0);
             op.base = slot_of(mcount);
             size_t first = i;
-            while (i < n && op.n < STAGE_WORDS) {
+            while (i < n && op.n < OP_WORDS_MAX) {
                 if (slot_of(mcount) == 0 && i > first) break;  // ring wrap
                 if (qubits_of) {
                     std::vector<uint32_t> qs = qubits_of(i);
@@ -525,7 +526,8 @@ struct Builder {
         build_epochs();
     }
 
-    // Groups ops into barrier-free runs whose staged words fit one staging buffer.
+    // Groups ops into barrier-free runs whose blob (header + op headers + words) fits one staging
+    // buffer, then packs the blobs.
     void build_epochs() {
         uint32_t b = 0;
         uint64_t tbegin = 0, tend = 0;
@@ -542,14 +544,17 @@ struct Builder {
         };
         for (uint32_t i = 0; i < (uint32_t)P.ops.size(); i++) {
             const uint32_t len = op_tlen[i];
-            if (len) {
-                const uint64_t s = P.ops[i].toff, e = s + len;
-                if (any && e - tbegin > STAGE_WORDS) {
-                    // Would overflow the staging buffer: barrier before this op.
+            const uint64_t s = P.ops[i].toff, e = s + len;
+            const uint64_t words = (uint64_t)BLOB_HEADER * (i - b + 2) + (len ? e - (any ? tbegin : s) : (any ? tend - tbegin : 0));
+            if (i > b && words > STAGE_WORDS) {
+                // Would overflow the staging buffer: barrier before this op.
+                if (!(P.ops[i - 1].code & OPF_SYNC)) {
                     P.ops[i - 1].code |= OPF_SYNC;
                     P.num_syncs++;
-                    close(i);
                 }
+                close(i);
+            }
+            if (len) {
                 if (!any) {
                     tbegin = s;
                     any = true;
@@ -559,6 +564,37 @@ struct Builder {
             if (P.ops[i].code & OPF_SYNC) close(i + 1);
         }
         if (b < P.ops.size()) close((uint32_t)P.ops.size());
+        // Pack blobs.
+        std::vector<uint32_t> offs(P.epochs.size() + 1, 0);
+        for (size_t e = 0; e < P.epochs.size(); e++) {
+            const DevEpoch &ep = P.epochs[e];
+            offs[e + 1] = offs[e] + BLOB_HEADER * (1 + ep.op_end - ep.op_begin) + ep.tlen;
+            if (offs[e + 1] - offs[e] > STAGE_WORDS) throw std::logic_error("epoch blob exceeds the staging buffer");
+        }
+        P.blob.assign(offs.back(), 0);
+        P.blob0_len = P.epochs.empty() ? 0 : offs[1];
+        for (size_t e = 0; e < P.epochs.size(); e++) {
+            const DevEpoch &ep = P.epochs[e];
+            uint32_t *blob = P.blob.data() + offs[e];
+            const uint32_t nops = ep.op_end - ep.op_begin;
+            blob[0] = nops;
+            blob[1] = offs[e + 1];
+            blob[2] = e + 1 < P.epochs.size() ? offs[e + 2] - offs[e + 1] : 0;
+            blob[3] = ep.op_begin;
+            const uint32_t words0 = BLOB_HEADER * (1 + nops);
+            for (uint32_t j = 0; j < nops; j++) {
+                DevOp op = P.ops[ep.op_begin + j];
+                if (op_tlen[ep.op_begin + j] || !is_noise(op.code & 0xFF)) {
+                    op.toff = op_tlen[ep.op_begin + j] ? words0 + (op.toff - ep.tbegin) : 0;
+                }
+                std::memcpy(blob + BLOB_HEADER * (1 + j), &op, sizeof(DevOp));
+            }
+            for (uint32_t k = 0; k < ep.tlen; k++) blob[words0 + k] = P.tgt[ep.tbegin + k];
+        }
+    }
+
+    static bool is_noise(uint32_t code) {
+        return code == OP_NOISE1 || code == OP_NOISE2 || code == OP_MFLIP || code == OP_CORR || code == OP_HERALD;
     }
 
     void step(const Instr &in, uint32_t k) {

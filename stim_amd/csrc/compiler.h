@@ -20,11 +20,13 @@ struct SiteRange {
 struct Program {
     std::vector<DevOp> ops;
     std::vector<DevEpoch> epochs;
-    std::vector<uint32_t> tgt;    // frame-op words (staged into LDS per epoch)
+    std::vector<uint32_t> tgt;    // frame-op words (before packing into blobs)
+    std::vector<uint32_t> blob;   // epoch blobs (program.h layout) as the device walks them
     std::vector<uint32_t> ntgt;   // noise-op words
     std::vector<uint32_t> otab;
     uint32_t num_qubits = 0;      // dense qubit rows
     uint32_t ring = 1;            // record ring rows
+    uint32_t blob0_len = 0;       // words of the first epoch blob
     uint32_t num_detectors = 0;
     uint32_t num_observables = 0;
     uint64_t num_measurements = 0;

@@ -253,19 +253,20 @@ __device__ __forceinline__ Wd<T> rand_words(const Rng &rng, uint32_t op_i, uint3
 }
 
 template <int T>
-__global__ __launch_bounds__(NT) void frame_sim_kernel(const DevOp *__restrict__ ops, const DevEpoch *__restrict__ epochs,
-                                                       const uint32_t *__restrict__ tgt,
+__global__ __launch_bounds__(NT) void frame_sim_kernel(const uint32_t *__restrict__ blob,
                                                        const uint32_t *__restrict__ ntgt,
                                                        const uint32_t *__restrict__ otab, LaunchParams P) {
     extern __shared__ uint32_t lds[];
+    // LDS layout: two blob staging buffers, then the tile (frame rows, record ring, staged
+    // detector rows, observable rows, chain mask).
+    uint32_t *const STAGE = lds;
     Tile<T> t;
-    t.X = lds;
+    t.X = lds + 2 * STAGE_WORDS;
     t.Z = t.X + P.num_qubits * T;
     t.REC = t.Z + P.num_qubits * T;
     t.STG = t.REC + P.ring * T;
     t.OBS = t.STG + STAGE_ROWS * T;
     t.CHAIN = t.OBS + P.num_obs * T;
-    uint32_t *const STAGE = t.CHAIN + T;
     const uint32_t tid = threadIdx.x;
     const uint32_t W = 32 * T;
     const uint64_t shot0 = (uint64_t)blockIdx.x * W;
@@ -277,10 +278,7 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(const DevOp *__restrict__
         row<T>(t.Z, i) = randomize ? rand_words<T>(rng, 0xFFFFFFFFu, i) : wzero<T>();
     }
     for (uint32_t i = tid; i < (P.num_obs + 1) * T; i += NT) t.OBS[i] = 0;
-    if (P.n_epochs) {
-        const DevEpoch e0 = epochs[0];
-        for (uint32_t i = tid; i < e0.tlen; i += NT) STAGE[i] = tgt[e0.tbegin + i];
-    }
+    for (uint32_t i = tid; i < P.blob0_len; i += NT) STAGE[i] = blob[i];
     __syncthreads();
 
     uint32_t fptr = 0, fend = 0;
@@ -290,30 +288,25 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(const DevOp *__restrict__
     }
 
     uint32_t buf = 0;
-    for (uint32_t e = 0; e < P.n_epochs; e++) {
-        const DevEpoch ep = epochs[e];
-        // Prefetch the next epoch's words into registers; they land in the other staging buffer
-        // after this epoch's ops, so their latency hides behind this epoch's work.
-        uint32_t nlen = 0, nbeg = 0;
-        if (e + 1 < P.n_epochs) {
-            const DevEpoch ne = epochs[e + 1];
-            nlen = ne.tlen;
-            nbeg = ne.tbegin;
-        }
+    while (P.blob0_len) {
+        const uint32_t *cur = STAGE + buf * STAGE_WORDS;
+        const uint32_t n_ops = cur[0], nbeg = cur[1], nlen = cur[2], op0 = cur[3];
+        // Prefetch the next blob into registers; it lands in the other staging buffer after this
+        // epoch's ops, so its latency hides behind this epoch's work.
         uint32_t pre[STAGE_WORDS / NT];
 #pragma unroll
         for (uint32_t k = 0; k < STAGE_WORDS / NT; k++) {
             const uint32_t idx = tid + k * NT;
-            pre[k] = idx < nlen ? tgt[nbeg + idx] : 0u;
+            pre[k] = idx < nlen ? blob[nbeg + idx] : 0u;
         }
-        // st[op.toff + j] reads word j of a frame op from the staging buffer.
-        const uint32_t *st = STAGE + buf * STAGE_WORDS - ep.tbegin;
+        const DevOp *hdrs = reinterpret_cast<const DevOp *>(cur + BLOB_HEADER);
 
-        for (uint32_t oi = ep.op_begin; oi < ep.op_end; oi++) {
-            const DevOp op = ops[oi];
+        for (uint32_t j = 0; j < n_ops; j++) {
+            const DevOp op = hdrs[j];
+            const uint32_t oi = op0 + j;
             const uint32_t code = op.code & 0xFF;
             const bool rnd = randomize && (op.code & OPF_RANDOMIZE);
-            const uint32_t *w = st + op.toff;
+            const uint32_t *w = cur + op.toff;
             switch (code) {
                 case OP_H:
                     for (uint32_t i = tid; i < op.n; i += NT) {
@@ -590,6 +583,7 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(const DevOp *__restrict__
                     break;
             }
         }
+        if (nlen == 0) break;
         uint32_t *nst = STAGE + (buf ^ 1) * STAGE_WORDS;
 #pragma unroll
         for (uint32_t k = 0; k < STAGE_WORDS / NT; k++) {
@@ -629,8 +623,7 @@ hipError_t launch_frame_kernel(int T, const DevProgram &prog, const LaunchParams
 #define STIM_AMD_LAUNCH(TW)                                                                                     \
     (void)hipFuncSetAttribute((const void *)frame_sim_kernel<TW>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
                               (int)lds);                                                                         \
-    hipLaunchKernelGGL(frame_sim_kernel<TW>, grid, block, lds, stream, prog.ops, prog.epochs, prog.tgt, prog.ntgt, \
-                       prog.otab, P);
+    hipLaunchKernelGGL(frame_sim_kernel<TW>, grid, block, lds, stream, prog.blob, prog.ntgt, prog.otab, P);
     switch (T) {
         case 1: STIM_AMD_LAUNCH(1) break;
         case 2: STIM_AMD_LAUNCH(2) break;

@@ -63,7 +63,7 @@ constexpr uint32_t TERM_QMASK = (1u << 24) - 1;
 struct DevOp {
     uint32_t code;   // opcode | flags
     uint32_t n;      // items (sites for noise ops)
-    uint32_t toff;   // frame ops: offset into tgt (staged per epoch); noise ops: offset into ntgt
+    uint32_t toff;   // frame ops: word offset in the epoch blob; noise ops: offset into ntgt
     uint32_t base;   // record slot base / detector base / term count (CORR)
     float p;         // total firing probability (noise)
     float inv_log;   // 1 / log2(1 - p) for geometric gaps (noise)
@@ -72,14 +72,23 @@ struct DevOp {
 };
 static_assert(sizeof(DevOp) == 32, "DevOp layout");
 
-// A run of ops with no barrier inside. Its frame-op words tgt[tbegin, tbegin + tlen) are copied
-// into an LDS staging buffer before the run starts (prefetched during the previous run).
+// A run of ops with no barrier inside (host bookkeeping).
 struct DevEpoch {
     uint32_t op_begin, op_end;
     uint32_t tbegin, tlen;
 };
 
-constexpr uint32_t STAGE_WORDS = 1024;  // staged words per epoch (4 KB; double-buffered)
+// The device walks the program as a chain of epoch blobs. Blob layout (32-bit words):
+//   [0] ops in this epoch  [1] word offset of the next blob  [2] its length (0: last)
+//   [3] global index of this epoch's first op  [4..7] unused
+//   [8 .. 8 + 8*ops)        DevOp headers
+//   [8 + 8*ops .. len)      the frame ops' words (targets, CSR lists); DevOp::toff is relative
+//                           to the blob start.
+// Each blob is copied into one of two LDS staging buffers while the previous epoch runs, so the
+// op loop reads headers and targets from LDS only.
+constexpr uint32_t BLOB_HEADER = 8;
+constexpr uint32_t STAGE_WORDS = 1024;  // staged words per epoch blob (4 KB; double-buffered)
+constexpr uint32_t OP_WORDS_MAX = STAGE_WORDS - 2 * BLOB_HEADER;  // words of one frame op
 
 constexpr uint32_t STAGE_ROWS = 256;   // detectors staged per flush
 
@@ -109,7 +118,7 @@ struct LaunchParams {
     uint64_t seed;
     uint64_t shots;            // valid shots in this launch
     uint64_t tile0;            // global tile index of block 0 (RNG stream)
-    uint32_t n_epochs;
+    uint32_t blob0_len;        // length of the first epoch blob (it starts at word 0)
     uint32_t num_qubits;       // dense qubit rows
     uint32_t ring;             // record ring rows
     uint32_t num_obs;
@@ -119,9 +128,7 @@ struct LaunchParams {
 
 // Device-resident program buffers.
 struct DevProgram {
-    const DevOp *ops;
-    const DevEpoch *epochs;
-    const uint32_t *tgt;       // frame-op words (staged)
+    const uint32_t *blob;      // epoch blobs
     const uint32_t *ntgt;      // noise-op words (read on hits)
     const uint32_t *otab;      // outcome table (threshold, code) pairs
 };

@@ -86,7 +86,7 @@ struct mfs_sampler {
     size_t lds = 0;
     bool randomize = false;
     hipStream_t stream = nullptr;
-    DevBuf d_ops, d_epochs, d_tgt, d_ntgt, d_otab, d_faults, d_fault_off, d_out_det, d_out_obs, d_tmp;
+    DevBuf d_blob, d_ntgt, d_otab, d_faults, d_fault_off, d_out_det, d_out_obs, d_tmp;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~mfs_sampler() {
@@ -117,7 +117,7 @@ struct mfs_sampler {
     LaunchParams base_params() {
         LaunchParams P{};
         P.seed = key;
-        P.n_epochs = (uint32_t)prog.epochs.size();
+        P.blob0_len = prog.blob0_len;
         P.num_qubits = prog.num_qubits;
         P.ring = prog.ring;
         P.num_obs = prog.num_observables;
@@ -126,8 +126,7 @@ struct mfs_sampler {
 
     void launch(const LaunchParams &P, uint64_t tiles) {
         if (tiles >= (1ull << 31)) throw std::invalid_argument("too many shots in one launch");
-        DevProgram dp{d_ops.as<DevOp>(), d_epochs.as<DevEpoch>(), d_tgt.as<uint32_t>(), d_ntgt.as<uint32_t>(),
-                      d_otab.as<uint32_t>()};
+        DevProgram dp{d_blob.as<uint32_t>(), d_ntgt.as<uint32_t>(), d_otab.as<uint32_t>()};
         check(launch_frame_kernel(T, dp, P, (uint32_t)tiles, lds, stream), "frame kernel launch");
     }
 
@@ -178,9 +177,7 @@ struct mfs_sampler {
         if (!v.empty()) check(hipMemcpy(b.p, v.data(), v.size() * sizeof(V), hipMemcpyHostToDevice), what);
     }
     void upload_program() {
-        upload(d_ops, prog.ops, "upload ops");
-        upload(d_epochs, prog.epochs, "upload epochs");
-        upload(d_tgt, prog.tgt, "upload targets");
+        upload(d_blob, prog.blob, "upload program");
         upload(d_ntgt, prog.ntgt, "upload noise targets");
         upload(d_otab, prog.otab, "upload outcome table");
     }
