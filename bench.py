@@ -104,6 +104,17 @@ def main():
 
     # Kernel-only duration from HIP events on the sampler's own stream.
     kernel_ms = sampler.bench_device(S, warmup=1, iters=max(3, min(args.steps, 10)))
+    engine = sampler.engine_info()
+    # The other engine, measured the same way, for comparison.
+    other = 'frame' if engine['engine'] == 'symptom' else 'symptom'
+    engines = {engine['engine']: {'shots_per_s': value, 'kernel_ms': kernel_ms}}
+    try:
+        alt = circuit.compile_detector_sampler(seed=2027)
+        alt.set_engine(other)
+        alt_ms = alt.bench_device(S, warmup=1, iters=3)
+        engines[other] = {'shots_per_s': S / (alt_ms * 1e-3), 'kernel_ms': alt_ms}
+    except ValueError:
+        pass
     b_alg = S * (drow + orow)
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
     info = sampler.info()
@@ -132,8 +143,10 @@ def main():
                    'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'kernel': 'frame_sim_kernel', 'kernel_ms': kernel_ms,
-                     'algorithmic_bytes_per_launch': b_alg},
+                     'kernel': 'sym_sample_kernel' if engine['engine'] == 'symptom' else 'frame_sim_kernel',
+                     'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
+        'engine': engine,
+        'engines': engines,
         'cpu_baseline': cpu,
     }
     print(json.dumps(line), flush=True)

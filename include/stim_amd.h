@@ -86,6 +86,18 @@ int mfs_sample_with_faults(mfs_sampler *s, uint64_t shots, const uint64_t *shot_
  * with HIP events on the sampler's stream (out_ms[0] = frame kernel, out_ms[1] = whole call). */
 int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, double out_ms[2]);
 
+/* Sampling engine: 0 = automatic, 1 = frame engine (the reference's per-instruction frame
+ * simulation, FrameSimulator::do_circuit, frame_simulator.inl:165-170), 2 = symptom engine
+ * (frame-kernel symptoms of every basis fault + sparse noise sampling; same distribution,
+ * deterministic-detector circuits only). Automatic picks the symptom engine when it applies.
+ * No reference counterpart (the reference has one engine). */
+int mfs_set_engine(mfs_sampler *s, int engine);
+
+/* out[0..5] = engine in use for sampling (1 frame, 2 symptom), site classes, symptom columns,
+ * symptom entries, symptom-table build time in microseconds, noise sites. Builds the symptom
+ * table if needed. */
+int mfs_engine_info(mfs_sampler *s, uint64_t out[6]);
+
 /* Configuration knobs: words per tile (1, 2 or 4; 0 = automatic). Returns the value in use. */
 int mfs_set_tile_words(mfs_sampler *s, int words);
 int mfs_get_info(const mfs_sampler *s, uint64_t out[8]); /* tile_words, lds_bytes, ops, syncs, noise_ops, randomize, ring, dense_qubits */

@@ -559,8 +559,13 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(const uint32_t *__restric
                         }
                         break;
                     }
+                    // The tile's (site, shot) positions are cut into chunks spread over all lanes (at
+                    // most ~4/p positions each); each chunk walks its own geometric gaps, two hits per
+                    // Philox block (gap, outcome, gap, outcome).
                     const uint64_t N = (uint64_t)op.n * W;
-                    const uint32_t clog = (op.nout >> 16) & 0xFF;
+                    const uint32_t cmax = (op.nout >> 16) & 0xFF;
+                    uint32_t clog = 6;
+                    while (clog < cmax + 2 && ((uint64_t)NT << clog) < N) clog++;
                     const uint64_t C = (uint64_t)1 << clog;
                     const uint64_t nch = (N + C - 1) >> clog;
                     for (uint64_t ch = tid; ch < nch; ch += NT) {
@@ -572,8 +577,12 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(const uint32_t *__restric
                             const uint4 r = rng.draw(ctr++, (uint32_t)ch, TAG_NOISE | oi);
                             pos += geometric_gap(r.x, op.inv_log);
                             if (pos >= hi) break;
-                            const uint32_t site = (uint32_t)(pos / W), s = (uint32_t)(pos % W);
-                            apply_hit<T>(t, op, ntgt, site, s, pick_outcome(otab, op.woff, nout, r.y));
+                            apply_hit<T>(t, op, ntgt, (uint32_t)(pos / W), (uint32_t)(pos % W),
+                                         pick_outcome(otab, op.woff, nout, r.y));
+                            pos += 1 + geometric_gap(r.z, op.inv_log);
+                            if (pos >= hi) break;
+                            apply_hit<T>(t, op, ntgt, (uint32_t)(pos / W), (uint32_t)(pos % W),
+                                         pick_outcome(otab, op.woff, nout, r.w));
                             pos++;
                         }
                     }

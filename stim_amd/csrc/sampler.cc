@@ -19,12 +19,22 @@
 #include "circuit.h"
 #include "compiler.h"
 #include "program.h"
+#include "symptom.h"
 
 namespace sa {
 hipError_t launch_frame_kernel(int T, const DevProgram &prog, const LaunchParams &P, uint32_t num_tiles, size_t lds,
                                hipStream_t stream);
 hipError_t launch_pack_rows(const uint8_t *src, size_t src_stride, uint8_t *dst, size_t dst_stride, uint64_t rows,
                             uint32_t ncols, hipStream_t stream);
+hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream);
+hipError_t launch_sym_fixed(const SymDevParams &P, const uint64_t *off, const uint32_t *cols, uint32_t tiles,
+                            size_t lds, hipStream_t stream);
+hipError_t launch_sym_count(const uint8_t *det, uint32_t det_row, const uint8_t *obs, uint32_t obs_row, uint64_t rows,
+                            uint32_t *counts, hipStream_t stream);
+hipError_t launch_sym_fill(const uint8_t *det, uint32_t det_row, const uint8_t *obs, uint32_t obs_row, uint64_t rows,
+                           uint32_t D, const uint32_t *offs, uint32_t *ent, hipStream_t stream);
+hipError_t launch_unpack_rows(const uint8_t *src, uint64_t src_stride, uint8_t *dst, uint64_t dst_stride, uint64_t rows,
+                              uint32_t ncols, uint32_t base_char, hipStream_t stream);
 }  // namespace sa
 
 using namespace sa;
@@ -87,6 +97,14 @@ struct mfs_sampler {
     bool randomize = false;
     hipStream_t stream = nullptr;
     DevBuf d_blob, d_ntgt, d_otab, d_faults, d_fault_off, d_out_det, d_out_obs, d_tmp;
+    // Symptom engine state (built on first use).
+    int engine_mode = 0;         // 0 automatic, 1 frame engine, 2 symptom engine
+    int sym_state = 0;           // 0 not built, 1 ready, -1 unsupported
+    std::string sym_why;
+    SymPlan plan;
+    uint64_t sym_entries = 0;
+    double sym_build_ms = 0;
+    DevBuf d_classes, d_sotab, d_col_off, d_ent, d_sym_tmp, d_fix_off, d_fix_cols;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~mfs_sampler() {
@@ -221,6 +239,11 @@ struct mfs_sampler {
         }
         const bool device_out = flags & MFS_DEVICE_OUTPUT;
         if (shots == 0) return;
+        if (use_symptoms(faults != nullptr)) {
+            sample_symptom(L, shots, device_out, dets, dets_stride, obs, obs_stride, fault_offsets, faults, base_char,
+                           newline);
+            return;
+        }
         const uint64_t W64 = W();
         const size_t row_bytes = L.det_row + (L.separate ? L.obs_row : 0) + (L.via_unpacked ? L.ncols : 0);
         uint64_t batch = device_out && !L.via_unpacked ? shots
@@ -302,6 +325,277 @@ struct mfs_sampler {
         }
         if (device_out) check(hipStreamSynchronize(stream), "sample");
         tiles_used += (shots + W64 - 1) / W64;
+    }
+
+    // ------------------------------------------------------------------ symptom engine
+    // Builds the symptom table: one fixed-fault frame-kernel shot per basis fault, then the set
+    // bits of each shot's detector/observable row become that column's sparse symptom.
+    bool ensure_symptoms() {
+        if (sym_state) return sym_state > 0;
+        plan = plan_symptoms(prog);
+        if (!plan.supported) {
+            sym_state = -1;
+            sym_why = plan.why_unsupported;
+            return false;
+        }
+        const uint64_t F = plan.columns.size();
+        Layout L = layout(MFS_BIT_PACKED | MFS_SEPARATE_OBSERVABLES);
+        const size_t drow = std::max<size_t>(L.det_row, 1), orow = std::max<size_t>(L.obs_row, 1);
+        const uint64_t W64 = W();
+        uint64_t batch = std::max<uint64_t>(W64, ((size_t)2 << 30) / (drow + orow) / W64 * W64);
+        std::vector<uint32_t> offs(F + 1, 0);
+        std::vector<uint32_t> ents;
+        hipEvent_t e0, e1;
+        check(hipEventCreate(&e0), "event");
+        check(hipEventCreate(&e1), "event");
+        check(hipEventRecord(e0, stream), "event");
+        for (uint64_t c0 = 0; c0 < F; c0 += batch) {
+            const uint64_t n = std::min(batch, F - c0);
+            const uint64_t tiles = (n + W64 - 1) / W64;
+            // Fixed faults: shot i of the batch carries column c0 + i's basis fault.
+            std::vector<DevFault> fs(n);
+            std::vector<uint32_t> fo(tiles + 1, 0);
+            for (uint64_t t = 0; t < tiles; t++) {
+                fo[t] = (uint32_t)(t * W64);
+                const uint64_t e = std::min(n, (t + 1) * W64);
+                for (uint64_t i = t * W64; i < e; i++) {
+                    const SymColumn &cdef = plan.columns[c0 + i];
+                    fs[i] = DevFault{cdef.op, cdef.site, (uint32_t)(i - t * W64), cdef.code};
+                }
+                std::stable_sort(fs.begin() + (long)(t * W64), fs.begin() + (long)e,
+                                 [](const DevFault &a, const DevFault &b) { return a.op < b.op; });
+            }
+            fo[tiles] = (uint32_t)n;
+            d_faults.ensure(fs.size() * sizeof(DevFault));
+            d_fault_off.ensure(fo.size() * 4);
+            check(hipMemcpyAsync(d_faults.p, fs.data(), fs.size() * sizeof(DevFault), hipMemcpyHostToDevice, stream), "faults");
+            check(hipMemcpyAsync(d_fault_off.p, fo.data(), fo.size() * 4, hipMemcpyHostToDevice, stream), "fault offsets");
+            d_out_det.ensure(n * drow);
+            d_out_obs.ensure(n * orow);
+            LaunchParams P = base_params();
+            outputs(L, d_out_det.as<uint8_t>(), drow, d_out_obs.as<uint8_t>(), orow, P);
+            P.shots = n;
+            P.tile0 = 0;
+            P.mode = MODE_FIXED;
+            P.randomize = 0;
+            P.faults = d_faults.as<DevFault>();
+            P.fault_off = d_fault_off.as<uint32_t>();
+            if (L.det_row == 0) P.dets.enabled = 0;
+            if (L.obs_row == 0) P.obs.enabled = 0;
+            launch(P, tiles);
+            d_sym_tmp.ensure((n + 1) * 4);
+            check(launch_sym_count(d_out_det.as<uint8_t>(), (uint32_t)L.det_row, d_out_obs.as<uint8_t>(),
+                                   (uint32_t)L.obs_row, n, d_sym_tmp.as<uint32_t>(), stream), "symptom count");
+            std::vector<uint32_t> counts(n);
+            check(hipMemcpyAsync(counts.data(), d_sym_tmp.p, n * 4, hipMemcpyDeviceToHost, stream), "counts");
+            check(hipStreamSynchronize(stream), "symptom count");
+            std::vector<uint32_t> local(n + 1, 0);
+            for (uint64_t i = 0; i < n; i++) local[i + 1] = local[i] + counts[i];
+            for (uint64_t i = 0; i < n; i++) offs[c0 + i + 1] = offs[c0 + i] + counts[i];
+            d_sym_tmp.ensure((n + 1) * 4);
+            check(hipMemcpyAsync(d_sym_tmp.p, local.data(), (n + 1) * 4, hipMemcpyHostToDevice, stream), "offsets");
+            DevBuf bent;
+            bent.ensure(std::max<size_t>(local[n], 1) * 4);
+            check(launch_sym_fill(d_out_det.as<uint8_t>(), (uint32_t)L.det_row, d_out_obs.as<uint8_t>(),
+                                  (uint32_t)L.obs_row, n, prog.num_detectors, d_sym_tmp.as<uint32_t>(),
+                                  bent.as<uint32_t>(), stream), "symptom fill");
+            const size_t base = ents.size();
+            ents.resize(base + local[n]);
+            if (local[n]) check(hipMemcpyAsync(ents.data() + base, bent.p, local[n] * 4, hipMemcpyDeviceToHost, stream), "ents");
+            check(hipStreamSynchronize(stream), "symptom fill");
+        }
+        check(hipEventRecord(e1, stream), "event");
+        check(hipEventSynchronize(e1), "event");
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, e0, e1), "event");
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        sym_build_ms = ms;
+        sym_entries = ents.size();
+        upload(d_classes, plan.classes, "classes");
+        upload(d_sotab, plan.otab, "symptom outcome table");
+        upload(d_col_off, offs, "column offsets");
+        upload(d_ent, ents, "symptom entries");
+        sym_state = 1;
+        return true;
+    }
+
+    bool use_symptoms(bool fixed) {
+        if (engine_mode == 1) return false;
+        if ((prog.num_detectors + 2ull * prog.num_observables) / 8 > 64 * 1024) {
+            if (engine_mode == 2) throw std::invalid_argument("rows too wide for the symptom engine's LDS image");
+            return false;
+        }
+        if (fixed && engine_mode != 2) return false;
+        if (randomize) {
+            if (engine_mode == 2) throw std::invalid_argument(
+                "the symptom engine needs deterministic detectors and observables; this circuit has random ones");
+            return false;
+        }
+        const bool ok = ensure_symptoms();
+        if (!ok && engine_mode == 2) throw std::invalid_argument("symptom engine unavailable: " + sym_why);
+        return ok;
+    }
+
+    // Shots per symptom tile: LDS image of about 32 KB.
+    uint32_t sym_tile_shots(size_t row_bytes) const {
+        uint32_t S = 1;
+        while (S < 1024 && (size_t)(2 * S) * row_bytes <= 32 * 1024) S *= 2;
+        return S;
+    }
+
+    SymDevParams sym_params(uint32_t S) {
+        SymDevParams P{};
+        P.classes = d_classes.as<SymClass>();
+        P.otab = d_sotab.as<uint32_t>();
+        P.col_off = d_col_off.as<uint32_t>();
+        P.ent = d_ent.as<uint32_t>();
+        P.nclasses = (uint32_t)plan.classes.size();
+        P.D = prog.num_detectors;
+        P.O = prog.num_observables;
+        P.shots_per_tile = S;
+        uint32_t lg = 0;
+        while ((1u << lg) < S) lg++;
+        P.tile_log2 = lg;
+        P.total_sites = plan.total_sites;
+        P.seed = splitmix64(key ^ 0x5A5A5A5Aull);
+        return P;
+    }
+
+    // Sampled (or fixed-fault) shots through the symptom engine into packed device rows.
+    void sym_launch(const Layout &L, uint64_t n, uint8_t *det_dst, size_t det_stride, uint8_t *obs_dst,
+                    size_t obs_stride, uint64_t tile0, const uint64_t *fix_off, const uint32_t *fix_cols) {
+        const uint32_t D = prog.num_detectors, O = prog.num_observables;
+        const size_t obs_bytes = (L.separate && obs_dst) ? (O + 7) / 8 : 0;
+        const size_t det_bytes = (L.ncols + 7) / 8;
+        const uint32_t S = sym_tile_shots(std::max<size_t>(det_bytes + obs_bytes, 1));
+        SymDevParams P = sym_params(S);
+        P.shots = n;
+        P.tile0 = tile0;
+        P.dets = DevOut{det_dst, det_stride, L.prepend ? O : 0u, 1, 0, det_bytes ? 1u : 0u};
+        P.det_row_bytes = (uint32_t)det_bytes;
+        P.obs_mode = (L.prepend || L.append) ? 2u : obs_bytes ? 1u : 0u;
+        P.obs_col0 = L.append ? D : 0;
+        P.obs = DevOut{obs_dst, obs_stride, 0, 1, 0, obs_bytes ? 1u : 0u};
+        P.obs_row_bytes = (uint32_t)obs_bytes;
+        const size_t lds = ((size_t)S * det_bytes + 15) / 16 * 16 + ((size_t)S * obs_bytes + 15) / 16 * 16;
+        const uint64_t tiles = (n + S - 1) / S;
+        if (tiles >= (1ull << 31)) throw std::invalid_argument("too many shots in one launch");
+        if (fix_off) {
+            check(launch_sym_fixed(P, fix_off, fix_cols, (uint32_t)tiles, lds, stream), "symptom fixed launch");
+        } else {
+            check(launch_sym_sample(P, (uint32_t)tiles, lds, stream), "symptom sample launch");
+        }
+    }
+
+    void sample_symptom(const Layout &L, uint64_t shots, bool device_out, uint8_t *dets, size_t dets_stride,
+                        uint8_t *obs, size_t obs_stride, const uint64_t *fault_offsets, const uint32_t *faults,
+                        char base_char, bool newline) {
+        const uint32_t O = prog.num_observables;
+        const size_t pdet = (L.ncols + 7) / 8, pobs = (O + 7) / 8;
+        const bool want_obs = L.separate && obs && O;
+        const uint64_t batch = device_out && L.packed
+                                   ? shots
+                                   : std::max<uint64_t>(1024, HOST_BATCH_BYTES / std::max<size_t>(pdet + pobs + L.ncols + O, 1));
+        const uint64_t call_tile0 = tiles_used;
+        uint64_t tiles_total = 0;
+        for (uint64_t s0 = 0; s0 < shots; s0 += batch) {
+            const uint64_t n = std::min(batch, shots - s0);
+            // Packed destination rows for this batch.
+            uint8_t *pd, *po = nullptr;
+            size_t pd_stride, po_stride = 0;
+            if (L.packed && device_out) {
+                pd = dets + s0 * dets_stride;
+                pd_stride = dets_stride;
+                if (want_obs) {
+                    po = obs + s0 * obs_stride;
+                    po_stride = obs_stride;
+                }
+            } else {
+                d_out_det.ensure(n * std::max<size_t>(pdet, 1));
+                pd = d_out_det.as<uint8_t>();
+                pd_stride = pdet;
+                if (want_obs) {
+                    d_out_obs.ensure(n * std::max<size_t>(pobs, 1));
+                    po = d_out_obs.as<uint8_t>();
+                    po_stride = pobs;
+                }
+            }
+            const uint64_t *fo = nullptr;
+            const uint32_t *fc = nullptr;
+            if (faults) stage_symptom_faults(s0, n, fault_offsets, faults, &fo, &fc);
+            sym_launch(L, n, pd, pd_stride, po, po_stride, call_tile0 + tiles_total, fo, fc);
+            tiles_total += (n + 1023) / 1024 + 1;
+            if (!L.packed) {
+                // Expand bits to bytes (bool) or characters.
+                uint8_t *ud = device_out ? dets + s0 * dets_stride : nullptr;
+                uint8_t *uo = device_out && want_obs ? obs + s0 * obs_stride : nullptr;
+                size_t ud_stride = dets_stride, uo_stride = obs_stride;
+                if (!device_out) {
+                    d_tmp.ensure(n * std::max<size_t>(L.ncols + O, 1));
+                    ud = d_tmp.as<uint8_t>();
+                    ud_stride = L.ncols;
+                    uo = d_tmp.as<uint8_t>() + n * L.ncols;
+                    uo_stride = O;
+                }
+                check(launch_unpack_rows(pd, pd_stride, ud, ud_stride, n, L.ncols, (uint8_t)base_char, stream), "unpack");
+                if (want_obs) check(launch_unpack_rows(po, po_stride, uo, uo_stride, n, O, (uint8_t)base_char, stream), "unpack");
+                pd = ud;
+                pd_stride = ud_stride;
+                po = uo;
+                po_stride = uo_stride;
+            }
+            if (!device_out) {
+                const size_t drow = L.packed ? pdet : L.ncols, orow = L.packed ? pobs : O;
+                if (drow && dets)
+                    check(hipMemcpy2DAsync(dets + s0 * dets_stride, dets_stride, pd, pd_stride, drow, n,
+                                           hipMemcpyDeviceToHost, stream), "copy dets");
+                if (want_obs && orow)
+                    check(hipMemcpy2DAsync(obs + s0 * obs_stride, obs_stride, po, po_stride, orow, n,
+                                           hipMemcpyDeviceToHost, stream), "copy obs");
+                check(hipStreamSynchronize(stream), "sample");
+                if (newline) {
+                    for (uint64_t i = 0; i < n; i++) dets[(s0 + i) * dets_stride + L.ncols] = '\n';
+                }
+            }
+        }
+        check(hipStreamSynchronize(stream), "sample");
+        tiles_used += tiles_total + (shots + 1023) / 1024;
+    }
+
+    // Maps fixed faults to symptom columns (per shot, the basis columns of each fault's code).
+    void stage_symptom_faults(uint64_t s0, uint64_t n, const uint64_t *off, const uint32_t *faults,
+                              const uint64_t **fo, const uint32_t **fc) {
+        std::vector<uint64_t> offs(n + 1, 0);
+        std::vector<uint32_t> cols;
+        for (uint64_t s = 0; s < n; s++) {
+            for (uint64_t i = off[s0 + s]; i < off[s0 + s + 1]; i++) {
+                uint32_t op, site;
+                resolve_fault(prog, faults[3 * i], faults[3 * i + 1], &op, &site);
+                const uint32_t c = plan.op_class[op];
+                if (c == 0xFFFFFFFFu) {
+                    throw std::invalid_argument("the symptom engine has no column for a fault at a zero-probability site");
+                }
+                const SymClass &cl = plan.classes[c];
+                const uint32_t code = faults[3 * i + 2];
+                if ((prog.ops[op].code & 0xFF) == OP_CORR) {
+                    if (code & 1) cols.push_back(cl.col0 + plan.op_link[op]);
+                    continue;
+                }
+                const uint32_t cs = plan.op_site0[op] + site;
+                for (uint32_t j = 0; j < cl.basis; j++) {
+                    if (code & (1u << j)) cols.push_back(cl.col0 + cs * cl.basis + j);
+                }
+            }
+            offs[s + 1] = cols.size();
+        }
+        d_fix_off.ensure(offs.size() * 8);
+        d_fix_cols.ensure(std::max<size_t>(cols.size(), 1) * 4);
+        check(hipMemcpyAsync(d_fix_off.p, offs.data(), offs.size() * 8, hipMemcpyHostToDevice, stream), "fault offsets");
+        if (!cols.empty())
+            check(hipMemcpyAsync(d_fix_cols.p, cols.data(), cols.size() * 4, hipMemcpyHostToDevice, stream), "fault cols");
+        *fo = d_fix_off.as<uint64_t>();
+        *fc = d_fix_cols.as<uint32_t>();
     }
 
     void stage_faults(uint64_t s0, uint64_t n, uint64_t tiles, const uint64_t *off, const uint32_t *faults,
@@ -610,7 +904,14 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
             check(hipEventCreate(&s->ev1), "event");
         }
         const uint64_t tiles = (shots + s->W() - 1) / s->W();
+        const bool sym = s->use_symptoms(false);
         auto one = [&]() {
+            if (sym) {
+                s->sym_launch(L, shots, dd.as<uint8_t>(), L.det_row, od.as<uint8_t>(), L.obs_row, s->tiles_used, nullptr,
+                              nullptr);
+                s->tiles_used += shots / 1024 + 1;
+                return;
+            }
             LaunchParams P = s->base_params();
             s->outputs(L, dd.as<uint8_t>(), L.det_row, od.as<uint8_t>(), L.obs_row, P);
             P.shots = shots;
@@ -644,6 +945,26 @@ int mfs_set_tile_words(mfs_sampler *s, int words) {
     });
     if (rc) return -rc;
     return s->T;
+}
+
+int mfs_set_engine(mfs_sampler *s, int engine) {
+    return guarded([&] {
+        if (engine < 0 || engine > 2) throw std::invalid_argument("engine must be 0 (auto), 1 (frame) or 2 (symptom)");
+        s->engine_mode = engine;
+        if (engine == 2 && !s->use_symptoms(false)) throw std::invalid_argument("symptom engine unavailable");
+    });
+}
+
+int mfs_engine_info(mfs_sampler *s, uint64_t out[6]) {
+    return guarded([&] {
+        const bool sym = s->use_symptoms(false);
+        out[0] = sym ? 2 : 1;
+        out[1] = s->plan.classes.size();
+        out[2] = s->plan.columns.size();
+        out[3] = s->sym_entries;
+        out[4] = (uint64_t)(s->sym_build_ms * 1000.0);
+        out[5] = s->plan.total_sites;
+    });
 }
 
 int mfs_get_info(const mfs_sampler *s, uint64_t out[8]) {

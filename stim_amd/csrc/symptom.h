@@ -1,0 +1,78 @@
+// Sparse-symptom sampling engine.
+//
+// The frame rules are linear over GF(2) (src/stim/simulators/frame_simulator.inl:319-630 only XOR
+// rows; measurements XOR frame rows into the record; detectors and observables XOR record and frame
+// rows), so a shot's detection events are the XOR of the symptoms of the noise outcomes that fired
+// in it. This engine:
+//   1. runs the frame kernel in fixed-fault mode once per *basis fault* of every noise site (one
+//      shot column per basis fault: X or Z on each qubit of a channel, a record flip, a herald, a
+//      correlated-chain link) to get each basis fault's detector/observable symptom;
+//   2. keeps the symptoms as sparse column lists in HBM;
+//   3. samples shots by drawing the channels exactly as the reference does (Bernoulli(p) per site via
+//      geometric gaps, RareErrorIterator src/stim/util_bot/probability_util.h:33-71; then the
+//      outcome from the channel's disjoint distribution) and XORs the chosen outcome's basis
+//      symptoms into an LDS image of the shot-major output rows, which is streamed to HBM.
+// It is used for circuits whose detectors and observables are deterministic (frame randomization
+// cannot change any output); the frame engine samples the others.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "compiler.h"
+
+namespace sa {
+
+struct SymClass {
+    float p;         // firing probability of one site
+    float inv_log;   // 1 / log2(1 - p)
+    uint32_t nsites;
+    uint32_t basis;  // basis columns per site
+    uint32_t col0;   // first column of the class
+    uint32_t woff;   // outcome table offset ((threshold, code-bitmask) pairs)
+    uint32_t nout;
+    uint32_t site0;  // cumulative sites of earlier classes
+};
+static_assert(sizeof(SymClass) == 32, "SymClass layout");
+
+// Basis fault of one column, as a fixed-fault entry for the frame kernel.
+struct SymColumn {
+    uint32_t op, site, code;
+};
+
+struct SymPlan {
+    std::vector<SymClass> classes;
+    std::vector<uint32_t> otab;        // outcome tables (threshold, basis-bitmask code)
+    std::vector<SymColumn> columns;    // column -> basis fault
+    uint64_t total_sites = 0;
+    // op index -> (class, first class-site of the op's sites) for fixed-fault XOR; chain links map
+    // to (class, link index) via chain_link.
+    std::vector<uint32_t> op_class, op_site0, op_link;
+    bool supported = true;
+    std::string why_unsupported;
+};
+
+SymPlan plan_symptoms(const Program &prog);
+
+struct SymDevParams {
+    const SymClass *classes;
+    const uint32_t *otab;
+    const uint32_t *col_off;
+    const uint32_t *ent;
+    uint32_t nclasses;
+    uint32_t D, O;
+    uint32_t shots_per_tile;       // power of two
+    uint32_t tile_log2;
+    uint64_t total_sites;
+    uint64_t shots;                // valid shots in this launch
+    uint64_t tile0;
+    uint64_t seed;
+    DevOut dets;                   // packed rows (col0 in bits)
+    DevOut obs;                    // packed rows; enabled=0 when observables share the dets rows
+    uint32_t det_row_bytes;        // image pitch of the dets rows
+    uint32_t obs_row_bytes;        // image pitch of separate observable rows (0 when unused)
+    uint32_t obs_mode;             // 0: observables dropped, 1: separate image, 2: dets image at obs_col0
+    uint32_t obs_col0;
+};
+
+}  // namespace sa

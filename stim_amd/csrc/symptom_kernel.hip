@@ -1,0 +1,306 @@
+// Device side of the sparse-symptom engine (see symptom.h).
+//
+// sym_sample_kernel: one workgroup per tile of S shots. The tile's noise positions are
+// (site, shot) pairs over all site classes, concatenated class after class; lane t walks the
+// contiguous range [t*R, (t+1)*R) drawing geometric gaps with the class's probability (restarting
+// at class boundaries, exact because gaps are memoryless), picks each hit's outcome from the class's
+// disjoint distribution, and XORs the outcome's basis symptoms into an LDS image of the tile's
+// shot-major output rows (byte-exact copy of the destination rows, bit m of shot s at byte m/8, bit
+// m%8 as in src/stim/py/numpy.pybind.cc:20-63). The image is then streamed to HBM with 16-byte stores.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "program.h"
+#include "symptom.h"
+
+using namespace sa;
+
+namespace {
+
+constexpr int NT = 256;
+
+__device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; r++) {
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
+        const uint32_t lo0 = 0xD2511F53u * c.x;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
+        const uint32_t lo1 = 0xCD9E8D57u * c.z;
+        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    return c;
+}
+
+constexpr uint32_t TAG_SYM = 5u << 28;
+
+__device__ __forceinline__ uint64_t gap(uint32_t r, float inv_log2_1mp) {
+    const float u = ((float)(r >> 8) + 1.0f) * (1.0f / 16777216.0f);
+    const float g = __log2f(u) * inv_log2_1mp;
+    if (!(g < 1.0e12f)) return (uint64_t)1 << 40;
+    return (uint64_t)g;
+}
+
+__device__ __forceinline__ uint32_t outcome(const uint32_t *__restrict__ otab, uint32_t woff, uint32_t nout,
+                                            uint32_t r) {
+    for (uint32_t i = 0; i + 1 < nout; i++) {
+        if (r <= otab[woff + 2 * i]) return otab[woff + 2 * i + 1];
+    }
+    return otab[woff + 2 * (nout - 1) + 1];
+}
+
+struct Images {
+    uint32_t *det;   // dets image (u32 words)
+    uint32_t *obs;   // separate observable image
+};
+
+// XORs column `col`'s symptom into shot `shot` of the images.
+__device__ __forceinline__ void xor_column(const SymDevParams &P, const Images &im, uint32_t col, uint32_t shot) {
+    const uint32_t b = P.col_off[col], e = P.col_off[col + 1];
+    for (uint32_t k = b; k < e; k++) {
+        const uint32_t m = P.ent[k];
+        uint32_t bit;
+        uint32_t *img;
+        if (m < P.D) {
+            bit = shot * (P.det_row_bytes * 8) + P.dets.col0 + m;
+            img = im.det;
+        } else if (P.obs_mode == 2) {
+            bit = shot * (P.det_row_bytes * 8) + P.obs_col0 + (m - P.D);
+            img = im.det;
+        } else if (P.obs_mode == 1) {
+            bit = shot * (P.obs_row_bytes * 8) + (m - P.D);
+            img = im.obs;
+        } else {
+            continue;
+        }
+        atomicXor(&img[bit >> 5], 1u << (bit & 31));
+    }
+}
+
+// Copies `rows` image rows (pitch `pitch` bytes) to dst rows of stride `stride`.
+__device__ void write_rows(const uint8_t *img, uint32_t pitch, uint32_t rows, uint8_t *dst, uint64_t stride) {
+    const uint32_t tid = threadIdx.x;
+    if (stride == pitch) {
+        const uint32_t total = rows * pitch;
+        const uintptr_t a = (uintptr_t)dst;
+        if ((a & 15) == 0) {
+            const uint32_t n16 = total / 16;
+            for (uint32_t i = tid; i < n16; i += NT) {
+                const uint4 v = *reinterpret_cast<const uint4 *>(img + 16 * i);
+                __builtin_nontemporal_store(v.x, reinterpret_cast<uint32_t *>(dst + 16 * i));
+                __builtin_nontemporal_store(v.y, reinterpret_cast<uint32_t *>(dst + 16 * i + 4));
+                __builtin_nontemporal_store(v.z, reinterpret_cast<uint32_t *>(dst + 16 * i + 8));
+                __builtin_nontemporal_store(v.w, reinterpret_cast<uint32_t *>(dst + 16 * i + 12));
+            }
+            for (uint32_t i = 16 * n16 + tid; i < total; i += NT) dst[i] = img[i];
+        } else {
+            for (uint32_t i = tid; i < total; i += NT) dst[i] = img[i];
+        }
+    } else {
+        for (uint32_t i = tid; i < rows * pitch; i += NT) {
+            const uint32_t r = i / pitch, c = i % pitch;
+            dst[r * stride + c] = img[i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t S = P.shots_per_tile;
+    const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
+    const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
+    Images im{lds, lds + det_words};
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < det_words + obs_words; i += NT) lds[i] = 0;
+    __syncthreads();
+
+    const uint64_t s0 = (uint64_t)blockIdx.x * S;
+    const uint32_t tile = (uint32_t)(P.tile0 + blockIdx.x);
+    const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
+    const uint64_t total = P.total_sites << P.tile_log2;
+    const uint64_t R = (total + NT - 1) / NT;
+    uint64_t pos = (uint64_t)tid * R;
+    const uint64_t hi = pos + R < total ? pos + R : total;
+    if (pos < hi) {
+        // Class containing pos (binary search over cumulative sites).
+        uint32_t lo_c = 0, hi_c = P.nclasses;
+        const uint64_t site_at = pos >> P.tile_log2;
+        while (hi_c - lo_c > 1) {
+            const uint32_t mid = (lo_c + hi_c) / 2;
+            if (P.classes[mid].site0 <= site_at) lo_c = mid;
+            else hi_c = mid;
+        }
+        uint32_t c = lo_c;
+        uint32_t ctr = 0;
+        while (pos < hi && c < P.nclasses) {
+            const SymClass cl = P.classes[c];
+            const uint64_t cbeg = (uint64_t)cl.site0 << P.tile_log2;
+            const uint64_t cend_full = cbeg + ((uint64_t)cl.nsites << P.tile_log2);
+            const uint64_t cend = cend_full < hi ? cend_full : hi;
+            if (cl.p > 0.0f) {
+                while (true) {
+                    const uint4 r = philox10(make_uint4(ctr++, tid, tile, TAG_SYM), k0, k1);
+                    pos += gap(r.x, cl.inv_log);
+                    if (pos >= cend) break;
+                    {
+                        const uint64_t local = pos - cbeg;
+                        const uint32_t site = (uint32_t)(local >> P.tile_log2), shot = (uint32_t)(local & (S - 1));
+                        uint32_t code = outcome(P.otab, cl.woff, cl.nout, r.y);
+                        while (code) {
+                            const uint32_t j = __ffs(code) - 1;
+                            code &= code - 1;
+                            xor_column(P, im, cl.col0 + site * cl.basis + j, shot);
+                        }
+                    }
+                    pos += 1 + gap(r.z, cl.inv_log);
+                    if (pos >= cend) break;
+                    {
+                        const uint64_t local = pos - cbeg;
+                        const uint32_t site = (uint32_t)(local >> P.tile_log2), shot = (uint32_t)(local & (S - 1));
+                        uint32_t code = outcome(P.otab, cl.woff, cl.nout, r.w);
+                        while (code) {
+                            const uint32_t j = __ffs(code) - 1;
+                            code &= code - 1;
+                            xor_column(P, im, cl.col0 + site * cl.basis + j, shot);
+                        }
+                    }
+                    pos++;
+                }
+            }
+            pos = cend;
+            c++;
+        }
+    }
+    __syncthreads();
+
+    const uint64_t valid = P.shots > s0 ? (P.shots - s0 < S ? P.shots - s0 : S) : 0;
+    if (P.dets.enabled)
+        write_rows(reinterpret_cast<const uint8_t *>(im.det), P.det_row_bytes, (uint32_t)valid,
+                   P.dets.ptr + s0 * P.dets.stride, P.dets.stride);
+    if (P.obs.enabled && P.obs_row_bytes)
+        write_rows(reinterpret_cast<const uint8_t *>(im.obs), P.obs_row_bytes, (uint32_t)valid,
+                   P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
+}
+
+// Fixed-fault mode of the symptom engine: shot s gets the XOR of the columns listed in
+// cols[off[s] .. off[s+1]).
+__global__ void sym_fixed_kernel(SymDevParams P, const uint64_t *__restrict__ off, const uint32_t *__restrict__ cols) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t S = P.shots_per_tile;
+    const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
+    const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
+    Images im{lds, lds + det_words};
+    for (uint32_t i = threadIdx.x; i < det_words + obs_words; i += NT) lds[i] = 0;
+    __syncthreads();
+    const uint64_t s0 = (uint64_t)blockIdx.x * S;
+    const uint64_t valid = P.shots > s0 ? (P.shots - s0 < S ? P.shots - s0 : S) : 0;
+    for (uint32_t r = 0; r < valid; r++) {
+        for (uint64_t i = off[s0 + r] + threadIdx.x; i < off[s0 + r + 1]; i += NT) xor_column(P, im, cols[i], r);
+    }
+    __syncthreads();
+    if (P.dets.enabled)
+        write_rows(reinterpret_cast<const uint8_t *>(im.det), P.det_row_bytes, (uint32_t)valid,
+                   P.dets.ptr + s0 * P.dets.stride, P.dets.stride);
+    if (P.obs.enabled && P.obs_row_bytes)
+        write_rows(reinterpret_cast<const uint8_t *>(im.obs), P.obs_row_bytes, (uint32_t)valid,
+                   P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
+}
+
+// Counts set bits of each column's dets row + obs row.
+__global__ void sym_count_kernel(const uint8_t *__restrict__ det, uint32_t det_row, const uint8_t *__restrict__ obs,
+                                 uint32_t obs_row, uint64_t rows, uint32_t *__restrict__ counts) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    uint32_t c = 0;
+    const uint8_t *d = det + r * det_row;
+    for (uint32_t i = 0; i < det_row; i++) c += __popc(d[i]);
+    const uint8_t *o = obs + r * obs_row;
+    for (uint32_t i = 0; i < obs_row; i++) c += __popc(o[i]);
+    counts[r] = c;
+}
+
+__global__ void sym_fill_kernel(const uint8_t *__restrict__ det, uint32_t det_row, const uint8_t *__restrict__ obs,
+                                uint32_t obs_row, uint64_t rows, uint32_t D, const uint32_t *__restrict__ offs,
+                                uint32_t *__restrict__ ent) {
+    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    uint32_t k = offs[r];
+    const uint8_t *d = det + r * det_row;
+    for (uint32_t i = 0; i < det_row; i++) {
+        uint32_t v = d[i];
+        while (v) {
+            const uint32_t b = __ffs(v) - 1;
+            v &= v - 1;
+            ent[k++] = 8 * i + b;
+        }
+    }
+    const uint8_t *o = obs + r * obs_row;
+    for (uint32_t i = 0; i < obs_row; i++) {
+        uint32_t v = o[i];
+        while (v) {
+            const uint32_t b = __ffs(v) - 1;
+            v &= v - 1;
+            ent[k++] = D + 8 * i + b;
+        }
+    }
+}
+
+// Packed rows -> one byte per entry (value bit + base_char).
+__global__ void unpack_rows_kernel(const uint8_t *__restrict__ src, uint64_t src_stride, uint8_t *__restrict__ dst,
+                                   uint64_t dst_stride, uint64_t rows, uint32_t ncols, uint32_t base_char) {
+    const uint64_t total = rows * ncols;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t r = i / ncols;
+        const uint32_t c = (uint32_t)(i % ncols);
+        dst[r * dst_stride + c] = (uint8_t)(((src[r * src_stride + (c >> 3)] >> (c & 7)) & 1u) + base_char);
+    }
+}
+
+}  // namespace
+
+namespace sa {
+
+hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream) {
+    if (!tiles) return hipSuccess;
+    (void)hipFuncSetAttribute((const void *)sym_sample_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(sym_sample_kernel, dim3(tiles), dim3(NT), lds, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_fixed(const SymDevParams &P, const uint64_t *off, const uint32_t *cols, uint32_t tiles,
+                            size_t lds, hipStream_t stream) {
+    if (!tiles) return hipSuccess;
+    (void)hipFuncSetAttribute((const void *)sym_fixed_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(sym_fixed_kernel, dim3(tiles), dim3(NT), lds, stream, P, off, cols);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_count(const uint8_t *det, uint32_t det_row, const uint8_t *obs, uint32_t obs_row, uint64_t rows,
+                            uint32_t *counts, hipStream_t stream) {
+    if (!rows) return hipSuccess;
+    hipLaunchKernelGGL(sym_count_kernel, dim3((uint32_t)((rows + 255) / 256)), dim3(256), 0, stream, det, det_row, obs,
+                       obs_row, rows, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_fill(const uint8_t *det, uint32_t det_row, const uint8_t *obs, uint32_t obs_row, uint64_t rows,
+                           uint32_t D, const uint32_t *offs, uint32_t *ent, hipStream_t stream) {
+    if (!rows) return hipSuccess;
+    hipLaunchKernelGGL(sym_fill_kernel, dim3((uint32_t)((rows + 255) / 256)), dim3(256), 0, stream, det, det_row, obs,
+                       obs_row, rows, D, offs, ent);
+    return hipGetLastError();
+}
+
+hipError_t launch_unpack_rows(const uint8_t *src, uint64_t src_stride, uint8_t *dst, uint64_t dst_stride, uint64_t rows,
+                              uint32_t ncols, uint32_t base_char, hipStream_t stream) {
+    const uint64_t total = rows * ncols;
+    if (!total) return hipSuccess;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>((total + 255) / 256, 65536);
+    hipLaunchKernelGGL(unpack_rows_kernel, dim3(blocks), dim3(256), 0, stream, src, src_stride, dst, dst_stride, rows,
+                       ncols, base_char);
+    return hipGetLastError();
+}
+
+}  // namespace sa

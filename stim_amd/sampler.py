@@ -256,6 +256,17 @@ class CompiledDetectorSampler:
         keys = ['tile_words', 'lds_bytes', 'ops', 'syncs', 'noise_ops', 'randomize', 'ring', 'dense_qubits']
         return dict(zip(keys, list(out)))
 
+    def set_engine(self, engine: str):
+        """'auto', 'frame' or 'symptom' (see include/stim_amd.h mfs_set_engine)."""
+        N.check(N.lib().mfs_set_engine(self._h, {'auto': 0, 'frame': 1, 'symptom': 2}[engine]))
+
+    def engine_info(self):
+        out = (ctypes.c_uint64 * 6)()
+        N.check(N.lib().mfs_engine_info(self._h, out))
+        v = list(out)
+        return {'engine': {1: 'frame', 2: 'symptom'}[v[0]], 'classes': v[1], 'columns': v[2], 'entries': v[3],
+                'build_ms': v[4] / 1000.0, 'sites': v[5]}
+
     def bench_device(self, shots: int, warmup: int = 2, iters: int = 5):
         """Average device time (ms) of one sample of `shots` into HBM-resident packed outputs."""
         out = (ctypes.c_double * 2)()

@@ -17,9 +17,14 @@ import zoo
 pytestmark = pytest.mark.gpu
 
 
-def _sampler(text, seed=1):
+ENGINES = ['frame', 'symptom']
+
+
+def _sampler(text, seed=1, engine='auto'):
     import stim_amd
-    return stim_amd.Circuit(text).compile_detector_sampler(seed=seed)
+    s = stim_amd.Circuit(text).compile_detector_sampler(seed=seed)
+    s.set_engine(engine)
+    return s
 
 
 def _rows(dets, obs):
@@ -28,11 +33,12 @@ def _rows(dets, obs):
 
 # ---------------------------------------------------------------- fixed-fault parity (bit-exact)
 
-def test_fixed_faults_match_reference_vectors(golden):
+@pytest.mark.parametrize('engine', ENGINES)
+def test_fixed_faults_match_reference_vectors(golden, engine):
     fixtures = golden.load('fault_vectors.json')
     for f in fixtures:
-        print('circuit', f['name'], flush=True)
-        s = _sampler(f['circuit'])
+        print('circuit', f['name'], engine, flush=True)
+        s = _sampler(f['circuit'], engine=engine)
         faults = [[tuple(x) for x in st_['faults']] for st_ in f['sets']]
         dets, obs = s.sample_with_faults(faults, separate_observables=True)
         got = _rows(dets, obs)
@@ -51,37 +57,40 @@ def _random_fault_sets(flat, n_shots, rng, max_faults=6):
     return out
 
 
+@pytest.mark.parametrize('engine', ENGINES)
 @pytest.mark.parametrize('seed', range(6))
-def test_fixed_faults_match_oracle_on_gate_zoo(seed):
+def test_fixed_faults_match_oracle_on_gate_zoo(seed, engine):
     text = zoo.gate_zoo(100 + seed, n_qubits=12 + 4 * seed, layers=14)
     flat = st.parse_flat(text)
     rng = random.Random(seed)
     faults = _random_fault_sets(flat, 300, rng)
-    print('zoo seed', seed, flush=True)
-    s = _sampler(text)
+    print('zoo seed', seed, engine, flush=True)
+    s = _sampler(text, engine=engine)
     dets, obs = s.sample_with_faults(faults, separate_observables=True)
     edets, eobs = fo.FrameOracle(flat).run(len(faults), faults=faults)
     np.testing.assert_array_equal(dets, edets)
     np.testing.assert_array_equal(obs, eobs)
 
 
+@pytest.mark.parametrize('engine', ENGINES)
 @pytest.mark.parametrize('name', ['surface_d5_r5', 'color_d5_r3', 'surface_x_d5_r3'])
-def test_fixed_faults_bit_packed_match_oracle(golden, name):
+def test_fixed_faults_bit_packed_match_oracle(golden, name, engine):
     text = golden.circuit(name)
     flat = st.parse_flat(text)
     faults = _random_fault_sets(flat, 700, random.Random(11), max_faults=12)
-    s = _sampler(text)
+    s = _sampler(text, engine=engine)
     dets, obs = s.sample_with_faults(faults, separate_observables=True, bit_packed=True)
     edets, eobs = fo.FrameOracle(flat).run(len(faults), faults=faults)
     np.testing.assert_array_equal(dets, fo.pack_bits(edets))
     np.testing.assert_array_equal(obs, fo.pack_bits(eobs))
 
 
-def test_fixed_faults_d25_full_size_match_oracle(golden):
+@pytest.mark.parametrize('engine', ENGINES)
+def test_fixed_faults_d25_full_size_match_oracle(golden, engine):
     text = golden.circuit('surface_d25_r25')
     flat = st.parse_flat(text)
     faults = _random_fault_sets(flat, 64, random.Random(5), max_faults=40)
-    s = _sampler(text)
+    s = _sampler(text, engine=engine)
     dets, obs = s.sample_with_faults(faults, separate_observables=True, bit_packed=True)
     edets, eobs = fo.FrameOracle(flat).run(len(faults), faults=faults)
     np.testing.assert_array_equal(dets, fo.pack_bits(edets))
@@ -107,12 +116,13 @@ def test_xor_linearity_of_fault_injection(golden):
         np.testing.assert_array_equal(d_sets[i], acc)
 
 
-def test_noiseless_circuits_give_zero_detectors(golden):
+@pytest.mark.parametrize('engine', ENGINES)
+def test_noiseless_circuits_give_zero_detectors(golden, engine):
     for name in ['surface_d5_r5', 'surface_d25_r25', 'color_d5_r3', 'rep_d7_r5']:
         text = golden.circuit(name)
         for p in ['0.001', '0.002']:
             text = text.replace(f'({p})', '(0)')
-        s = _sampler(text)
+        s = _sampler(text, engine=engine)
         d, o = s.sample(5000, bit_packed=True, separate_observables=True)
         assert not d.any() and not o.any(), name
 
@@ -131,9 +141,10 @@ OBSERVABLE_INCLUDE(0) rec[-2]
 """
 
 
-def test_reference_sample_vectors():
+@pytest.mark.parametrize('engine', ENGINES)
+def test_reference_sample_vectors(engine):
     """Vectors from the reference's compiled_detector_sampler_pybind_test.py:37-160."""
-    s = _sampler(KAT)
+    s = _sampler(KAT, engine=engine)
     np.testing.assert_array_equal(s.sample(5), np.array([[1, 1, 0]] * 5, dtype=np.bool_))
     np.testing.assert_array_equal(s.sample(5, prepend_observables=True),
                                   np.array([[1, 0, 0, 0, 1, 1, 0]] * 5, dtype=np.bool_))
@@ -150,7 +161,7 @@ def test_reference_sample_vectors():
     dets, obs = s.sample(5, separate_observables=True, bit_packed=True)
     np.testing.assert_array_equal(dets, np.array([[0b011]] * 5, dtype=np.uint8))
     np.testing.assert_array_equal(obs, np.array([[0b0001]] * 5, dtype=np.uint8))
-    t = _sampler('X_ERROR(1) 0\nM 0\nDETECTOR rec[-1]\n')
+    t = _sampler('X_ERROR(1) 0\nM 0\nDETECTOR rec[-1]\n', engine=engine)
     np.testing.assert_array_equal(t.sample(shots=2), np.array([[1], [1]], dtype=np.bool_))
 
 
@@ -166,8 +177,9 @@ def test_sample_argument_errors():
         s.sample(3, bit_packed=True, dets_out=np.zeros((3,), dtype=np.uint8))
 
 
-def test_manual_output_buffers():
-    s = _sampler(KAT)
+@pytest.mark.parametrize('engine', ENGINES)
+def test_manual_output_buffers(engine):
+    s = _sampler(KAT, engine=engine)
     dets = np.zeros((11, 1), dtype=np.uint8)
     obs = np.zeros((11, 1), dtype=np.uint8)
     d2, o2 = s.sample(11, separate_observables=True, bit_packed=True, dets_out=dets, obs_out=obs)
@@ -178,9 +190,10 @@ def test_manual_output_buffers():
     assert dets[:, :2].all() and not dets[:, 2].any()
 
 
-def test_sample_write_formats_match_reference_bytes(golden):
+@pytest.mark.parametrize('engine', ENGINES)
+def test_sample_write_formats_match_reference_bytes(golden, engine):
     fx = golden.load('formats.json')
-    s = _sampler(fx['circuit'])
+    s = _sampler(fx['circuit'], engine=engine)
     with tempfile.TemporaryDirectory() as d:
         for key, hexdata in sorted(fx['outputs'].items()):
             fmt, mode, shots = key.split('/')
@@ -222,11 +235,12 @@ def _zcrit(n, alpha=1e-4):
     return abs(norm.ppf(alpha / (2 * n)))
 
 
-@pytest.mark.parametrize('name', ['surface_d5_r5', 'surface_d3_r3'])
-def test_seeded_rates_match_exact_rates(golden, name):
+@pytest.mark.parametrize('engine', ENGINES)
+@pytest.mark.parametrize('name', ['surface_d5_r5', 'surface_d3_r3', 'color_d5_r3'])
+def test_seeded_rates_match_exact_rates(golden, name, engine):
     """Per-detector firing rates pass a Bonferroni binomial test (alpha=1e-4) against exact rates."""
     rates = np.array(golden.load('dem_rates.json')[name])
-    s = _sampler(golden.circuit(name), seed=12345)
+    s = _sampler(golden.circuit(name), seed=12345, engine=engine)
     shots = 1 << 21
     d = s.sample(shots, bit_packed=True, append_observables=True)
     n = len(rates)
@@ -240,7 +254,8 @@ def test_seeded_rates_match_exact_rates(golden, name):
     assert stat < chi2.ppf(1 - 1e-4, df=n)
 
 
-def test_seeded_pair_correlations_match_exact(golden):
+@pytest.mark.parametrize('engine', ENGINES)
+def test_seeded_pair_correlations_match_exact(golden, engine):
     """Per-mechanism check: co-firing rate of detector pairs joined by single faults."""
     name = 'surface_d3_r3'
     text = golden.circuit(name).replace('0.001', '0.01')
@@ -265,7 +280,7 @@ def test_seeded_pair_correlations_match_exact(golden):
             pos += 1
         logs += np.log(np.abs(1 - 2 * q))
     exact = (1 - np.exp(logs)) / 2
-    s = _sampler(text, seed=99)
+    s = _sampler(text, seed=99, engine=engine)
     shots = 1 << 20
     d = s.sample(shots, append_observables=True)
     counts = np.array([(d[:, a] ^ d[:, b]).sum() for a, b in pairs])
@@ -273,9 +288,10 @@ def test_seeded_pair_correlations_match_exact(golden):
     assert np.max(np.abs(z)) < _zcrit(len(pairs))
 
 
-def test_d25_full_size_rates_and_shape(golden):
+@pytest.mark.parametrize('engine', ENGINES)
+def test_d25_full_size_rates_and_shape(golden, engine):
     text = golden.circuit('surface_d25_r25')
-    s = _sampler(text, seed=2)
+    s = _sampler(text, seed=2, engine=engine)
     shots = 1 << 20
     dets, obs = s.sample(shots, bit_packed=True, separate_observables=True)
     assert dets.shape == (shots, 1950) and obs.shape == (shots, 1)
@@ -285,5 +301,37 @@ def test_d25_full_size_rates_and_shape(golden):
     # reference's error model is checked in the statistics tests at smaller distance).
     assert 0.005 < rate < 0.05
     # Different calls draw different randomness; same seed reproduces.
-    again = _sampler(text, seed=2).sample(1024, bit_packed=True, separate_observables=True)[0]
+    again = _sampler(text, seed=2, engine=engine).sample(1024, bit_packed=True, separate_observables=True)[0]
     np.testing.assert_array_equal(again, dets[:1024])
+
+
+def test_engines_agree_statistically_on_zoo():
+    """Frame and symptom engines sample the same distribution on an all-gate circuit."""
+    text = zoo.gate_zoo(7, n_qubits=10, layers=12)
+    shots = 1 << 18
+    a = _sampler(text, seed=3, engine='frame').sample(shots, append_observables=True)
+    b = _sampler(text, seed=4, engine='symptom').sample(shots, append_observables=True)
+    ra, rb = a.mean(axis=0), b.mean(axis=0)
+    p = (ra + rb) / 2
+    sd = np.sqrt(np.maximum(p * (1 - p), 1e-12) * 2 / shots)
+    z = (ra - rb) / sd
+    assert np.max(np.abs(z)) < _zcrit(len(z))
+
+
+def test_auto_engine_is_symptom_for_deterministic_circuits(golden):
+    s = _sampler(golden.circuit('surface_d5_r5'))
+    info = s.engine_info()
+    assert info['engine'] == 'symptom'
+    assert info['columns'] > 0 and info['entries'] > 0
+
+
+def test_random_detectors_use_frame_engine_and_are_fair():
+    """A detector on a random measurement is uniform, as with the reference's frame randomization."""
+    text = 'H 0\nM 0\nDETECTOR rec[-1]\nR 1\nX_ERROR(0.25) 1\nM 1\nDETECTOR rec[-1]\n'
+    s = _sampler(text, seed=8)
+    assert s.engine_info()['engine'] == 'frame'
+    d = s.sample(1 << 16)
+    z = _z(d.sum(axis=0), 1 << 16, np.array([0.5, 0.25]))
+    assert np.max(np.abs(z)) < _zcrit(2)
+    with pytest.raises(ValueError):
+        s.set_engine('symptom')
