@@ -98,6 +98,10 @@ int mfs_set_engine(mfs_sampler *s, int engine);
  * table if needed. */
 int mfs_engine_info(mfs_sampler *s, uint64_t out[6]);
 
+/* Symptom engine: target size (KB) of one workgroup's LDS image of output rows (sets the shots per
+ * workgroup). Default 32. */
+int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb);
+
 /* Configuration knobs: words per tile (1, 2 or 4; 0 = automatic). Returns the value in use. */
 int mfs_set_tile_words(mfs_sampler *s, int words);
 int mfs_get_info(const mfs_sampler *s, uint64_t out[8]); /* tile_words, lds_bytes, ops, syncs, noise_ops, randomize, ring, dense_qubits */

@@ -24,7 +24,7 @@ MFS_DEVICE_OUTPUT = 1 << 4
 EXPORTED = [
     'mfs_compile', 'mfs_destroy', 'mfs_circuit_stats', 'mfs_parse_stats', 'mfs_sample', 'mfs_sample_write',
     'mfs_sample_with_faults', 'mfs_bench_device', 'mfs_set_tile_words', 'mfs_get_info', 'mfs_last_error',
-    'mfs_set_engine', 'mfs_engine_info',
+    'mfs_set_engine', 'mfs_engine_info', 'mfs_set_symptom_tile_kb',
 ]
 
 _lib = None
@@ -53,6 +53,7 @@ def lib():
     L.mfs_get_info.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_set_engine.argtypes = [p, ctypes.c_int]
     L.mfs_engine_info.argtypes = [p, ctypes.POINTER(u64)]
+    L.mfs_set_symptom_tile_kb.argtypes = [p, ctypes.c_int]
     L.mfs_last_error.argtypes = []
     L.mfs_last_error.restype = ctypes.c_char_p
     for name in EXPORTED:

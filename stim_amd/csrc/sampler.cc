@@ -104,7 +104,8 @@ struct mfs_sampler {
     SymPlan plan;
     uint64_t sym_entries = 0;
     double sym_build_ms = 0;
-    DevBuf d_classes, d_sotab, d_col_off, d_ent, d_sym_tmp, d_fix_off, d_fix_cols;
+    DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols;
+    int sym_tile_kb = 32;        // target LDS image size per symptom tile
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~mfs_sampler() {
@@ -414,8 +415,27 @@ struct mfs_sampler {
         sym_entries = ents.size();
         upload(d_classes, plan.classes, "classes");
         upload(d_sotab, plan.otab, "symptom outcome table");
-        upload(d_col_off, offs, "column offsets");
-        upload(d_ent, ents, "symptom entries");
+        // Column records: up to two entries inline, longer symptoms in the overflow list.
+        std::vector<uint2> recs(F);
+        std::vector<uint32_t> ovf;
+        for (uint64_t c = 0; c < F; c++) {
+            const uint32_t n = offs[c + 1] - offs[c];
+            const uint32_t *e = ents.data() + offs[c];
+            if (n <= SYM_INLINE_MAX) {
+                recs[c] = make_uint2((n << 28) | (n >= 1 ? e[0] : 0u), n == 2 ? e[1] : 0u);
+            } else {
+                if (ovf.size() >= (1u << 28)) throw std::invalid_argument("symptom table too large");
+                recs[c] = make_uint2((3u << 28) | (uint32_t)ovf.size(), n);
+                ovf.insert(ovf.end(), e, e + n);
+            }
+        }
+        if (prog.num_detectors + prog.num_observables >= (1u << 28)) {
+            sym_state = -1;
+            sym_why = "too many detectors for symptom records";
+            return false;
+        }
+        upload(d_recs, recs, "symptom records");
+        upload(d_ovf, ovf, "symptom overflow entries");
         sym_state = 1;
         return true;
     }
@@ -440,7 +460,7 @@ struct mfs_sampler {
     // Shots per symptom tile: LDS image of about 32 KB.
     uint32_t sym_tile_shots(size_t row_bytes) const {
         uint32_t S = 1;
-        while (S < 1024 && (size_t)(2 * S) * row_bytes <= 32 * 1024) S *= 2;
+        while (S < 1024 && (size_t)(2 * S) * row_bytes <= (size_t)sym_tile_kb * 1024) S *= 2;
         return S;
     }
 
@@ -448,9 +468,10 @@ struct mfs_sampler {
         SymDevParams P{};
         P.classes = d_classes.as<SymClass>();
         P.otab = d_sotab.as<uint32_t>();
-        P.col_off = d_col_off.as<uint32_t>();
-        P.ent = d_ent.as<uint32_t>();
+        P.recs = d_recs.as<uint2>();
+        P.ovf = d_ovf.as<uint32_t>();
         P.nclasses = (uint32_t)plan.classes.size();
+        P.otab_words = (uint32_t)plan.otab.size();
         P.D = prog.num_detectors;
         P.O = prog.num_observables;
         P.shots_per_tile = S;
@@ -478,7 +499,8 @@ struct mfs_sampler {
         P.obs_col0 = L.append ? D : 0;
         P.obs = DevOut{obs_dst, obs_stride, 0, 1, 0, obs_bytes ? 1u : 0u};
         P.obs_row_bytes = (uint32_t)obs_bytes;
-        const size_t lds = ((size_t)S * det_bytes + 15) / 16 * 16 + ((size_t)S * obs_bytes + 15) / 16 * 16;
+        const size_t lds = ((size_t)S * det_bytes + 15) / 16 * 16 + ((size_t)S * obs_bytes + 15) / 16 * 16 +
+                           (P.otab_words <= SYM_OTAB_LDS_WORDS ? (size_t)P.otab_words * 4 : 0);
         const uint64_t tiles = (n + S - 1) / S;
         if (tiles >= (1ull << 31)) throw std::invalid_argument("too many shots in one launch");
         if (fix_off) {
@@ -964,6 +986,13 @@ int mfs_engine_info(mfs_sampler *s, uint64_t out[6]) {
         out[3] = s->sym_entries;
         out[4] = (uint64_t)(s->sym_build_ms * 1000.0);
         out[5] = s->plan.total_sites;
+    });
+}
+
+int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb) {
+    return guarded([&] {
+        if (kb < 1 || kb > 128) throw std::invalid_argument("symptom tile size must be 1..128 KB");
+        s->sym_tile_kb = kb;
     });
 }
 

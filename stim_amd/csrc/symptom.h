@@ -19,6 +19,8 @@
 #include <cstdint>
 #include <vector>
 
+#include <hip/hip_runtime.h>
+
 #include "compiler.h"
 
 namespace sa {
@@ -54,12 +56,18 @@ struct SymPlan {
 
 SymPlan plan_symptoms(const Program &prog);
 
+// Column record (8 bytes): x bits 28..31 = entry count c; c <= 2: entries inline (x bits 0..27,
+// then y); c == 3: x bits 0..27 = offset into the overflow entries, y = entry count.
+constexpr uint32_t SYM_INLINE_MAX = 2;
+constexpr uint32_t SYM_OTAB_LDS_WORDS = 1024;  // outcome tables up to this size are read from LDS
+
 struct SymDevParams {
     const SymClass *classes;
     const uint32_t *otab;
-    const uint32_t *col_off;
-    const uint32_t *ent;
+    const uint2 *recs;             // one record per column
+    const uint32_t *ovf;           // overflow entries
     uint32_t nclasses;
+    uint32_t otab_words;           // outcome-table words (copied to LDS when small)
     uint32_t D, O;
     uint32_t shots_per_tile;       // power of two
     uint32_t tile_log2;

@@ -43,8 +43,7 @@ __device__ __forceinline__ uint64_t gap(uint32_t r, float inv_log2_1mp) {
     return (uint64_t)g;
 }
 
-__device__ __forceinline__ uint32_t outcome(const uint32_t *__restrict__ otab, uint32_t woff, uint32_t nout,
-                                            uint32_t r) {
+__device__ __forceinline__ uint32_t outcome(const uint32_t *otab, uint32_t woff, uint32_t nout, uint32_t r) {
     for (uint32_t i = 0; i + 1 < nout; i++) {
         if (r <= otab[woff + 2 * i]) return otab[woff + 2 * i + 1];
     }
@@ -56,26 +55,69 @@ struct Images {
     uint32_t *obs;   // separate observable image
 };
 
-// XORs column `col`'s symptom into shot `shot` of the images.
+__device__ __forceinline__ void xor_entry(const SymDevParams &P, const Images &im, uint32_t m, uint32_t shot) {
+    uint32_t bit;
+    uint32_t *img;
+    if (m < P.D) {
+        bit = shot * (P.det_row_bytes * 8) + P.dets.col0 + m;
+        img = im.det;
+    } else if (P.obs_mode == 2) {
+        bit = shot * (P.det_row_bytes * 8) + P.obs_col0 + (m - P.D);
+        img = im.det;
+    } else if (P.obs_mode == 1) {
+        bit = shot * (P.obs_row_bytes * 8) + (m - P.D);
+        img = im.obs;
+    } else {
+        return;
+    }
+    atomicXor(&img[bit >> 5], 1u << (bit & 31));
+}
+
+// XORs one column record's symptom into shot `shot` of the images.
+__device__ __forceinline__ void xor_record(const SymDevParams &P, const Images &im, uint2 rec, uint32_t shot) {
+    const uint32_t c = rec.x >> 28;
+    if (c == 3) {
+        const uint32_t off = rec.x & 0x0FFFFFFFu;
+        for (uint32_t k = 0; k < rec.y; k++) xor_entry(P, im, P.ovf[off + k], shot);
+        return;
+    }
+    if (c >= 1) xor_entry(P, im, rec.x & 0x0FFFFFFFu, shot);
+    if (c == 2) xor_entry(P, im, rec.y, shot);
+}
+
 __device__ __forceinline__ void xor_column(const SymDevParams &P, const Images &im, uint32_t col, uint32_t shot) {
-    const uint32_t b = P.col_off[col], e = P.col_off[col + 1];
-    for (uint32_t k = b; k < e; k++) {
-        const uint32_t m = P.ent[k];
-        uint32_t bit;
-        uint32_t *img;
-        if (m < P.D) {
-            bit = shot * (P.det_row_bytes * 8) + P.dets.col0 + m;
-            img = im.det;
-        } else if (P.obs_mode == 2) {
-            bit = shot * (P.det_row_bytes * 8) + P.obs_col0 + (m - P.D);
-            img = im.det;
-        } else if (P.obs_mode == 1) {
-            bit = shot * (P.obs_row_bytes * 8) + (m - P.D);
-            img = im.obs;
-        } else {
-            continue;
+    xor_record(P, im, P.recs[col], shot);
+}
+
+// Applies the outcomes of two hits: loads every basis record of both hits first (independent
+// loads in flight together), then XORs them into the images. Classes with more than 4 basis
+// columns (correlated chains) take the one-at-a-time path.
+__device__ __forceinline__ void apply_two(const SymDevParams &P, const Images &im, const SymClass &cl, uint32_t code1,
+                                          uint32_t col1, uint32_t shot1, uint32_t code2, uint32_t col2,
+                                          uint32_t shot2) {
+    if (cl.basis > 4) {
+        while (code1) {
+            const uint32_t j = __ffs(code1) - 1;
+            code1 &= code1 - 1;
+            xor_column(P, im, col1 + j, shot1);
         }
-        atomicXor(&img[bit >> 5], 1u << (bit & 31));
+        while (code2) {
+            const uint32_t j = __ffs(code2) - 1;
+            code2 &= code2 - 1;
+            xor_column(P, im, col2 + j, shot2);
+        }
+        return;
+    }
+    uint2 r1[4], r2[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        r1[j] = (code1 >> j) & 1 ? P.recs[col1 + j] : make_uint2(0, 0);
+        r2[j] = (code2 >> j) & 1 ? P.recs[col2 + j] : make_uint2(0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if ((code1 >> j) & 1) xor_record(P, im, r1[j], shot1);
+        if ((code2 >> j) & 1) xor_record(P, im, r2[j], shot2);
     }
 }
 
@@ -112,8 +154,14 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
     const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
     const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
     Images im{lds, lds + det_words};
+    uint32_t *const otab_lds = lds + det_words + obs_words;
+    const bool otab_in_lds = P.otab_words <= SYM_OTAB_LDS_WORDS;
+    const uint32_t *otab = otab_in_lds ? otab_lds : P.otab;
     const uint32_t tid = threadIdx.x;
     for (uint32_t i = tid; i < det_words + obs_words; i += NT) lds[i] = 0;
+    if (otab_in_lds) {
+        for (uint32_t i = tid; i < P.otab_words; i += NT) otab_lds[i] = P.otab[i];
+    }
     __syncthreads();
 
     const uint64_t s0 = (uint64_t)blockIdx.x * S;
@@ -142,31 +190,19 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
             if (cl.p > 0.0f) {
                 while (true) {
                     const uint4 r = philox10(make_uint4(ctr++, tid, tile, TAG_SYM), k0, k1);
-                    pos += gap(r.x, cl.inv_log);
-                    if (pos >= cend) break;
-                    {
-                        const uint64_t local = pos - cbeg;
-                        const uint32_t site = (uint32_t)(local >> P.tile_log2), shot = (uint32_t)(local & (S - 1));
-                        uint32_t code = outcome(P.otab, cl.woff, cl.nout, r.y);
-                        while (code) {
-                            const uint32_t j = __ffs(code) - 1;
-                            code &= code - 1;
-                            xor_column(P, im, cl.col0 + site * cl.basis + j, shot);
-                        }
-                    }
-                    pos += 1 + gap(r.z, cl.inv_log);
-                    if (pos >= cend) break;
-                    {
-                        const uint64_t local = pos - cbeg;
-                        const uint32_t site = (uint32_t)(local >> P.tile_log2), shot = (uint32_t)(local & (S - 1));
-                        uint32_t code = outcome(P.otab, cl.woff, cl.nout, r.w);
-                        while (code) {
-                            const uint32_t j = __ffs(code) - 1;
-                            code &= code - 1;
-                            xor_column(P, im, cl.col0 + site * cl.basis + j, shot);
-                        }
-                    }
-                    pos++;
+                    const uint64_t p1 = pos + gap(r.x, cl.inv_log);
+                    if (p1 >= cend) break;
+                    const uint64_t p2 = p1 + 1 + gap(r.z, cl.inv_log);
+                    const bool two = p2 < cend;
+                    const uint64_t l1 = p1 - cbeg, l2 = p2 - cbeg;
+                    const uint32_t code1 = outcome(otab, cl.woff, cl.nout, r.y);
+                    const uint32_t code2 = two ? outcome(otab, cl.woff, cl.nout, r.w) : 0u;
+                    apply_two(P, im, cl, code1, cl.col0 + (uint32_t)(l1 >> P.tile_log2) * cl.basis,
+                              (uint32_t)(l1 & (S - 1)), code2,
+                              cl.col0 + (uint32_t)((two ? l2 : l1) >> P.tile_log2) * cl.basis,
+                              (uint32_t)(l2 & (S - 1)));
+                    if (!two) break;
+                    pos = p2 + 1;
                 }
             }
             pos = cend;

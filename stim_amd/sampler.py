@@ -260,6 +260,9 @@ class CompiledDetectorSampler:
         """'auto', 'frame' or 'symptom' (see include/stim_amd.h mfs_set_engine)."""
         N.check(N.lib().mfs_set_engine(self._h, {'auto': 0, 'frame': 1, 'symptom': 2}[engine]))
 
+    def set_symptom_tile_kb(self, kb: int):
+        N.check(N.lib().mfs_set_symptom_tile_kb(self._h, int(kb)))
+
     def engine_info(self):
         out = (ctypes.c_uint64 * 6)()
         N.check(N.lib().mfs_engine_info(self._h, out))
