@@ -64,6 +64,8 @@ def main():
     ap.add_argument('--shots', type=int, default=1 << 20)
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--tile-words', type=int, default=0)
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='run a 1 GiB device-to-device copy first (reference for FETCH_SIZE/WRITE_SIZE counters)')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path (shots would shard '
@@ -86,6 +88,12 @@ def main():
     dets = torch.empty((S, drow), dtype=torch.uint8, device=dev)
     obs = torch.empty((S, max(orow, 1)), dtype=torch.uint8, device=dev)
     flags = N.MFS_BIT_PACKED | N.MFS_SEPARATE_OBSERVABLES | N.MFS_DEVICE_OUTPUT
+    if args.calibrate_copy:
+        src = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     L = N.lib()
 
     def step():

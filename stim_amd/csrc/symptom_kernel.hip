@@ -20,6 +20,8 @@ namespace {
 
 constexpr int NT = 256;
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
     for (int r = 0; r < 10; r++) {
@@ -130,11 +132,8 @@ __device__ void write_rows(const uint8_t *img, uint32_t pitch, uint32_t rows, ui
         if ((a & 15) == 0) {
             const uint32_t n16 = total / 16;
             for (uint32_t i = tid; i < n16; i += NT) {
-                const uint4 v = *reinterpret_cast<const uint4 *>(img + 16 * i);
-                __builtin_nontemporal_store(v.x, reinterpret_cast<uint32_t *>(dst + 16 * i));
-                __builtin_nontemporal_store(v.y, reinterpret_cast<uint32_t *>(dst + 16 * i + 4));
-                __builtin_nontemporal_store(v.z, reinterpret_cast<uint32_t *>(dst + 16 * i + 8));
-                __builtin_nontemporal_store(v.w, reinterpret_cast<uint32_t *>(dst + 16 * i + 12));
+                const u32x4 v = *reinterpret_cast<const u32x4 *>(img + 16 * i);
+                __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(dst + 16 * i));
             }
             for (uint32_t i = 16 * n16 + tid; i < total; i += NT) dst[i] = img[i];
         } else {
