@@ -500,7 +500,7 @@ struct mfs_sampler {
         P.obs = DevOut{obs_dst, obs_stride, 0, 1, 0, obs_bytes ? 1u : 0u};
         P.obs_row_bytes = (uint32_t)obs_bytes;
         const size_t lds = ((size_t)S * det_bytes + 15) / 16 * 16 + ((size_t)S * obs_bytes + 15) / 16 * 16 +
-                           (P.otab_words <= SYM_OTAB_LDS_WORDS ? (size_t)P.otab_words * 4 : 0);
+                           SYM_QUEUE_BYTES + (P.otab_words <= SYM_OTAB_LDS_WORDS ? (size_t)P.otab_words * 4 : 0);
         const uint64_t tiles = (n + S - 1) / S;
         if (tiles >= (1ull << 31)) throw std::invalid_argument("too many shots in one launch");
         if (fix_off) {

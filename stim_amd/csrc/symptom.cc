@@ -41,9 +41,9 @@ SymPlan plan_symptoms(const Program &prog) {
 
     auto add_chain = [&](const std::vector<uint32_t> &links) {
         if (links.empty()) return;
-        if (links.size() > 24) {
+        if (links.size() > 22) {
             plan.supported = false;
-            plan.why_unsupported = "correlated-error chain longer than 24 links";
+            plan.why_unsupported = "correlated-error chain longer than 22 links";
             return;
         }
         // Link k fires with probability p_k * prod_{j<k} (1 - p_j) (frame_simulator.inl:750-776).
@@ -142,6 +142,16 @@ SymPlan plan_symptoms(const Program &prog) {
             sc.basis = basis_of(code);
             sc.nout = first.nout & 0xFFFF;
             for (uint32_t k = 0; k < 2 * sc.nout; k++) plan.otab.push_back(prog.otab[first.woff + k]);
+            // Equal outcome weights (DEPOLARIZE1/2, HERALDED_ERASE, single-outcome channels) let the
+            // kernel index the outcome directly instead of scanning thresholds.
+            {
+                bool uniform = true;
+                for (uint32_t k = 0; k + 1 < sc.nout; k++) {
+                    const double want = (double)(k + 1) / sc.nout * 4294967296.0 - 1.0;
+                    if (std::fabs((double)prog.otab[first.woff + 2 * k] - want) > 2.0) uniform = false;
+                }
+                if (uniform) sc.nout |= SYM_UNIFORM;
+            }
             uint32_t n = 0;
             for (uint32_t oi : class_ops[c]) {
                 const DevOp &op = prog.ops[oi];

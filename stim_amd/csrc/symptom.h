@@ -32,9 +32,10 @@ struct SymClass {
     uint32_t basis;  // basis columns per site
     uint32_t col0;   // first column of the class
     uint32_t woff;   // outcome table offset ((threshold, code-bitmask) pairs)
-    uint32_t nout;
+    uint32_t nout;   // outcomes | SYM_UNIFORM when all outcomes are equally likely
     uint32_t site0;  // cumulative sites of earlier classes
 };
+constexpr uint32_t SYM_UNIFORM = 1u << 31;
 static_assert(sizeof(SymClass) == 32, "SymClass layout");
 
 // Basis fault of one column, as a fixed-fault entry for the frame kernel.
@@ -60,6 +61,8 @@ SymPlan plan_symptoms(const Program &prog);
 // then y); c == 3: x bits 0..27 = offset into the overflow entries, y = entry count.
 constexpr uint32_t SYM_INLINE_MAX = 2;
 constexpr uint32_t SYM_OTAB_LDS_WORDS = 1024;  // outcome tables up to this size are read from LDS
+// LDS hit queue of the sampling kernel: 256 lanes x 8 hits x 8 bytes, plus a counter.
+constexpr size_t SYM_QUEUE_BYTES = 256 * 8 * 8 + 16;
 
 struct SymDevParams {
     const SymClass *classes;

@@ -45,7 +45,9 @@ __device__ __forceinline__ uint64_t gap(uint32_t r, float inv_log2_1mp) {
     return (uint64_t)g;
 }
 
-__device__ __forceinline__ uint32_t outcome(const uint32_t *otab, uint32_t woff, uint32_t nout, uint32_t r) {
+__device__ __forceinline__ uint32_t outcome(const uint32_t *otab, uint32_t woff, uint32_t nout_flags, uint32_t r) {
+    const uint32_t nout = nout_flags & 0xFFFFu;
+    if (nout_flags & SYM_UNIFORM) return otab[woff + 2 * __umulhi(r, nout) + 1];
     for (uint32_t i = 0; i + 1 < nout; i++) {
         if (r <= otab[woff + 2 * i]) return otab[woff + 2 * i + 1];
     }
@@ -147,13 +149,18 @@ __device__ void write_rows(const uint8_t *img, uint32_t pitch, uint32_t rows, ui
     }
 }
 
+// Hits queued per lane per round before the workgroup applies them together.
+constexpr uint32_t HITS_PER_ROUND = 8;
+
 __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
     extern __shared__ uint32_t lds[];
     const uint32_t S = P.shots_per_tile;
     const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
     const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
     Images im{lds, lds + det_words};
-    uint32_t *const otab_lds = lds + det_words + obs_words;
+    uint2 *const queue = reinterpret_cast<uint2 *>(lds + det_words + obs_words);  // NT*HITS_PER_ROUND
+    uint32_t *const qcount = lds + det_words + obs_words + 2 * NT * HITS_PER_ROUND;
+    uint32_t *const otab_lds = qcount + 4;
     const bool otab_in_lds = P.otab_words <= SYM_OTAB_LDS_WORDS;
     const uint32_t *otab = otab_in_lds ? otab_lds : P.otab;
     const uint32_t tid = threadIdx.x;
@@ -161,7 +168,7 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
     if (otab_in_lds) {
         for (uint32_t i = tid; i < P.otab_words; i += NT) otab_lds[i] = P.otab[i];
     }
-    __syncthreads();
+    if (tid == 0) qcount[0] = 0;
 
     const uint64_t s0 = (uint64_t)blockIdx.x * S;
     const uint32_t tile = (uint32_t)(P.tile0 + blockIdx.x);
@@ -170,8 +177,12 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
     const uint64_t R = (total + NT - 1) / NT;
     uint64_t pos = (uint64_t)tid * R;
     const uint64_t hi = pos + R < total ? pos + R : total;
-    if (pos < hi) {
-        // Class containing pos (binary search over cumulative sites).
+    // This lane's walk state: class c with bounds [cbeg, cend) and the Philox counter.
+    uint32_t c = 0, ctr = 0;
+    SymClass cl{};
+    uint64_t cbeg = 0, cend = 0;
+    bool active = pos < hi;
+    if (active) {
         uint32_t lo_c = 0, hi_c = P.nclasses;
         const uint64_t site_at = pos >> P.tile_log2;
         while (hi_c - lo_c > 1) {
@@ -179,34 +190,81 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
             if (P.classes[mid].site0 <= site_at) lo_c = mid;
             else hi_c = mid;
         }
-        uint32_t c = lo_c;
-        uint32_t ctr = 0;
-        while (pos < hi && c < P.nclasses) {
-            const SymClass cl = P.classes[c];
-            const uint64_t cbeg = (uint64_t)cl.site0 << P.tile_log2;
-            const uint64_t cend_full = cbeg + ((uint64_t)cl.nsites << P.tile_log2);
-            const uint64_t cend = cend_full < hi ? cend_full : hi;
-            if (cl.p > 0.0f) {
-                while (true) {
-                    const uint4 r = philox10(make_uint4(ctr++, tid, tile, TAG_SYM), k0, k1);
-                    const uint64_t p1 = pos + gap(r.x, cl.inv_log);
-                    if (p1 >= cend) break;
-                    const uint64_t p2 = p1 + 1 + gap(r.z, cl.inv_log);
-                    const bool two = p2 < cend;
-                    const uint64_t l1 = p1 - cbeg, l2 = p2 - cbeg;
-                    const uint32_t code1 = outcome(otab, cl.woff, cl.nout, r.y);
-                    const uint32_t code2 = two ? outcome(otab, cl.woff, cl.nout, r.w) : 0u;
-                    apply_two(P, im, cl, code1, cl.col0 + (uint32_t)(l1 >> P.tile_log2) * cl.basis,
-                              (uint32_t)(l1 & (S - 1)), code2,
-                              cl.col0 + (uint32_t)((two ? l2 : l1) >> P.tile_log2) * cl.basis,
-                              (uint32_t)(l2 & (S - 1)));
-                    if (!two) break;
-                    pos = p2 + 1;
+        c = lo_c;
+        cl = P.classes[c];
+        cbeg = (uint64_t)cl.site0 << P.tile_log2;
+        const uint64_t ce = cbeg + ((uint64_t)cl.nsites << P.tile_log2);
+        cend = ce < hi ? ce : hi;
+    }
+    __syncthreads();
+
+    // Rounds: every lane queues up to HITS_PER_ROUND hits (walking geometric gaps, two per Philox
+    // block), then all lanes apply the queued hits evenly.
+    while (__syncthreads_or(active)) {
+        uint32_t queued = 0;
+        while (active && queued + 2 <= HITS_PER_ROUND) {
+            if (pos >= cend || !(cl.p > 0.0f)) {
+                // Next class (restart the gap walk at its first position).
+                pos = cend;
+                c++;
+                if (pos >= hi || c >= P.nclasses) {
+                    active = false;
+                    break;
+                }
+                cl = P.classes[c];
+                cbeg = (uint64_t)cl.site0 << P.tile_log2;
+                const uint64_t ce = cbeg + ((uint64_t)cl.nsites << P.tile_log2);
+                cend = ce < hi ? ce : hi;
+                continue;
+            }
+            const uint4 r = philox10(make_uint4(ctr++, tid, tile, TAG_SYM), k0, k1);
+            const uint64_t p1 = pos + gap(r.x, cl.inv_log);
+            if (p1 >= cend) {
+                pos = cend;
+                continue;
+            }
+            const uint64_t p2 = p1 + 1 + gap(r.z, cl.inv_log);
+            uint32_t slot = atomicAdd(qcount, p2 < cend ? 2u : 1u);
+            {
+                const uint64_t l = p1 - cbeg;
+                queue[slot] = make_uint2(cl.col0 + (uint32_t)(l >> P.tile_log2) * cl.basis,
+                                         (uint32_t)(l & (S - 1)) | (outcome(otab, cl.woff, cl.nout, r.y) << 10));
+            }
+            queued++;
+            if (p2 < cend) {
+                const uint64_t l = p2 - cbeg;
+                queue[slot + 1] = make_uint2(cl.col0 + (uint32_t)(l >> P.tile_log2) * cl.basis,
+                                             (uint32_t)(l & (S - 1)) | (outcome(otab, cl.woff, cl.nout, r.w) << 10));
+                queued++;
+                pos = p2 + 1;
+            } else {
+                pos = cend;
+            }
+        }
+        __syncthreads();
+        const uint32_t n = qcount[0];
+        for (uint32_t i = tid; i < n; i += NT) {
+            const uint2 h = queue[i];
+            const uint32_t shot = h.y & 1023u;
+            uint32_t code = h.y >> 10;
+            if (code < 16) {
+                uint2 rec[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) rec[j] = (code >> j) & 1 ? P.recs[h.x + j] : make_uint2(0, 0);
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if ((code >> j) & 1) xor_record(P, im, rec[j], shot);
+                }
+            } else {
+                while (code) {
+                    const uint32_t j = __ffs(code) - 1;
+                    code &= code - 1;
+                    xor_column(P, im, h.x + j, shot);
                 }
             }
-            pos = cend;
-            c++;
         }
+        __syncthreads();
+        if (tid == 0) qcount[0] = 0;
     }
     __syncthreads();
 
