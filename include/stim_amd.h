@@ -82,8 +82,9 @@ int mfs_sample_with_faults(mfs_sampler *s, uint64_t shots, const uint64_t *shot_
                            uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs, size_t obs_stride);
 
 /* Benchmark helper: runs warmup + iters samples of `shots` into device-resident packed outputs
- * (separate observables) and reports the average per-call kernel time in milliseconds measured
- * with HIP events on the sampler's stream (out_ms[0] = frame kernel, out_ms[1] = whole call). */
+ * (separate observables) and reports average times in milliseconds measured with HIP events on the
+ * sampler's stream: out_ms[0] = the whole sampling step (all its kernels), out_ms[1] = the kernel
+ * that writes the output rows (the symptom engine's apply kernel; the frame kernel otherwise). */
 int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, double out_ms[2]);
 
 /* Sampling engine: 0 = automatic, 1 = frame engine (the reference's per-instruction frame
@@ -99,7 +100,8 @@ int mfs_set_engine(mfs_sampler *s, int engine);
 int mfs_engine_info(mfs_sampler *s, uint64_t out[6]);
 
 /* Symptom engine: target size (KB) of one workgroup's LDS image of output rows (sets the shots per
- * workgroup). Default 32. */
+ * workgroup). Default 32. A negative value selects the single-kernel variant (hits generated and
+ * applied in one kernel) with image size -kb. */
 int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb);
 
 /* Configuration knobs: words per tile (1, 2 or 4; 0 = automatic). Returns the value in use. */

@@ -277,6 +277,136 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
                    P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
 }
 
+// Generation kernel of the two-kernel path: 2^lanes_log2 lanes per tile each walk 1/2^lanes_log2 of
+// the tile's (site, shot) positions (class-ordered, geometric gaps two per Philox block, restarts at
+// class boundaries) and append packed hits to the tile's bucket, eight at a time.
+__global__ __launch_bounds__(NT) void sym_gen_kernel(SymDevParams P, uint32_t ntiles) {
+    const uint32_t S = P.shots_per_tile;
+    const uint32_t gid = blockIdx.x * NT + threadIdx.x;
+    const uint32_t tl = gid >> P.lanes_log2;             // tile within the launch
+    const uint32_t lane = gid & ((1u << P.lanes_log2) - 1);
+    if (tl >= ntiles) return;
+    const uint32_t tile = (uint32_t)(P.tile0 + tl);
+    const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
+    const uint64_t total = P.total_sites << P.tile_log2;
+    const uint64_t R = (total + (1u << P.lanes_log2) - 1) >> P.lanes_log2;
+    uint64_t pos = (uint64_t)lane * R;
+    const uint64_t hi = pos + R < total ? pos + R : total;
+    if (pos >= hi) return;
+    uint32_t lo_c = 0, hi_c = P.nclasses;
+    const uint64_t site_at = pos >> P.tile_log2;
+    while (hi_c - lo_c > 1) {
+        const uint32_t mid = (lo_c + hi_c) / 2;
+        if (P.classes[mid].site0 <= site_at) lo_c = mid;
+        else hi_c = mid;
+    }
+    uint32_t c = lo_c, ctr = 0;
+    SymClass cl = P.classes[c];
+    uint64_t cbeg = (uint64_t)cl.site0 << P.tile_log2;
+    uint64_t cend = cbeg + ((uint64_t)cl.nsites << P.tile_log2);
+    if (cend > hi) cend = hi;
+    const uint32_t shift = P.tile_log2 + 4;
+    uint32_t buf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint32_t nb = 0;
+    // Register-resident buffer (constant indices only).
+    auto push = [&](uint32_t v) {
+#pragma unroll
+        for (uint32_t i = 0; i < 8; i++) {
+            if (i == nb) buf[i] = v;
+        }
+        nb++;
+    };
+    uint32_t *bucket = P.hits + (size_t)tl * P.cap;
+    auto flush = [&]() {
+        if (!nb) return;
+        const uint32_t slot = atomicAdd(&P.counts[tl], nb);
+        if (slot + nb > P.cap) {
+            atomicOr(P.overflow, 1u);
+        } else {
+#pragma unroll
+            for (uint32_t i = 0; i < 8; i++) {
+                if (i < nb) bucket[slot + i] = buf[i];
+            }
+        }
+        nb = 0;
+    };
+    while (true) {
+        if (pos >= cend || !(cl.p > 0.0f)) {
+            pos = cend;
+            c++;
+            if (pos >= hi || c >= P.nclasses) break;
+            cl = P.classes[c];
+            cbeg = (uint64_t)cl.site0 << P.tile_log2;
+            cend = cbeg + ((uint64_t)cl.nsites << P.tile_log2);
+            if (cend > hi) cend = hi;
+            continue;
+        }
+        const uint4 r = philox10(make_uint4(ctr++, lane, tile, TAG_SYM), k0, k1);
+        const uint64_t p1 = pos + gap(r.x, cl.inv_log);
+        if (p1 >= cend) {
+            pos = cend;
+            continue;
+        }
+        {
+            const uint64_t l = p1 - cbeg;
+            const uint32_t colbase = cl.col0 + (uint32_t)(l >> P.tile_log2) * cl.basis;
+            push((colbase << shift) | ((uint32_t)(l & (S - 1)) << 4) | outcome(P.otab, cl.woff, cl.nout, r.y));
+        }
+        const uint64_t p2 = p1 + 1 + gap(r.z, cl.inv_log);
+        if (p2 < cend) {
+            const uint64_t l = p2 - cbeg;
+            const uint32_t colbase = cl.col0 + (uint32_t)(l >> P.tile_log2) * cl.basis;
+            push((colbase << shift) | ((uint32_t)(l & (S - 1)) << 4) | outcome(P.otab, cl.woff, cl.nout, r.w));
+            pos = p2 + 1;
+        } else {
+            pos = cend;
+        }
+        if (nb > 6) flush();
+    }
+    flush();
+}
+
+// Application kernel of the two-kernel path: one workgroup per tile XORs its bucket's hits into
+// the LDS image (spread evenly over the lanes) and streams the rows to HBM.
+__global__ __launch_bounds__(NT) void sym_apply_kernel(SymDevParams P) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t S = P.shots_per_tile;
+    const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
+    const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
+    Images im{lds, lds + det_words};
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < det_words + obs_words; i += NT) lds[i] = 0;
+    __syncthreads();
+    const uint32_t tl = blockIdx.x;
+    const uint32_t cnt = P.counts[tl];
+    const uint32_t n = cnt < P.cap ? cnt : P.cap;
+    const uint32_t *bucket = P.hits + (size_t)tl * P.cap;
+    const uint32_t shift = P.tile_log2 + 4;
+    for (uint32_t i = tid; i < n; i += NT) {
+        const uint32_t h = bucket[i];
+        const uint32_t code = h & 15u;
+        const uint32_t shot = (h >> 4) & (S - 1);
+        const uint32_t colbase = h >> shift;
+        uint2 rec[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) rec[j] = (code >> j) & 1 ? P.recs[colbase + j] : make_uint2(0, 0);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if ((code >> j) & 1) xor_record(P, im, rec[j], shot);
+        }
+    }
+    __syncthreads();
+    if (tid == 0) P.counts[tl] = 0;
+    const uint64_t s0 = (uint64_t)blockIdx.x * S;
+    const uint64_t valid = P.shots > s0 ? (P.shots - s0 < S ? P.shots - s0 : S) : 0;
+    if (P.dets.enabled)
+        write_rows(reinterpret_cast<const uint8_t *>(im.det), P.det_row_bytes, (uint32_t)valid,
+                   P.dets.ptr + s0 * P.dets.stride, P.dets.stride);
+    if (P.obs.enabled && P.obs_row_bytes)
+        write_rows(reinterpret_cast<const uint8_t *>(im.obs), P.obs_row_bytes, (uint32_t)valid,
+                   P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
+}
+
 // Fixed-fault mode of the symptom engine: shot s gets the XOR of the columns listed in
 // cols[off[s] .. off[s+1]).
 __global__ void sym_fixed_kernel(SymDevParams P, const uint64_t *__restrict__ off, const uint32_t *__restrict__ cols) {
@@ -359,6 +489,16 @@ hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, 
     if (!tiles) return hipSuccess;
     (void)hipFuncSetAttribute((const void *)sym_sample_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(sym_sample_kernel, dim3(tiles), dim3(NT), lds, stream, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_sym_two(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream, hipEvent_t mid) {
+    if (!tiles) return hipSuccess;
+    const uint64_t threads = (uint64_t)tiles << P.lanes_log2;
+    hipLaunchKernelGGL(sym_gen_kernel, dim3((uint32_t)((threads + NT - 1) / NT)), dim3(NT), 0, stream, P, tiles);
+    if (mid) (void)hipEventRecord(mid, stream);
+    (void)hipFuncSetAttribute((const void *)sym_apply_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(sym_apply_kernel, dim3(tiles), dim3(NT), lds, stream, P);
     return hipGetLastError();
 }
 

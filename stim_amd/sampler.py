@@ -270,8 +270,10 @@ class CompiledDetectorSampler:
         return {'engine': {1: 'frame', 2: 'symptom'}[v[0]], 'classes': v[1], 'columns': v[2], 'entries': v[3],
                 'build_ms': v[4] / 1000.0, 'sites': v[5]}
 
-    def bench_device(self, shots: int, warmup: int = 2, iters: int = 5):
-        """Average device time (ms) of one sample of `shots` into HBM-resident packed outputs."""
+    def bench_device(self, shots: int, warmup: int = 2, iters: int = 5, split: bool = False):
+        """Average device time (ms) of one sample of `shots` into HBM-resident packed outputs.
+
+        With split=True returns (whole step, output-writing kernel)."""
         out = (ctypes.c_double * 2)()
         N.check(N.lib().mfs_bench_device(self._h, int(shots), int(warmup), int(iters), out))
-        return out[0]
+        return (out[0], out[1]) if split else out[0]
