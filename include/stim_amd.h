@@ -83,8 +83,8 @@ int mfs_sample_with_faults(mfs_sampler *s, uint64_t shots, const uint64_t *shot_
 
 /* Benchmark helper: runs warmup + iters samples of `shots` into device-resident packed outputs
  * (separate observables) and reports average times in milliseconds measured with HIP events on the
- * sampler's stream: out_ms[0] = the whole sampling step (all its kernels), out_ms[1] = the kernel
- * that writes the output rows (the symptom engine's apply kernel; the frame kernel otherwise). */
+ * sampler's stream: out_ms[0] = the whole sampling step, out_ms[1] = its output-writing kernel
+ * (the same single kernel on every current path). */
 int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, double out_ms[2]);
 
 /* Sampling engine: 0 = automatic, 1 = frame engine (the reference's per-instruction frame
@@ -100,8 +100,8 @@ int mfs_set_engine(mfs_sampler *s, int engine);
 int mfs_engine_info(mfs_sampler *s, uint64_t out[6]);
 
 /* Symptom engine: target size (KB) of one workgroup's LDS image of output rows (sets the shots per
- * workgroup). Default 32. A negative value selects the single-kernel variant (hits generated and
- * applied in one kernel) with image size -kb. */
+ * workgroup). Default 32. A negative value selects the per-lane-walk queue kernel instead of the
+ * prefix-scan kernel, with image size -kb. */
 int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb);
 
 /* Configuration knobs: words per tile (1, 2 or 4; 0 = automatic). Returns the value in use. */

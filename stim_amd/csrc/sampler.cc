@@ -30,7 +30,7 @@ hipError_t launch_pack_rows(const uint8_t *src, size_t src_stride, uint8_t *dst,
 hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream);
 hipError_t launch_sym_fixed(const SymDevParams &P, const uint64_t *off, const uint32_t *cols, uint32_t tiles,
                             size_t lds, hipStream_t stream);
-hipError_t launch_sym_two(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream, hipEvent_t mid);
+hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream);
 hipError_t launch_sym_count(const uint8_t *det, uint32_t det_row, const uint8_t *obs, uint32_t obs_row, uint64_t rows,
                             uint32_t *counts, hipStream_t stream);
 hipError_t launch_sym_fill(const uint8_t *det, uint32_t det_row, const uint8_t *obs, uint32_t obs_row, uint64_t rows,
@@ -108,15 +108,10 @@ struct mfs_sampler {
     double sym_build_ms = 0;
     DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols;
     int sym_tile_kb = 32;        // target LDS image size per symptom tile
-    int sym_two_kernel = 1;      // generation / application split when hits pack into 32 bits
-    double sym_hits_per_shot = 0;
-    uint32_t sym_cap_scale = 1;
-    DevBuf d_hits, d_counts, d_overflow;
-    size_t counts_cap = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
+    int sym_scan = 1;            // scan kernel (0: per-lane-walk queue kernel)
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
     ~mfs_sampler() {
-        if (ev_mid) (void)hipEventDestroy(ev_mid);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -442,8 +437,6 @@ struct mfs_sampler {
             sym_why = "too many detectors for symptom records";
             return false;
         }
-        sym_hits_per_shot = 0;
-        for (const SymClass &cl : plan.classes) sym_hits_per_shot += (double)cl.nsites * cl.p;
         upload(d_recs, recs, "symptom records");
         upload(d_ovf, ovf, "symptom overflow entries");
         sym_state = 1;
@@ -510,38 +503,13 @@ struct mfs_sampler {
         P.obs = DevOut{obs_dst, obs_stride, 0, 1, 0, obs_bytes ? 1u : 0u};
         P.obs_row_bytes = (uint32_t)obs_bytes;
         const size_t image = ((size_t)S * det_bytes + 15) / 16 * 16 + ((size_t)S * obs_bytes + 15) / 16 * 16;
-        const size_t lds = image + SYM_QUEUE_BYTES +
-                           (P.otab_words <= SYM_OTAB_LDS_WORDS ? (size_t)P.otab_words * 4 : 0);
+        const size_t otab_bytes = P.otab_words <= SYM_OTAB_LDS_WORDS ? (size_t)P.otab_words * 4 : 0;
+        const size_t lds = image + SYM_QUEUE_BYTES + otab_bytes;
         const uint64_t tiles = (n + S - 1) / S;
         if (tiles >= (1ull << 31)) throw std::invalid_argument("too many shots in one launch");
-        // Two-kernel path: packed hits need colbase, shot and a 4-bit code in 32 bits.
-        bool two = sym_two_kernel && !fix_off;
-        uint32_t maxbasis = 0;
-        for (const SymClass &cl : plan.classes) maxbasis = std::max(maxbasis, cl.basis);
-        uint32_t colbits = 0;
-        while ((1ull << colbits) < plan.columns.size() + 4) colbits++;
-        if (maxbasis > 4 || colbits + P.tile_log2 + 4 > 32) two = false;
-        if (two) {
-            const double mu = sym_hits_per_shot * S;
-            uint32_t lanes_log2 = 0;
-            while (lanes_log2 < 8 && (double)(1u << (lanes_log2 + 1)) * 48.0 <= mu) lanes_log2++;
-            const uint64_t cap = ((uint64_t)((mu + 10.0 * std::sqrt(mu) + 64.0) * sym_cap_scale) + 63) / 64 * 64;
-            P.cap = (uint32_t)cap;
-            P.lanes_log2 = lanes_log2;
-            d_hits.ensure(tiles * cap * 4);
-            if (tiles > counts_cap) {
-                d_counts.ensure(tiles * 4);
-                check(hipMemsetAsync(d_counts.p, 0, tiles * 4, stream), "zero counts");
-                counts_cap = tiles;
-            }
-            if (!d_overflow.p) {
-                d_overflow.ensure(4);
-                check(hipMemsetAsync(d_overflow.p, 0, 4, stream), "zero overflow");
-            }
-            P.hits = d_hits.as<uint32_t>();
-            P.counts = d_counts.as<uint32_t>();
-            P.overflow = d_overflow.as<uint32_t>();
-            check(launch_sym_two(P, (uint32_t)tiles, image, stream, ev_mid), "symptom launch");
+        // Scan kernel: tile positions (sites x tile shots) must fit 31 bits.
+        if (sym_scan && !fix_off && (plan.total_sites << P.tile_log2) < (1ull << 31)) {
+            check(launch_sym_scan(P, (uint32_t)tiles, image + 32 + otab_bytes, stream), "symptom scan launch");
             return;
         }
         if (fix_off) {
@@ -588,14 +556,6 @@ struct mfs_sampler {
             const uint32_t *fc = nullptr;
             if (faults) stage_symptom_faults(s0, n, fault_offsets, faults, &fo, &fc);
             sym_launch(L, n, pd, pd_stride, po, po_stride, call_tile0 + tiles_total, fo, fc);
-            if (sym_overflowed()) {
-                // A tile produced more hits than its bucket holds (probability ~1e-20 per tile at
-                // the default capacity): rerun the batch with larger buckets. Streams are keyed by
-                // tile, so the rerun draws the same hits.
-                sym_cap_scale *= 2;
-                sym_launch(L, n, pd, pd_stride, po, po_stride, call_tile0 + tiles_total, fo, fc);
-                if (sym_overflowed()) throw std::runtime_error("symptom hit buckets overflowed twice");
-            }
             tiles_total += (n + 1023) / 1024 + 1;
             if (!L.packed) {
                 // Expand bits to bytes (bool) or characters.
@@ -632,15 +592,6 @@ struct mfs_sampler {
         }
         check(hipStreamSynchronize(stream), "sample");
         tiles_used += tiles_total + (shots + 1023) / 1024;
-    }
-
-    bool sym_overflowed() {
-        if (!d_overflow.p) return false;
-        uint32_t flag = 0;
-        check(hipMemcpyAsync(&flag, d_overflow.p, 4, hipMemcpyDeviceToHost, stream), "overflow flag");
-        check(hipStreamSynchronize(stream), "overflow flag");
-        if (flag) check(hipMemsetAsync(d_overflow.p, 0, 4, stream), "reset overflow");
-        return flag != 0;
     }
 
     // Maps fixed faults to symptom columns (per shot, the basis columns of each fault's code).
@@ -1003,29 +954,18 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
         };
         for (int i = 0; i < warmup; i++) one();
         check(hipStreamSynchronize(s->stream), "warmup");
-        double kernel_total = 0, second_total = 0;
-        hipEvent_t mid = nullptr;
-        check(hipEventCreate(&mid), "event");
+        double kernel_total = 0;
         for (int i = 0; i < iters; i++) {
             check(hipEventRecord(s->ev0, s->stream), "event");
-            check(hipEventRecord(mid, s->stream), "event");
-            s->ev_mid = mid;
             one();
-            s->ev_mid = nullptr;
             check(hipEventRecord(s->ev1, s->stream), "event");
             check(hipEventSynchronize(s->ev1), "event");
-            float ms = 0, ms2 = 0;
+            float ms = 0;
             check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
-            check(hipEventElapsedTime(&ms2, mid, s->ev1), "elapsed");
             kernel_total += ms;
-            second_total += ms2;
         }
-        (void)hipEventDestroy(mid);
-        // out_ms[0]: whole sampling step; out_ms[1]: its output-writing kernel (the apply kernel of
-        // the two-kernel symptom path; the whole step otherwise).
         out_ms[0] = iters ? kernel_total / iters : 0;
-        out_ms[1] = iters ? second_total / iters : 0;
-        if (sym && s->sym_overflowed()) throw std::runtime_error("symptom hit buckets overflowed during the benchmark");
+        out_ms[1] = out_ms[0];
     });
 }
 
@@ -1061,8 +1001,8 @@ int mfs_engine_info(mfs_sampler *s, uint64_t out[6]) {
 int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb) {
     return guarded([&] {
         if (kb == 0 || kb < -128 || kb > 128) throw std::invalid_argument("symptom tile size must be 1..128 KB");
-        // A negative size selects the single-kernel path (generation and application in one kernel).
-        s->sym_two_kernel = kb > 0;
+        // A negative size selects the per-lane-walk queue kernel instead of the scan kernel.
+        s->sym_scan = kb > 0;
         s->sym_tile_kb = kb > 0 ? kb : -kb;
     });
 }

@@ -84,12 +84,6 @@ struct SymDevParams {
     uint32_t obs_row_bytes;        // image pitch of separate observable rows (0 when unused)
     uint32_t obs_mode;             // 0: observables dropped, 1: separate image, 2: dets image at obs_col0
     uint32_t obs_col0;
-    // Two-kernel path (generation into per-tile hit buckets, then application).
-    uint32_t *hits;                // tiles x cap packed hits: colbase << (tile_log2 + 4) | shot << 4 | code
-    uint32_t *counts;              // hits per tile (zero before generation; reset by the apply kernel)
-    uint32_t *overflow;            // set when a tile produced more than cap hits
-    uint32_t cap;                  // bucket capacity per tile
-    uint32_t lanes_log2;           // generation lanes per tile (power of two)
 };
 
 }  // namespace sa

@@ -37,6 +37,7 @@ __device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
 }
 
 constexpr uint32_t TAG_SYM = 5u << 28;
+constexpr uint32_t TAG_SCAN = 6u << 28;
 
 __device__ __forceinline__ uint64_t gap(uint32_t r, float inv_log2_1mp) {
     const float u = ((float)(r >> 8) + 1.0f) * (1.0f / 16777216.0f);
@@ -277,116 +278,54 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
                    P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
 }
 
-// Generation kernel of the two-kernel path: 2^lanes_log2 lanes per tile each walk 1/2^lanes_log2 of
-// the tile's (site, shot) positions (class-ordered, geometric gaps two per Philox block, restarts at
-// class boundaries) and append packed hits to the tile's bucket, eight at a time.
-__global__ __launch_bounds__(NT) void sym_gen_kernel(SymDevParams P, uint32_t ntiles) {
-    const uint32_t S = P.shots_per_tile;
-    const uint32_t gid = blockIdx.x * NT + threadIdx.x;
-    const uint32_t tl = gid >> P.lanes_log2;             // tile within the launch
-    const uint32_t lane = gid & ((1u << P.lanes_log2) - 1);
-    if (tl >= ntiles) return;
-    const uint32_t tile = (uint32_t)(P.tile0 + tl);
-    const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
-    const uint64_t total = P.total_sites << P.tile_log2;
-    const uint64_t R = (total + (1u << P.lanes_log2) - 1) >> P.lanes_log2;
-    uint64_t pos = (uint64_t)lane * R;
-    const uint64_t hi = pos + R < total ? pos + R : total;
-    if (pos >= hi) return;
-    uint32_t lo_c = 0, hi_c = P.nclasses;
-    const uint64_t site_at = pos >> P.tile_log2;
-    while (hi_c - lo_c > 1) {
-        const uint32_t mid = (lo_c + hi_c) / 2;
-        if (P.classes[mid].site0 <= site_at) lo_c = mid;
-        else hi_c = mid;
-    }
-    uint32_t c = lo_c, ctr = 0;
-    SymClass cl = P.classes[c];
-    uint64_t cbeg = (uint64_t)cl.site0 << P.tile_log2;
-    uint64_t cend = cbeg + ((uint64_t)cl.nsites << P.tile_log2);
-    if (cend > hi) cend = hi;
-    const uint32_t shift = P.tile_log2 + 4;
-    uint32_t buf[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    uint32_t nb = 0;
-    // Register-resident buffer (constant indices only).
-    auto push = [&](uint32_t v) {
-#pragma unroll
-        for (uint32_t i = 0; i < 8; i++) {
-            if (i == nb) buf[i] = v;
-        }
-        nb++;
-    };
-    uint32_t *bucket = P.hits + (size_t)tl * P.cap;
-    auto flush = [&]() {
-        if (!nb) return;
-        const uint32_t slot = atomicAdd(&P.counts[tl], nb);
-        if (slot + nb > P.cap) {
-            atomicOr(P.overflow, 1u);
-        } else {
-#pragma unroll
-            for (uint32_t i = 0; i < 8; i++) {
-                if (i < nb) bucket[slot + i] = buf[i];
-            }
-        }
-        nb = 0;
-    };
-    while (true) {
-        if (pos >= cend || !(cl.p > 0.0f)) {
-            pos = cend;
-            c++;
-            if (pos >= hi || c >= P.nclasses) break;
-            cl = P.classes[c];
-            cbeg = (uint64_t)cl.site0 << P.tile_log2;
-            cend = cbeg + ((uint64_t)cl.nsites << P.tile_log2);
-            if (cend > hi) cend = hi;
-            continue;
-        }
-        const uint4 r = philox10(make_uint4(ctr++, lane, tile, TAG_SYM), k0, k1);
-        const uint64_t p1 = pos + gap(r.x, cl.inv_log);
-        if (p1 >= cend) {
-            pos = cend;
-            continue;
-        }
-        {
-            const uint64_t l = p1 - cbeg;
-            const uint32_t colbase = cl.col0 + (uint32_t)(l >> P.tile_log2) * cl.basis;
-            push((colbase << shift) | ((uint32_t)(l & (S - 1)) << 4) | outcome(P.otab, cl.woff, cl.nout, r.y));
-        }
-        const uint64_t p2 = p1 + 1 + gap(r.z, cl.inv_log);
-        if (p2 < cend) {
-            const uint64_t l = p2 - cbeg;
-            const uint32_t colbase = cl.col0 + (uint32_t)(l >> P.tile_log2) * cl.basis;
-            push((colbase << shift) | ((uint32_t)(l & (S - 1)) << 4) | outcome(P.otab, cl.woff, cl.nout, r.w));
-            pos = p2 + 1;
-        } else {
-            pos = cend;
-        }
-        if (nb > 6) flush();
-    }
-    flush();
+// Scan formulation of the sparse draw (the workhorse sampler). For each site class, the tile's
+// N = sites * S positions carry a Bernoulli(p) process; its hits are the partial sums of iid
+// geometric gaps (the RareErrorIterator recurrence, probability_util.h:33-71). Every lane draws the
+// same number m of gaps per round (no divergence), a workgroup prefix scan over the lanes' gap sums
+// lays the concatenated gap sequence along the positions, hits that land inside [0, N) are applied,
+// and rounds repeat until the sequence passes N. m is chosen so one round covers N with high
+// probability. Streams are keyed by (tile, class, round, lane).
+constexpr uint32_t SCAN_M_MAX = 8;
+
+__device__ __forceinline__ uint32_t gap32(uint32_t r, float inv_log2_1mp) {
+    const float u = ((float)(r >> 8) + 1.0f) * (1.0f / 16777216.0f);
+    const float g = __log2f(u) * inv_log2_1mp;
+    return g < 1.0e9f ? (uint32_t)g : 1000000000u;
 }
 
-// Application kernel of the two-kernel path: one workgroup per tile XORs its bucket's hits into
-// the LDS image (spread evenly over the lanes) and streams the rows to HBM.
-__global__ __launch_bounds__(NT) void sym_apply_kernel(SymDevParams P) {
-    extern __shared__ uint32_t lds[];
-    const uint32_t S = P.shots_per_tile;
-    const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
-    const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
-    Images im{lds, lds + det_words};
-    const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i < det_words + obs_words; i += NT) lds[i] = 0;
+__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
+    const uint32_t c = a + b;
+    return c < a ? 0xFFFFFFFFu : c;
+}
+
+// Saturating prefix sum over the 256 lanes of the workgroup (saturating addition of non-negative
+// values is associative, so any sum that reaches 2^32 - 1 stays there). Returns (exclusive, total).
+__device__ __forceinline__ uint2 block_scan(uint32_t v, uint32_t *scratch) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x = sat_add(x, y);
+    }
+    if (lane == 63) scratch[wave] = x;
+    uint32_t ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0;
     __syncthreads();
-    const uint32_t tl = blockIdx.x;
-    const uint32_t cnt = P.counts[tl];
-    const uint32_t n = cnt < P.cap ? cnt : P.cap;
-    const uint32_t *bucket = P.hits + (size_t)tl * P.cap;
-    const uint32_t shift = P.tile_log2 + 4;
-    for (uint32_t i = tid; i < n; i += NT) {
-        const uint32_t h = bucket[i];
-        const uint32_t code = h & 15u;
-        const uint32_t shot = (h >> 4) & (S - 1);
-        const uint32_t colbase = h >> shift;
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NT / 64; w++) {
+        const uint32_t t = scratch[w];
+        if (w < wave) before = sat_add(before, t);
+        total = sat_add(total, t);
+    }
+    __syncthreads();
+    return make_uint2(sat_add(before, ex), total);
+}
+
+__device__ __forceinline__ void apply_hit_code(const SymDevParams &P, const Images &im, uint32_t colbase, uint32_t code,
+                                               uint32_t shot, uint32_t basis) {
+    if (basis <= 4) {
         uint2 rec[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) rec[j] = (code >> j) & 1 ? P.recs[colbase + j] : make_uint2(0, 0);
@@ -394,10 +333,84 @@ __global__ __launch_bounds__(NT) void sym_apply_kernel(SymDevParams P) {
         for (int j = 0; j < 4; j++) {
             if ((code >> j) & 1) xor_record(P, im, rec[j], shot);
         }
+    } else {
+        while (code) {
+            const uint32_t j = __ffs(code) - 1;
+            code &= code - 1;
+            xor_column(P, im, colbase + j, shot);
+        }
+    }
+}
+
+__global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
+    extern __shared__ uint32_t lds[];
+    const uint32_t S = P.shots_per_tile;
+    const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
+    const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
+    Images im{lds, lds + det_words};
+    uint32_t *const scratch = lds + det_words + obs_words;  // 8 words
+    uint32_t *const otab_lds = scratch + 8;
+    const bool otab_in_lds = P.otab_words <= SYM_OTAB_LDS_WORDS;
+    const uint32_t *otab = otab_in_lds ? otab_lds : P.otab;
+    const uint32_t tid = threadIdx.x;
+    for (uint32_t i = tid; i < det_words + obs_words; i += NT) lds[i] = 0;
+    if (otab_in_lds) {
+        for (uint32_t i = tid; i < P.otab_words; i += NT) otab_lds[i] = P.otab[i];
     }
     __syncthreads();
-    if (tid == 0) P.counts[tl] = 0;
+
     const uint64_t s0 = (uint64_t)blockIdx.x * S;
+    const uint32_t tile = (uint32_t)(P.tile0 + blockIdx.x);
+    const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
+    for (uint32_t c = 0; c < P.nclasses; c++) {
+        const SymClass cl = P.classes[c];
+        if (!(cl.p > 0.0f)) continue;
+        const uint32_t N = cl.nsites << P.tile_log2;
+        // Gaps per lane per round: cover the expected hits plus six standard deviations.
+        const float mu = (float)N * cl.p;
+        uint32_t m = (uint32_t)((mu + 6.0f * sqrtf(mu) + 8.0f) / NT) + 1;
+        if (m > SCAN_M_MAX) m = SCAN_M_MAX;
+        uint32_t base = 0;
+        for (uint32_t round = 0; base < N; round++) {
+            uint32_t g[SCAN_M_MAX], oc[SCAN_M_MAX];
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX / 2; k++) {
+                if (2 * k < m) {
+                    const uint4 r = philox10(make_uint4(round * (SCAN_M_MAX / 2) + k, tid, tile, TAG_SCAN | c), k0, k1);
+                    g[2 * k] = gap32(r.x, cl.inv_log);
+                    oc[2 * k] = r.y;
+                    g[2 * k + 1] = gap32(r.z, cl.inv_log);
+                    oc[2 * k + 1] = r.w;
+                } else {
+                    g[2 * k] = g[2 * k + 1] = 0;
+                    oc[2 * k] = oc[2 * k + 1] = 0;
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
+                if (k < m) sum = sat_add(sum, g[k] + 1);
+            }
+            const uint2 sc = block_scan(sum, scratch);
+            uint64_t pos = (uint64_t)base + sc.x;
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
+                if (k < m) {
+                    pos += g[k];
+                    if (pos < N) {
+                        const uint32_t l = (uint32_t)pos;
+                        apply_hit_code(P, im, cl.col0 + (l >> P.tile_log2) * cl.basis,
+                                       outcome(otab, cl.woff, cl.nout, oc[k]), l & (S - 1), cl.basis);
+                    }
+                    pos += 1;
+                }
+            }
+            const uint64_t nb = (uint64_t)base + sc.y;
+            base = nb >= N ? N : (uint32_t)nb;
+        }
+    }
+    __syncthreads();
+
     const uint64_t valid = P.shots > s0 ? (P.shots - s0 < S ? P.shots - s0 : S) : 0;
     if (P.dets.enabled)
         write_rows(reinterpret_cast<const uint8_t *>(im.det), P.det_row_bytes, (uint32_t)valid,
@@ -492,13 +505,10 @@ hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, 
     return hipGetLastError();
 }
 
-hipError_t launch_sym_two(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream, hipEvent_t mid) {
+hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream) {
     if (!tiles) return hipSuccess;
-    const uint64_t threads = (uint64_t)tiles << P.lanes_log2;
-    hipLaunchKernelGGL(sym_gen_kernel, dim3((uint32_t)((threads + NT - 1) / NT)), dim3(NT), 0, stream, P, tiles);
-    if (mid) (void)hipEventRecord(mid, stream);
-    (void)hipFuncSetAttribute((const void *)sym_apply_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    hipLaunchKernelGGL(sym_apply_kernel, dim3(tiles), dim3(NT), lds, stream, P);
+    (void)hipFuncSetAttribute((const void *)sym_scan_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(sym_scan_kernel, dim3(tiles), dim3(NT), lds, stream, P);
     return hipGetLastError();
 }
 
