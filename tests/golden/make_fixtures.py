@@ -173,7 +173,47 @@ def seeded_counts(path, shots, seed):
     return bits.sum(axis=0).astype(int).tolist()
 
 
+GEN_PARAM_FLAGS = ['after_clifford_depolarization', 'before_round_data_depolarization',
+                   'before_measure_flip_probability', 'after_reset_flip_probability']
+
+
+def write_gen_pins():
+    """stim gen text for the families the build's own generator reproduces, including the edge cases
+    of its concatenation rules (one round, two rounds, zero noise, unequal noise, six-digit printing)."""
+    def uniform(p):
+        return {'after_clifford_depolarization': p, 'before_measure_flip_probability': p,
+                'after_reset_flip_probability': p} if p else {}
+    cases = [('surface_code', 'rotated_memory_z', 3, 3, uniform(1e-3)),
+             ('surface_code', 'rotated_memory_z', 7, 4, uniform(0.002)),
+             ('surface_code', 'rotated_memory_z', 11, 11, uniform(1e-4)),
+             ('surface_code', 'rotated_memory_x', 5, 5, uniform(0.003)),
+             ('repetition_code', 'memory', 3, 3, uniform(1e-3)),
+             ('repetition_code', 'memory', 11, 7, uniform(0.005)),
+             ('surface_code', 'rotated_memory_z', 5, 1, uniform(1e-3)),
+             ('surface_code', 'rotated_memory_z', 4, 2, uniform(0.00123456789)),
+             ('surface_code', 'rotated_memory_z', 2, 3, {}),
+             ('surface_code', 'rotated_memory_x', 3, 2, {'after_clifford_depolarization': 0.01,
+                                                         'before_round_data_depolarization': 0.02,
+                                                         'before_measure_flip_probability': 0.03,
+                                                         'after_reset_flip_probability': 0.04}),
+             ('surface_code', 'unrotated_memory_z', 3, 2, uniform(1e-3)),
+             ('surface_code', 'unrotated_memory_x', 4, 3, {'before_round_data_depolarization': 0.001}),
+             ('repetition_code', 'memory', 2, 1, {}),
+             ('repetition_code', 'memory', 51, 51, uniform(0.0007777777))]
+    pins = []
+    for (code, task, d, r, params) in cases:
+        args = ['gen', '--code', code, '--task', task, '--distance', str(d), '--rounds', str(r)]
+        for k, v in params.items():
+            args += ['--' + k, repr(v)]
+        pins.append({'code': code, 'task': task, 'distance': d, 'rounds': r, 'params': params,
+                     'text': run(args)})
+    json.dump(pins, open(os.path.join(HERE, 'gen_pins.json'), 'w'), indent=0)
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == '--gen-pins-only':
+        write_gen_pins()
+        return
     if not os.path.isdir('/root/reference') or not os.path.exists(STIM):
         sys.exit('make_fixtures.py needs /root/reference and oracle/_ref/stim (make -C oracle ref)')
     rng = random.Random(20261202)
@@ -194,17 +234,7 @@ def main():
         with open(os.path.join(cdir, name + '.stim'), 'w') as f:
             f.write(text)
 
-    # Generator pins: stim gen text for the families the build's own generator reproduces.
-    gen_pins = []
-    for (code, task, d, r, p) in [('surface_code', 'rotated_memory_z', 3, 3, 1e-3),
-                                  ('surface_code', 'rotated_memory_z', 7, 4, 0.002),
-                                  ('surface_code', 'rotated_memory_z', 11, 11, 1e-4),
-                                  ('surface_code', 'rotated_memory_x', 5, 5, 0.003),
-                                  ('repetition_code', 'memory', 3, 3, 1e-3),
-                                  ('repetition_code', 'memory', 11, 7, 0.005)]:
-        gen_pins.append({'code': code, 'task': task, 'distance': d, 'rounds': r, 'p': p,
-                         'text': gen(code, task, d, r, p)})
-    json.dump(gen_pins, open(os.path.join(HERE, 'gen_pins.json'), 'w'), indent=0)
+    write_gen_pins()
 
     # Fixed-fault vectors from the reference sampler.
     fixtures = []
