@@ -487,8 +487,9 @@ struct mfs_sampler {
     }
 
     // Sampled (or fixed-fault) shots through the symptom engine into packed device rows.
-    void sym_launch(const Layout &L, uint64_t n, uint8_t *det_dst, size_t det_stride, uint8_t *obs_dst,
-                    size_t obs_stride, uint64_t tile0, const uint64_t *fix_off, const uint32_t *fix_cols) {
+    // Launches the symptom engine over n shots starting at RNG tile tile0; returns the tiles used.
+    uint64_t sym_launch(const Layout &L, uint64_t n, uint8_t *det_dst, size_t det_stride, uint8_t *obs_dst,
+                        size_t obs_stride, uint64_t tile0, const uint64_t *fix_off, const uint32_t *fix_cols) {
         const uint32_t D = prog.num_detectors, O = prog.num_observables;
         const size_t obs_bytes = (L.separate && obs_dst) ? (O + 7) / 8 : 0;
         const size_t det_bytes = (L.ncols + 7) / 8;
@@ -510,13 +511,14 @@ struct mfs_sampler {
         // Scan kernel: tile positions (sites x tile shots) must fit 31 bits.
         if (sym_scan && !fix_off && (plan.total_sites << P.tile_log2) < (1ull << 31)) {
             check(launch_sym_scan(P, (uint32_t)tiles, image + 32 + otab_bytes, stream), "symptom scan launch");
-            return;
+            return tiles;
         }
         if (fix_off) {
             check(launch_sym_fixed(P, fix_off, fix_cols, (uint32_t)tiles, lds, stream), "symptom fixed launch");
         } else {
             check(launch_sym_sample(P, (uint32_t)tiles, lds, stream), "symptom sample launch");
         }
+        return tiles;
     }
 
     void sample_symptom(const Layout &L, uint64_t shots, bool device_out, uint8_t *dets, size_t dets_stride,
@@ -555,8 +557,7 @@ struct mfs_sampler {
             const uint64_t *fo = nullptr;
             const uint32_t *fc = nullptr;
             if (faults) stage_symptom_faults(s0, n, fault_offsets, faults, &fo, &fc);
-            sym_launch(L, n, pd, pd_stride, po, po_stride, call_tile0 + tiles_total, fo, fc);
-            tiles_total += (n + 1023) / 1024 + 1;
+            tiles_total += sym_launch(L, n, pd, pd_stride, po, po_stride, call_tile0 + tiles_total, fo, fc);
             if (!L.packed) {
                 // Expand bits to bytes (bool) or characters.
                 uint8_t *ud = device_out ? dets + s0 * dets_stride : nullptr;
@@ -591,7 +592,7 @@ struct mfs_sampler {
             }
         }
         check(hipStreamSynchronize(stream), "sample");
-        tiles_used += tiles_total + (shots + 1023) / 1024;
+        tiles_used += tiles_total;
     }
 
     // Maps fixed faults to symptom columns (per shot, the basis columns of each fault's code).
@@ -938,9 +939,8 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
         const bool sym = s->use_symptoms(false);
         auto one = [&]() {
             if (sym) {
-                s->sym_launch(L, shots, dd.as<uint8_t>(), L.det_row, od.as<uint8_t>(), L.obs_row, s->tiles_used, nullptr,
-                              nullptr);
-                s->tiles_used += shots / 1024 + 1;
+                s->tiles_used += s->sym_launch(L, shots, dd.as<uint8_t>(), L.det_row, od.as<uint8_t>(), L.obs_row,
+                                               s->tiles_used, nullptr, nullptr);
                 return;
             }
             LaunchParams P = s->base_params();

@@ -305,6 +305,26 @@ def test_d25_full_size_rates_and_shape(golden, engine):
     np.testing.assert_array_equal(again, dets[:1024])
 
 
+@pytest.mark.parametrize('engine', ENGINES)
+def test_consecutive_calls_draw_fresh_randomness(engine):
+    """Each call continues the sampler's random stream: no row block of one call reappears in the
+    next (a reused stream would repeat whole tiles of rows), and a call split in two has the rate of
+    one call."""
+    from stim_amd import gen
+    text = gen.generate('surface_code:rotated_memory_z', distance=5, rounds=5, after_clifford_depolarization=0.02,
+                        after_reset_flip_probability=0.02, before_measure_flip_probability=0.02)
+    s = _sampler(text, seed=11, engine=engine)
+    calls = [s.sample(8192, bit_packed=True) for _ in range(3)]
+    weight = [np.unpackbits(c, axis=1).sum(axis=1) for c in calls]
+    seen = set()
+    for c, w in zip(calls[:2], weight[:2]):
+        seen.update(bytes(r) for r, k in zip(c, w) if k >= 12)
+    heavy = [bytes(r) for r, k in zip(calls[2], weight[2]) if k >= 12]
+    assert len(heavy) > 1000
+    repeats = sum(r in seen for r in heavy)
+    assert repeats <= 2, repeats
+
+
 def test_engines_agree_statistically_on_zoo():
     """Frame and symptom engines sample the same distribution on an all-gate circuit."""
     text = zoo.gate_zoo(7, n_qubits=10, layers=12)
