@@ -104,6 +104,35 @@ int mfs_engine_info(mfs_sampler *s, uint64_t out[6]);
  * prefix-scan kernel, with image size -kb. */
 int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb);
 
+/* ---- Batches: many sampling jobs in one persistent kernel ----------------------------------------
+ * No reference counterpart (the reference samples one circuit per call). A batch holds jobs, each a
+ * sampler, a shot count and device-resident bit-packed output rows (dets: shots rows of >= ceil(D/8)
+ * bytes at dets_stride; obs: shots rows of >= ceil(O/8) bytes at obs_stride, or NULL to drop the
+ * observables). Jobs may come from different circuits. A run samples every job: symptom-engine jobs in
+ * one persistent kernel that hands out shot tiles from a work counter, heaviest tiles first; other
+ * jobs as their own launches. Each job draws exactly what mfs_sample(s, shots, MFS_BIT_PACKED |
+ * MFS_SEPARATE_OBSERVABLES | MFS_DEVICE_OUTPUT, ...) would at the same point of its sampler's stream
+ * with the same tile size, and advances the sampler's stream the same way. A batch and its samplers
+ * are single-threaded; samplers must outlive the batch. */
+typedef struct mfs_batch mfs_batch;
+int mfs_batch_create(mfs_batch **out);
+void mfs_batch_destroy(mfs_batch *b);
+int mfs_batch_add(mfs_batch *b, mfs_sampler *s, uint64_t shots, uint8_t *dets, size_t dets_stride, uint8_t *obs,
+                  size_t obs_stride);
+/* LDS output image per workgroup in KB for every job (0 = each sampler's own setting). */
+int mfs_batch_set_tile_kb(mfs_batch *b, int kb);
+/* Runs every job once and waits for the results. use_graph != 0 captures the persistent kernel and its
+ * bookkeeping kernels into a HIP graph on the first run and replays it afterwards. */
+int mfs_batch_run(mfs_batch *b, int use_graph);
+/* Runs warmup + iters times; out_ms[0] = average device time per run (HIP events on the batch's
+ * stream), out_ms[1] = average wall time per run. */
+int mfs_batch_time(mfs_batch *b, int use_graph, int warmup, int iters, double out_ms[2]);
+/* Autotuner: times iters runs for each candidate tile size (KB) and keeps the fastest, also setting it
+ * on every job's sampler so plain calls stay comparable. */
+int mfs_batch_autotune(mfs_batch *b, const int *candidates, int n, int iters, int *best_kb, double *best_ms);
+/* out[0..5] = jobs, persistent-kernel jobs, separately launched jobs, tiles, workgroups, LDS bytes. */
+int mfs_batch_info(mfs_batch *b, uint64_t out[6]);
+
 /* Configuration knobs: words per tile (1, 2 or 4; 0 = automatic). Returns the value in use. */
 int mfs_set_tile_words(mfs_sampler *s, int words);
 int mfs_get_info(const mfs_sampler *s, uint64_t out[8]); /* tile_words, lds_bytes, ops, syncs, noise_ops, randomize, ring, dense_qubits */

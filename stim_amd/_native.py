@@ -25,6 +25,8 @@ EXPORTED = [
     'mfs_compile', 'mfs_destroy', 'mfs_circuit_stats', 'mfs_parse_stats', 'mfs_sample', 'mfs_sample_write',
     'mfs_sample_with_faults', 'mfs_bench_device', 'mfs_set_tile_words', 'mfs_get_info', 'mfs_last_error',
     'mfs_set_engine', 'mfs_engine_info', 'mfs_set_symptom_tile_kb',
+    'mfs_batch_create', 'mfs_batch_destroy', 'mfs_batch_add', 'mfs_batch_set_tile_kb', 'mfs_batch_run',
+    'mfs_batch_time', 'mfs_batch_autotune', 'mfs_batch_info',
 ]
 
 _lib = None
@@ -36,6 +38,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError(f'{LIB_PATH} is not built; run stim_amd/csrc/build.sh (or __graft_entry__.build())')
+    # PyTorch ships its own HIP runtime. When torch is installed, load it first so that this library
+    # binds to the same runtime (its libamdhip64.so.7 dependency then resolves to the one already
+    # loaded); a second runtime loaded before torch's leaves torch unable to see the GPU.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     u64, u32, sz, p = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_size_t, ctypes.c_void_p
     L.mfs_compile.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.POINTER(p)]
@@ -54,10 +63,20 @@ def lib():
     L.mfs_set_engine.argtypes = [p, ctypes.c_int]
     L.mfs_engine_info.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_set_symptom_tile_kb.argtypes = [p, ctypes.c_int]
+    L.mfs_batch_create.argtypes = [ctypes.POINTER(p)]
+    L.mfs_batch_destroy.argtypes = [p]
+    L.mfs_batch_destroy.restype = None
+    L.mfs_batch_add.argtypes = [p, p, u64, p, sz, p, sz]
+    L.mfs_batch_set_tile_kb.argtypes = [p, ctypes.c_int]
+    L.mfs_batch_run.argtypes = [p, ctypes.c_int]
+    L.mfs_batch_time.argtypes = [p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.mfs_batch_autotune.argtypes = [p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
+                                     ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
+    L.mfs_batch_info.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_last_error.argtypes = []
     L.mfs_last_error.restype = ctypes.c_char_p
     for name in EXPORTED:
-        if name not in ('mfs_destroy', 'mfs_last_error', 'mfs_compile'):
+        if name not in ('mfs_destroy', 'mfs_last_error', 'mfs_compile', 'mfs_batch_destroy'):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

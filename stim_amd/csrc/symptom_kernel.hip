@@ -154,7 +154,7 @@ __device__ void write_rows(const uint8_t *img, uint32_t pitch, uint32_t rows, ui
 constexpr uint32_t HITS_PER_ROUND = 8;
 
 __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];
     const uint32_t S = P.shots_per_tile;
     const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
     const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
@@ -323,27 +323,79 @@ __device__ __forceinline__ uint2 block_scan(uint32_t v, uint32_t *scratch) {
     return make_uint2(sat_add(before, ex), total);
 }
 
-__device__ __forceinline__ void apply_hit_code(const SymDevParams &P, const Images &im, uint32_t colbase, uint32_t code,
-                                               uint32_t shot, uint32_t basis) {
+// Where a hit's symptom entries land in a tile's LDS image. Entry m < D (a detector) of shot s is bit
+// s * det_bits + det_col0 + m of the image; entry m >= D (observable m - D) is bit
+// obs_base + s * obs_bits + obs_col0 + (m - D), or is dropped when observables are not written. The
+// per-shot parts are computed once per hit (dbase, obase), so an entry costs a select, a shift and
+// one LDS XOR, with no branch.
+struct Sink {
+    uint32_t *img;
+    uint32_t D;
+    uint32_t det_bits, det_col0;
+    uint32_t obs_base, obs_bits, obs_col0;
+    bool obs_on;
+};
+
+__device__ __forceinline__ Sink make_sink(const SymDevParams &P, uint32_t *lds, uint32_t det_words) {
+    Sink k;
+    k.img = lds;
+    k.D = P.D;
+    k.det_bits = P.det_row_bytes * 8;
+    k.det_col0 = P.dets.col0;
+    k.obs_on = P.obs_mode != 0;
+    if (P.obs_mode == 2) {
+        k.obs_base = 0;
+        k.obs_bits = k.det_bits;
+        k.obs_col0 = P.obs_col0;
+    } else {
+        k.obs_base = det_words * 32;
+        k.obs_bits = P.obs_row_bytes * 8;
+        k.obs_col0 = 0;
+    }
+    return k;
+}
+
+__device__ __forceinline__ void sink_bit(const Sink &k, uint32_t m, uint32_t dbase, uint32_t obase) {
+    const bool det = m < k.D;
+    const uint32_t bit = (det ? dbase : obase) + m;
+    if (det || k.obs_on) atomicXor(&k.img[bit >> 5], 1u << (bit & 31));
+}
+
+__device__ __forceinline__ void sink_record(const Sink &k, const uint32_t *ovf, uint2 rec, uint32_t dbase,
+                                            uint32_t obase) {
+    const uint32_t c = rec.x >> 28;
+    if (c == 3) {
+        const uint32_t off = rec.x & 0x0FFFFFFFu;
+        for (uint32_t e = 0; e < rec.y; e++) sink_bit(k, ovf[off + e], dbase, obase);
+        return;
+    }
+    if (c >= 1) sink_bit(k, rec.x & 0x0FFFFFFFu, dbase, obase);
+    if (c == 2) sink_bit(k, rec.y, dbase, obase);
+}
+
+__device__ __forceinline__ void sink_hit(const SymDevParams &P, const Sink &k, uint32_t colbase, uint32_t code,
+                                         uint32_t shot, uint32_t basis) {
+    const uint32_t dbase = shot * k.det_bits + k.det_col0;
+    const uint32_t obase = k.obs_base + shot * k.obs_bits + k.obs_col0 - k.D;
     if (basis <= 4) {
         uint2 rec[4];
 #pragma unroll
         for (int j = 0; j < 4; j++) rec[j] = (code >> j) & 1 ? P.recs[colbase + j] : make_uint2(0, 0);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            if ((code >> j) & 1) xor_record(P, im, rec[j], shot);
+            if ((code >> j) & 1) sink_record(k, P.ovf, rec[j], dbase, obase);
         }
     } else {
         while (code) {
             const uint32_t j = __ffs(code) - 1;
             code &= code - 1;
-            xor_column(P, im, colbase + j, shot);
+            sink_record(k, P.ovf, P.recs[colbase + j], dbase, obase);
         }
     }
 }
 
-__global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
-    extern __shared__ uint32_t lds[];
+// One tile of the scan sampler: tile `btile` of the launch described by P, in the workgroup's LDS.
+__device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile, uint32_t *lds) {
     const uint32_t S = P.shots_per_tile;
     const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
     const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
@@ -353,14 +405,19 @@ __global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
     const bool otab_in_lds = P.otab_words <= SYM_OTAB_LDS_WORDS;
     const uint32_t *otab = otab_in_lds ? otab_lds : P.otab;
     const uint32_t tid = threadIdx.x;
-    for (uint32_t i = tid; i < det_words + obs_words; i += NT) lds[i] = 0;
+    {
+        // The image is a whole number of 16-byte words.
+        uint4 *const v = reinterpret_cast<uint4 *>(lds);
+        for (uint32_t i = tid; i < (det_words + obs_words) / 4; i += NT) v[i] = make_uint4(0, 0, 0, 0);
+    }
     if (otab_in_lds) {
         for (uint32_t i = tid; i < P.otab_words; i += NT) otab_lds[i] = P.otab[i];
     }
+    const Sink sink = make_sink(P, lds, det_words);
     __syncthreads();
 
-    const uint64_t s0 = (uint64_t)blockIdx.x * S;
-    const uint32_t tile = (uint32_t)(P.tile0 + blockIdx.x);
+    const uint64_t s0 = (uint64_t)btile * S;
+    const uint32_t tile = (uint32_t)(P.tile0 + btile);
     const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
     for (uint32_t c = 0; c < P.nclasses; c++) {
         const SymClass cl = P.classes[c];
@@ -399,8 +456,8 @@ __global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
                     pos += g[k];
                     if (pos < N) {
                         const uint32_t l = (uint32_t)pos;
-                        apply_hit_code(P, im, cl.col0 + (l >> P.tile_log2) * cl.basis,
-                                       outcome(otab, cl.woff, cl.nout, oc[k]), l & (S - 1), cl.basis);
+                        sink_hit(P, sink, cl.col0 + (l >> P.tile_log2) * cl.basis,
+                                 outcome(otab, cl.woff, cl.nout, oc[k]), l & (S - 1), cl.basis);
                     }
                     pos += 1;
                 }
@@ -420,10 +477,54 @@ __global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
                    P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
 }
 
+__global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    scan_tile(P, blockIdx.x, lds);
+}
+
+// Batched form: many launches (jobs, possibly of different circuits) in one persistent kernel.
+// Workgroups take global tile numbers from a counter in order; job j owns tiles
+// [prefix[j], prefix[j+1]). Every workgroup leaves once the counter passes the last tile.
+__global__ __launch_bounds__(NT) void sym_batch_kernel(const SymDevParams *__restrict__ jobs,
+                                                       const uint64_t *__restrict__ prefix, uint32_t njobs,
+                                                       uint32_t *counter) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    __shared__ uint32_t next;
+    const uint64_t total = prefix[njobs];
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+        __syncthreads();
+        const uint32_t t = next;
+        if (t >= total) break;
+        uint32_t lo = 0, hi = njobs;
+        while (hi - lo > 1) {
+            const uint32_t mid = (lo + hi) / 2;
+            if (prefix[mid] <= t) lo = mid;
+            else hi = mid;
+        }
+        const uint32_t j = __builtin_amdgcn_readfirstlane(lo);
+        const SymDevParams P = jobs[j];
+        scan_tile(P, (uint32_t)(t - prefix[j]), lds);
+    }
+}
+
+// Batch bookkeeping kernels (kernels rather than memory operations, so the batch can be captured in a
+// graph): reset the tile counter before a run; advance every job's RNG tile base by its tile count
+// after it.
+__global__ void sym_batch_reset_kernel(uint32_t *counter) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
+}
+
+__global__ void sym_batch_advance_kernel(SymDevParams *jobs, const uint64_t *prefix, uint32_t njobs) {
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < njobs) jobs[j].tile0 += prefix[j + 1] - prefix[j];
+}
+
 // Fixed-fault mode of the symptom engine: shot s gets the XOR of the columns listed in
 // cols[off[s] .. off[s+1]).
 __global__ void sym_fixed_kernel(SymDevParams P, const uint64_t *__restrict__ off, const uint32_t *__restrict__ cols) {
-    extern __shared__ uint32_t lds[];
+    extern __shared__ __align__(16) uint32_t lds[];
     const uint32_t S = P.shots_per_tile;
     const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
     const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
@@ -510,6 +611,22 @@ hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hi
     (void)hipFuncSetAttribute((const void *)sym_scan_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     hipLaunchKernelGGL(sym_scan_kernel, dim3(tiles), dim3(NT), lds, stream, P);
     return hipGetLastError();
+}
+
+hipError_t launch_sym_batch(SymDevParams *jobs, const uint64_t *prefix, uint32_t njobs, uint32_t *counter,
+                            uint32_t grid, size_t lds, hipStream_t stream) {
+    (void)hipFuncSetAttribute((const void *)sym_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    hipLaunchKernelGGL(sym_batch_reset_kernel, dim3(1), dim3(64), 0, stream, counter);
+    hipLaunchKernelGGL(sym_batch_kernel, dim3(grid), dim3(NT), lds, stream, jobs, prefix, njobs, counter);
+    hipLaunchKernelGGL(sym_batch_advance_kernel, dim3((njobs + 255) / 256), dim3(256), 0, stream, jobs, prefix, njobs);
+    return hipGetLastError();
+}
+
+int sym_batch_blocks_per_cu(size_t lds) {
+    (void)hipFuncSetAttribute((const void *)sym_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sym_batch_kernel, NT, lds) != hipSuccess) return 1;
+    return n > 0 ? n : 1;
 }
 
 hipError_t launch_sym_fixed(const SymDevParams &P, const uint64_t *off, const uint32_t *cols, uint32_t tiles,
