@@ -1,18 +1,23 @@
 #!/usr/bin/env python3
-"""Benchmark: detector shots/s of the d=25, 25-round rotated surface code at p=1e-3 on one MI355X.
+"""Benchmarks of the MI355X detection-event sampler. Each run prints one JSON line.
 
-One step = one GPU sample of 2^20 shots (bit-packed detection events + separate observables) into
-HBM-resident output rows. Prints one JSON line (see the "metric"/"roofline"/"cpu_baseline" keys).
+Default workload (the headline): detector shots/s of the d=25, 25-round rotated surface-code memory
+circuit at p=1e-3 on one MI355X. One step = one sample of 2^20 shots (bit-packed detection events +
+separate observables) into HBM-resident output rows.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots S] [--no-cpu]
+    python bench.py --workload batch64      # 64 heterogeneous surface codes: one persistent kernel vs 64 calls
+    python bench.py --workload serve        # request loop with continuous batching, p50/p99 latency
 """
 
 import argparse
 import json
+import math
 import os
+import random
+import re
 import subprocess
 import sys
-import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -21,11 +26,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 CIRCUIT = os.path.join(ROOT, 'tests', 'golden', 'circuits', 'surface_d25_r25.stim')
 REF_STIM = os.path.join(ROOT, 'oracle', '_ref', 'stim')
+TRAFFIC = os.path.join(ROOT, 'profiles', 'sym_scan_kernel_d25_r25_2p20_traffic.md')
 
 
-def cpu_baseline(circuit_path, n_det, n_obs):
-    """Reference CLI `stim detect` (built from the reference's sources into oracle/_ref), one process
-    per core, on a bounded sample; falls back to the numpy oracle when the CLI is absent."""
+def cpu_baseline(circuit_path):
+    """The reference CLI `stim detect` (built from the reference's sources into oracle/_ref), one process
+    per core, on a bounded sample; the numpy oracle port when the CLI is absent."""
     try:
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
@@ -47,31 +53,35 @@ def cpu_baseline(circuit_path, n_det, n_obs):
                     'sample': f'stim detect b8 to /dev/null, {cores} processes x {shots} shots '
                               f'(d=25 r=25 p=1e-3), wall {wall:.2f}s'}
     from oracle import frame_oracle as fo
+    import numpy as np
     text = open(circuit_path).read()
     shots = 64
     t0 = time.perf_counter()
-    fo.FrameOracle(text).run(shots, rng=__import__('numpy').random.default_rng(0))
+    fo.FrameOracle(text).run(shots, rng=np.random.default_rng(0))
     wall = time.perf_counter() - t0
     return {'value': shots / wall, 'unit': 'shots/s', 'cores': 1, 'kind': 'port',
             'sample': f'numpy oracle, 1 process x {shots} shots, wall {wall:.2f}s'}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument('--gpus', type=int, default=1)
-    ap.add_argument('--steps', type=int, default=20)
-    ap.add_argument('--warmup', type=int, default=3)
-    ap.add_argument('--shots', type=int, default=1 << 20)
-    ap.add_argument('--no-cpu', action='store_true')
-    ap.add_argument('--tile-words', type=int, default=0)
-    ap.add_argument('--calibrate-copy', action='store_true',
-                    help='run a 1 GiB device-to-device copy first (reference for FETCH_SIZE/WRITE_SIZE counters)')
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print('bench.py measures one GPU only: this build has no multi-GPU path (shots would shard '
-              'across GPUs with no exchange; see DESIGN.md). Run with --gpus 1.', file=sys.stderr)
-        sys.exit(2)
+def measured_traffic():
+    """HBM bytes per launch of the headline kernel, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
+    summary (scaled by the 1 GiB copy calibration recorded there)."""
+    try:
+        text = open(TRAFFIC).read()
+    except OSError:
+        return None
+    m = re.search(r'^traffic_bytes_per_launch:\s*(\d+)', text, re.M)
+    return int(m.group(1)) if m else None
 
+
+def device_buffers(torch, shots, D, O):
+    drow, orow = (D + 7) // 8, (O + 7) // 8
+    dets = torch.empty((shots, max(drow, 1)), dtype=torch.uint8, device='cuda:0')
+    obs = torch.empty((shots, max(orow, 1)), dtype=torch.uint8, device='cuda:0')
+    return dets, obs, drow, orow
+
+
+def headline(args):
     import torch
     import stim_amd
     from stim_amd import _native as N
@@ -83,13 +93,11 @@ def main():
         sampler.set_tile_words(args.tile_words)
     D, O = sampler.num_detectors, sampler.num_observables
     S = args.shots
-    drow, orow = (D + 7) // 8, (O + 7) // 8
-    dev = torch.device('cuda:0')
-    dets = torch.empty((S, drow), dtype=torch.uint8, device=dev)
-    obs = torch.empty((S, max(orow, 1)), dtype=torch.uint8, device=dev)
+    dets, obs, drow, orow = device_buffers(torch, S, D, O)
     flags = N.MFS_BIT_PACKED | N.MFS_SEPARATE_OBSERVABLES | N.MFS_DEVICE_OUTPUT
     if args.calibrate_copy:
-        src = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
+        # A device-to-device copy of exactly 1 GiB: the reference dispatch for FETCH_SIZE / WRITE_SIZE.
+        src = torch.ones(1 << 30, dtype=torch.uint8, device='cuda:0')
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
@@ -97,7 +105,7 @@ def main():
     L = N.lib()
 
     def step():
-        N.check(L.mfs_sample(sampler._h, S, flags, dets.data_ptr(), drow, obs.data_ptr(), max(orow, 1)))
+        N.check(L.mfs_sample(sampler._h, S, flags, dets.data_ptr(), dets.stride(0), obs.data_ptr(), obs.stride(0)))
 
     for _ in range(args.warmup):
         step()
@@ -111,9 +119,8 @@ def main():
     value = args.steps * S / wall
 
     # Kernel-only duration from HIP events on the sampler's own stream.
-    kernel_ms = sampler.bench_device(S, warmup=1, iters=max(3, min(args.steps, 10)))
+    kernel_ms = sampler.bench_device(S, warmup=5, iters=max(10, min(args.steps, 50)))
     engine = sampler.engine_info()
-    # The other engine, measured the same way, for comparison.
     other = 'frame' if engine['engine'] == 'symptom' else 'symptom'
     engines = {engine['engine']: {'shots_per_s': value, 'kernel_ms': kernel_ms}}
     try:
@@ -125,13 +132,14 @@ def main():
         pass
     b_alg = S * (drow + orow)
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
-    info = sampler.info()
-
-    cpu = None
-    if not args.no_cpu:
-        cpu = cpu_baseline(CIRCUIT, D, O)
-
-    line = {
+    traffic = measured_traffic() if S == 1 << 20 and engine['engine'] == 'symptom' else None
+    cpu = None if args.no_cpu else cpu_baseline(CIRCUIT)
+    if engine['engine'] == 'symptom':
+        tile = {'tile_shots': engine['tile_shots'], 'lds_bytes_per_tile': engine['lds_bytes_per_tile']}
+    else:
+        info = sampler.info()
+        tile = {'tile_shots': 32 * info['tile_words'], 'lds_bytes_per_tile': info['lds_bytes']}
+    return {
         'metric': 'detector shots/s @ d=25 r=25 p=1e-3 (HBM GB/s vs peak); served p50/p99 ms',
         'value': value,
         'unit': 'shots/s',
@@ -142,21 +150,148 @@ def main():
         'higher_is_better': True,
         'scaling': 'weak',
         'vs_baseline': None,
-        'dtype': 'u32 (GF(2) bit words)',
+        'dtype': 'u32',
         'data': 'synthetic: rotated surface code memory-Z d=25 r=25 p=1e-3 (stim gen text), seeded Philox noise',
-        'config': {'workload': 'surface_code:rotated_memory_z d=25 rounds=25 p=1e-3, 2^20 shots, '
-                               'bit_packed dets + separate observables, outputs resident in HBM',
-                   'shots_per_step': S, 'detectors': D, 'observables': O,
-                   'tile_shots': 32 * info['tile_words'], 'lds_bytes_per_tile': info['lds_bytes'],
-                   'parallelism': 'single GPU'},
+        'config': dict({'workload': 'surface_code:rotated_memory_z d=25 rounds=25 p=1e-3, 2^20 shots, '
+                                    'bit_packed dets + separate observables, outputs resident in HBM',
+                        'shots_per_step': S, 'detectors': D, 'observables': O, 'parallelism': 'single GPU'}, **tile),
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'kernel': 'sym_sample_kernel' if engine['engine'] == 'symptom' else 'frame_sim_kernel',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
+                     'kernel': 'sym_scan_kernel' if engine['engine'] == 'symptom' else 'frame_sim_kernel',
                      'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
         'engine': engine,
         'engines': engines,
         'cpu_baseline': cpu,
     }
+
+
+def batch64_configs(seed=2026, n=64):
+    """64 rotated memory-Z surface codes: d uniform over {3, 5, ..., 21}, rounds = d, p log-uniform over
+    [1e-4, 5e-3], shots log-uniform over [1e4, 1e7]; the draw is seeded."""
+    rng = random.Random(seed)
+    out = []
+    for _ in range(n):
+        d = rng.choice(range(3, 22, 2))
+        p = math.exp(rng.uniform(math.log(1e-4), math.log(5e-3)))
+        shots = int(math.exp(rng.uniform(math.log(1e4), math.log(1e7))))
+        out.append((d, p, shots))
+    return out
+
+
+def batch64(args):
+    import torch
+    import stim_amd
+    from stim_amd import _native as N
+    from stim_amd import gen
+    from stim_amd.batch import BatchSampler
+
+    cfgs = batch64_configs()
+    texts = [gen.generate('surface_code:rotated_memory_z', distance=d, rounds=d, after_clifford_depolarization=p,
+                          after_reset_flip_probability=p, before_measure_flip_probability=p) for d, p, _ in cfgs]
+    t0 = time.perf_counter()
+    batch_s = [stim_amd.Circuit(t).compile_detector_sampler(seed=1000 + k) for k, t in enumerate(texts)]
+    plain_s = [stim_amd.Circuit(t).compile_detector_sampler(seed=1000 + k) for k, t in enumerate(texts)]
+    tune_s = [stim_amd.Circuit(t).compile_detector_sampler(seed=5000 + k) for k, t in enumerate(texts)]
+    for s in batch_s + plain_s + tune_s:
+        s.engine_info()      # builds the symptom tables
+    compile_s = time.perf_counter() - t0
+    shots = [c[2] for c in cfgs]
+    total_shots = sum(shots)
+    total_bytes = sum(n * ((s.num_detectors + 7) // 8 + (s.num_observables + 7) // 8) for s, n in zip(batch_s, shots))
+
+    # Autotuner on its own samplers (so the compared samplers start at the same stream position).
+    tuner = BatchSampler(list(zip(tune_s, shots)))
+    kb, tune_ms = tuner.autotune(candidates=(8, 16, 32, 48), iters=1)
+    del tuner
+    for s in batch_s + plain_s:
+        s.set_symptom_tile_kb(kb)
+    batch = BatchSampler(list(zip(batch_s, shots)))
+    batch.set_tile_kb(kb)
+    info = batch.info()
+
+    # Results: the batch (graph replay) against 64 plain calls, bit for bit, on the device.
+    outs = batch.run(use_graph=True)
+    flags = N.MFS_BIT_PACKED | N.MFS_SEPARATE_OBSERVABLES | N.MFS_DEVICE_OUTPUT
+    L = N.lib()
+    mismatches = 0
+    for (bd, bo), s, n in zip(outs, plain_s, shots):
+        d = torch.empty_like(bd)
+        o = torch.empty_like(bo)
+        N.check(L.mfs_sample(s._h, n, flags, d.data_ptr(), d.stride(0), o.data_ptr(), o.stride(0)))
+        mismatches += int(not (torch.equal(d, bd) and torch.equal(o, bo)))
+        del d, o
+    torch.cuda.synchronize()
+
+    # Timing: graph-replayed persistent kernel vs 64 separate plain calls (each call synchronous, as the
+    # reference's sample() is).
+    dev_ms, wall_ms = batch.time(use_graph=True, warmup=1, iters=args.steps)
+    bufs = []
+    for s, n in zip(plain_s, shots):
+        d, o, _, _ = device_buffers(torch, n, s.num_detectors, s.num_observables)
+        bufs.append((d, o))
+    for (d, o), s, n in zip(bufs, plain_s, shots):
+        N.check(L.mfs_sample(s._h, n, flags, d.data_ptr(), d.stride(0), o.data_ptr(), o.stride(0)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        for (d, o), s, n in zip(bufs, plain_s, shots):
+            N.check(L.mfs_sample(s._h, n, flags, d.data_ptr(), d.stride(0), o.data_ptr(), o.stride(0)))
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) / args.steps * 1e3
+    return {
+        'metric': 'batched detector shots/s: 64 heterogeneous surface codes in one persistent kernel',
+        'value': total_shots / (wall_ms * 1e-3),
+        'unit': 'shots/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': 1,
+        'ms_per_step': wall_ms,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'u32',
+        'data': 'synthetic: 64 rotated memory-Z surface codes, d in {3..21}, r=d, p log-uniform [1e-4,5e-3], '
+                'shots log-uniform [1e4,1e7] (seeded draw 2026)',
+        'config': {'workload': 'batch64', 'circuits': len(cfgs), 'total_shots': total_shots,
+                   'output_bytes': total_bytes, 'autotuned_tile_kb': kb, 'autotune_best_ms': tune_ms,
+                   'parallelism': 'single GPU', **info},
+        'batch_device_ms': dev_ms,
+        'batch_wall_ms': wall_ms,
+        'plain_64_calls_wall_ms': plain_ms,
+        'speedup_vs_plain': plain_ms / wall_ms,
+        'achieved_GBps': total_bytes / (dev_ms * 1e-3) / 1e9,
+        'bit_exact_vs_plain': mismatches == 0,
+        'mismatched_jobs': mismatches,
+        'compile_and_build_s': compile_s,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=50)
+    ap.add_argument('--warmup', type=int, default=10)
+    ap.add_argument('--shots', type=int, default=1 << 20)
+    ap.add_argument('--workload', choices=['surface_d25', 'batch64', 'serve'], default='surface_d25')
+    ap.add_argument('--no-cpu', action='store_true')
+    ap.add_argument('--tile-words', type=int, default=0)
+    ap.add_argument('--calibrate-copy', action='store_true',
+                    help='run a 1 GiB device-to-device copy first (reference for FETCH_SIZE/WRITE_SIZE counters)')
+    ap.add_argument('--qps', type=float, default=1000.0)
+    ap.add_argument('--requests', type=int, default=3000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print('bench.py measures one GPU only: this build has no multi-GPU path (shots would shard across GPUs '
+              'with no exchange). Run with --gpus 1.', file=sys.stderr)
+        sys.exit(2)
+    if args.workload == 'surface_d25':
+        line = headline(args)
+    elif args.workload == 'batch64':
+        args.steps = min(args.steps, 5)
+        line = batch64(args)
+    else:
+        from stim_amd import serve
+        line = serve.bench(qps=args.qps, requests=args.requests)
     print(json.dumps(line), flush=True)
 
 

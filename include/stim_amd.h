@@ -94,10 +94,11 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
  * No reference counterpart (the reference has one engine). */
 int mfs_set_engine(mfs_sampler *s, int engine);
 
-/* out[0..5] = engine in use for sampling (1 frame, 2 symptom), site classes, symptom columns,
- * symptom entries, symptom-table build time in microseconds, noise sites. Builds the symptom
- * table if needed. */
-int mfs_engine_info(mfs_sampler *s, uint64_t out[6]);
+/* out[0..7] = engine in use for sampling (1 frame, 2 symptom), site classes, symptom columns,
+ * symptom entries, symptom-table build time in microseconds, noise sites, and for the symptom engine
+ * the shots per workgroup tile and the sampling kernel's LDS bytes per tile (bit-packed output with
+ * separate observables). Builds the symptom table if needed. */
+int mfs_engine_info(mfs_sampler *s, uint64_t out[8]);
 
 /* Symptom engine: target size (KB) of one workgroup's LDS image of output rows (sets the shots per
  * workgroup). Default 32. A negative value selects the per-lane-walk queue kernel instead of the
@@ -132,6 +133,40 @@ int mfs_batch_time(mfs_batch *b, int use_graph, int warmup, int iters, double ou
 int mfs_batch_autotune(mfs_batch *b, const int *candidates, int n, int iters, int *best_kb, double *best_ms);
 /* out[0..5] = jobs, persistent-kernel jobs, separately launched jobs, tiles, workgroups, LDS bytes. */
 int mfs_batch_info(mfs_batch *b, uint64_t out[6]);
+
+/* ---- Serving: resident circuits, memory-budget admission, continuous batching -------------------
+ * No reference counterpart. A server takes requests (sampler, shots, request seed) for circuits with
+ * deterministic detectors (symptom engine). A request's bit-packed rows are allocated in device memory
+ * when it is admitted, which happens in arrival order as soon as they fit budget_bytes; they stay
+ * until mfs_server_release. Each mfs_server_step admits what fits, takes a fair share of every running
+ * request's shot tiles (about step_cost_bytes of estimated work in total: output bytes plus 16 bytes
+ * per sampled fault), runs them in one persistent-kernel launch and waits. A request's result depends
+ * only on its sampler, seed and shot count, never on the other requests or on how its tiles were
+ * split across steps. Single-threaded; samplers must outlive their requests. */
+typedef struct mfs_server mfs_server;
+int mfs_server_create(uint64_t budget_bytes, double step_cost_bytes, mfs_server **out);
+void mfs_server_destroy(mfs_server *v);
+int mfs_server_submit(mfs_server *v, mfs_sampler *s, uint64_t shots, uint64_t seed, uint64_t *out_id);
+/* *did_work = 0 when nothing was queued or running. */
+int mfs_server_step(mfs_server *v, int *did_work);
+/* Moves up to cap finished request ids into ids; *n = how many. */
+int mfs_server_poll(mfs_server *v, uint64_t *ids, uint32_t cap, uint32_t *n);
+/* Device pointers and strides of a finished request's rows; times = submitted, admitted, finished
+ * (seconds, steady clock). */
+int mfs_server_result(mfs_server *v, uint64_t id, uint8_t **dets, size_t *dets_stride, uint8_t **obs,
+                      size_t *obs_stride, uint64_t *shots, double times[3]);
+/* Copies a finished request's rows to host buffers. */
+int mfs_server_copy_result(mfs_server *v, uint64_t id, uint8_t *dets, size_t dets_stride, uint8_t *obs,
+                           size_t obs_stride);
+/* Frees a finished (or still queued) request and its rows. */
+int mfs_server_release(mfs_server *v, uint64_t id);
+/* out[0..5] = steps, queued, running, bytes in use, peak bytes in use, budget. */
+int mfs_server_stats(mfs_server *v, uint64_t out[6]);
+/* Replays a request trace in real time: request i (samplers[i], shots[i], seeds[i]) arrives
+ * arrival_s[i] seconds after the call starts (sorted); each result is released as soon as it is
+ * complete. latency_ms[i] = finish - arrival; *wall_s = duration of the whole replay. */
+int mfs_server_replay(mfs_server *v, uint64_t n, mfs_sampler *const *samplers, const uint64_t *shots,
+                      const uint64_t *seeds, const double *arrival_s, double *latency_ms, double *wall_s);
 
 /* Configuration knobs: words per tile (1, 2 or 4; 0 = automatic). Returns the value in use. */
 int mfs_set_tile_words(mfs_sampler *s, int words);

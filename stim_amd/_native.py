@@ -27,6 +27,8 @@ EXPORTED = [
     'mfs_set_engine', 'mfs_engine_info', 'mfs_set_symptom_tile_kb',
     'mfs_batch_create', 'mfs_batch_destroy', 'mfs_batch_add', 'mfs_batch_set_tile_kb', 'mfs_batch_run',
     'mfs_batch_time', 'mfs_batch_autotune', 'mfs_batch_info',
+    'mfs_server_create', 'mfs_server_destroy', 'mfs_server_submit', 'mfs_server_step', 'mfs_server_poll',
+    'mfs_server_result', 'mfs_server_copy_result', 'mfs_server_release', 'mfs_server_stats', 'mfs_server_replay',
 ]
 
 _lib = None
@@ -73,10 +75,24 @@ def lib():
     L.mfs_batch_autotune.argtypes = [p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
     L.mfs_batch_info.argtypes = [p, ctypes.POINTER(u64)]
+    dbl = ctypes.c_double
+    L.mfs_server_create.argtypes = [u64, dbl, ctypes.POINTER(p)]
+    L.mfs_server_destroy.argtypes = [p]
+    L.mfs_server_destroy.restype = None
+    L.mfs_server_submit.argtypes = [p, p, u64, u64, ctypes.POINTER(u64)]
+    L.mfs_server_step.argtypes = [p, ctypes.POINTER(ctypes.c_int)]
+    L.mfs_server_poll.argtypes = [p, ctypes.POINTER(u64), u32, ctypes.POINTER(u32)]
+    L.mfs_server_result.argtypes = [p, u64, ctypes.POINTER(p), ctypes.POINTER(sz), ctypes.POINTER(p),
+                                    ctypes.POINTER(sz), ctypes.POINTER(u64), ctypes.POINTER(dbl)]
+    L.mfs_server_copy_result.argtypes = [p, u64, p, sz, p, sz]
+    L.mfs_server_release.argtypes = [p, u64]
+    L.mfs_server_stats.argtypes = [p, ctypes.POINTER(u64)]
+    L.mfs_server_replay.argtypes = [p, u64, ctypes.POINTER(p), ctypes.POINTER(u64), ctypes.POINTER(u64),
+                                    ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
     L.mfs_last_error.argtypes = []
     L.mfs_last_error.restype = ctypes.c_char_p
     for name in EXPORTED:
-        if name not in ('mfs_destroy', 'mfs_last_error', 'mfs_compile', 'mfs_batch_destroy'):
+        if name not in ('mfs_destroy', 'mfs_last_error', 'mfs_compile', 'mfs_batch_destroy', 'mfs_server_destroy'):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

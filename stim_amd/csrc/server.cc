@@ -1,0 +1,412 @@
+// Serving: a request loop over resident compiled circuits with memory-budget admission and continuous
+// batching.
+//
+// A request is (sampler, shots, request seed). Its output rows (bit-packed detection events and
+// observables) live in device memory from admission until the caller releases them; admission waits
+// until the request's rows fit the memory budget. Every step builds one work list from all admitted
+// requests -- a fair share of shot tiles from each, so a small request that arrives behind a large one
+// finishes within a step or two -- and runs it in one launch of the persistent symptom kernel. A
+// request's random stream is keyed by its sampler and its request seed, and a tile's randomness by the
+// tile's index inside the request, so a request's result does not depend on what it was batched with
+// or how its tiles were split across steps.
+#include <chrono>
+#include <deque>
+#include <map>
+#include <thread>
+#include <unordered_map>
+
+#include "sampler_impl.h"
+
+namespace {
+
+// Device memory in size classes (eighths of a power of two), kept for reuse after release.
+struct Pool {
+    std::map<size_t, std::vector<void *>> free_lists;
+    size_t cached = 0;
+
+    static size_t size_class(size_t n) {
+        n = std::max<size_t>(n, 1 << 16);
+        size_t p = 1;
+        while (p < n) p <<= 1;
+        const size_t step = std::max<size_t>(p >> 3, 1 << 16);
+        return (n + step - 1) / step * step;
+    }
+
+    void *get(size_t cls) {
+        auto it = free_lists.find(cls);
+        if (it != free_lists.end() && !it->second.empty()) {
+            void *p = it->second.back();
+            it->second.pop_back();
+            cached -= cls;
+            return p;
+        }
+        void *p = nullptr;
+        if (hipMalloc(&p, cls) != hipSuccess) {
+            (void)hipGetLastError();
+            trim(0);
+            check(hipMalloc(&p, cls), "hipMalloc (request output)");
+        }
+        return p;
+    }
+
+    void put(void *p, size_t cls) {
+        free_lists[cls].push_back(p);
+        cached += cls;
+    }
+
+    // Frees cached blocks until at most `keep` bytes stay cached.
+    void trim(size_t keep) {
+        for (auto &kv : free_lists) {
+            while (cached > keep && !kv.second.empty()) {
+                (void)hipFree(kv.second.back());
+                kv.second.pop_back();
+                cached -= kv.first;
+            }
+        }
+    }
+
+    ~Pool() { trim(0); }
+};
+
+double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct mfs_server {
+    enum State { QUEUED, ACTIVE, DONE };
+    struct Req {
+        mfs_sampler *s = nullptr;
+        uint64_t shots = 0;
+        State state = QUEUED;
+        SymDevParams P{};       // whole-request launch parameters (tile 0 at the start of the rows)
+        size_t lds = 0;
+        uint64_t tiles = 0, next_tile = 0;
+        double tile_cost = 0;   // estimated work per tile (bytes written + 16 bytes per hit)
+        uint8_t *dets = nullptr, *obs = nullptr;
+        size_t det_row = 0, obs_row = 0;
+        size_t bytes = 0;       // size class held against the budget
+        double t_submit = 0, t_admit = 0, t_done = 0;
+    };
+
+    uint64_t budget;
+    double step_cost;           // work per step (same unit as tile_cost)
+    uint64_t in_use = 0, peak_in_use = 0, steps = 0, next_id = 1;
+    std::unordered_map<uint64_t, Req> reqs;
+    std::deque<uint64_t> queue;
+    std::vector<uint64_t> active, finished;
+    Pool pool;
+    hipStream_t stream = nullptr;
+    DevBuf d_jobs, d_prefix, d_counter;
+    SymDevParams *h_jobs = nullptr;   // pinned staging for the work list
+    uint64_t *h_prefix = nullptr;
+    size_t h_cap = 0;
+    std::map<size_t, uint32_t> grid_for_lds;
+
+    mfs_server(uint64_t budget_bytes, double step_cost_bytes) : budget(budget_bytes), step_cost(step_cost_bytes) {
+        check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+        d_counter.ensure(4);
+    }
+
+    ~mfs_server() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto &kv : reqs) release_memory(kv.second);
+        if (h_jobs) (void)hipHostFree(h_jobs);
+        if (h_prefix) (void)hipHostFree(h_prefix);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    void release_memory(Req &r) {
+        if (r.dets) pool.put(r.dets, r.bytes);
+        r.dets = r.obs = nullptr;
+        if (r.state != QUEUED) in_use -= r.bytes;
+    }
+
+    uint64_t submit(mfs_sampler *s, uint64_t shots, uint64_t seed) {
+        if (!s) throw std::invalid_argument("sampler is null");
+        if (!shots) throw std::invalid_argument("shots must be positive");
+        if (!s->use_symptoms(false)) {
+            throw std::invalid_argument("the server runs symptom-engine circuits (deterministic detectors) only: " +
+                                        s->sym_why);
+        }
+        check(hipStreamSynchronize(s->stream), "symptom tables");
+        Req r;
+        r.s = s;
+        r.shots = shots;
+        r.det_row = (s->prog.num_detectors + 7) / 8;
+        r.obs_row = (s->prog.num_observables + 7) / 8;
+        const size_t det_bytes = ((size_t)shots * r.det_row + 255) / 256 * 256;
+        const size_t obs_bytes = (size_t)shots * r.obs_row;
+        r.bytes = Pool::size_class(det_bytes + obs_bytes);
+        if (r.bytes > budget) throw std::invalid_argument("request output exceeds the server's memory budget");
+        static uint8_t probe;
+        const auto L = s->layout(MFS_BIT_PACKED | MFS_DEVICE_OUTPUT | (r.obs_row ? MFS_SEPARATE_OBSERVABLES : 0));
+        auto sl = s->sym_prepare(L, shots, &probe, r.det_row, r.obs_row ? &probe : nullptr, r.obs_row, 0);
+        if (!sl.scan_ok) throw std::invalid_argument("circuit too large for the serving kernel's tile positions");
+        // The request's own stream: the sampler's key mixed with the request seed.
+        sl.P.seed = splitmix64(sl.P.seed ^ splitmix64(seed ^ 0x7E9A3C15D2B84F61ull));
+        r.P = sl.P;
+        r.lds = sl.lds_scan;
+        r.tiles = sl.tiles;
+        double hits = 0;
+        for (const SymClass &cl : s->plan.classes) hits += (double)cl.nsites * cl.p;
+        r.tile_cost = (double)r.P.shots_per_tile * ((double)(r.det_row + r.obs_row) + 16.0 * hits) + 4096.0;
+        r.t_submit = now_s();
+        const uint64_t id = next_id++;
+        reqs.emplace(id, r);
+        queue.push_back(id);
+        return id;
+    }
+
+    // Admits queued requests, in arrival order, while their rows fit the budget.
+    void admit() {
+        while (!queue.empty()) {
+            Req &r = reqs.at(queue.front());
+            if (in_use + r.bytes > budget) break;
+            if (in_use + r.bytes + pool.cached > budget) pool.trim(budget - in_use - r.bytes);
+            uint8_t *base = static_cast<uint8_t *>(pool.get(r.bytes));
+            r.dets = base;
+            r.obs = base + ((size_t)r.shots * r.det_row + 255) / 256 * 256;
+            r.P.dets.ptr = r.dets;
+            r.P.dets.stride = r.det_row;
+            r.P.obs.ptr = r.obs;
+            r.P.obs.stride = r.obs_row;
+            in_use += r.bytes;
+            peak_in_use = std::max(peak_in_use, in_use);
+            r.state = ACTIVE;
+            r.t_admit = now_s();
+            active.push_back(queue.front());
+            queue.pop_front();
+        }
+    }
+
+    uint32_t grid(size_t lds, uint64_t tiles) {
+        auto it = grid_for_lds.find(lds);
+        if (it == grid_for_lds.end()) {
+            int dev = 0, cus = 0;
+            check(hipGetDevice(&dev), "device");
+            check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device");
+            it = grid_for_lds.emplace(lds, (uint32_t)std::max(cus, 1) * (uint32_t)sym_batch_blocks_per_cu(lds)).first;
+        }
+        return (uint32_t)std::min<uint64_t>(it->second, tiles);
+    }
+
+    // One step: admit, take a fair share of tiles from every active request, run them in one launch,
+    // wait, and retire the requests that are complete. Returns false when there was nothing to do.
+    bool step() {
+        admit();
+        if (active.empty()) return false;
+        // Shares: each active request gets up to step_cost / n of work; what is left goes in order.
+        std::vector<std::pair<uint64_t, uint64_t>> take;   // (request id, tiles this step)
+        double left = step_cost;
+        const double fair = step_cost / (double)active.size();
+        for (uint64_t id : active) {
+            Req &r = reqs.at(id);
+            const uint64_t want = r.tiles - r.next_tile;
+            const uint64_t share = std::max<uint64_t>(1, (uint64_t)(fair / r.tile_cost));
+            const uint64_t n = std::min(want, share);
+            take.push_back({id, n});
+            left -= (double)n * r.tile_cost;
+        }
+        for (auto &t : take) {
+            if (left <= 0) break;
+            Req &r = reqs.at(t.first);
+            const uint64_t more = std::min<uint64_t>(r.tiles - r.next_tile - t.second, (uint64_t)(left / r.tile_cost));
+            t.second += more;
+            left -= (double)more * r.tile_cost;
+        }
+        // Work list: one job per request chunk.
+        if (take.size() > h_cap) {
+            if (h_jobs) (void)hipHostFree(h_jobs);
+            if (h_prefix) (void)hipHostFree(h_prefix);
+            h_cap = std::max<size_t>(take.size(), 2 * h_cap);
+            check(hipHostMalloc((void **)&h_jobs, h_cap * sizeof(SymDevParams), hipHostMallocDefault), "pinned");
+            check(hipHostMalloc((void **)&h_prefix, (h_cap + 1) * sizeof(uint64_t), hipHostMallocDefault), "pinned");
+            d_jobs.ensure(h_cap * sizeof(SymDevParams));
+            d_prefix.ensure((h_cap + 1) * sizeof(uint64_t));
+        }
+        size_t lds = 64;
+        h_prefix[0] = 0;
+        for (size_t k = 0; k < take.size(); k++) {
+            Req &r = reqs.at(take[k].first);
+            SymDevParams P = r.P;
+            const uint64_t a = r.next_tile;
+            const uint64_t s0 = a * P.shots_per_tile;
+            P.tile0 = a;
+            P.shots = r.shots - s0;
+            P.dets.ptr = r.dets + s0 * r.det_row;
+            P.obs.ptr = r.obs + s0 * r.obs_row;
+            h_jobs[k] = P;
+            h_prefix[k + 1] = h_prefix[k] + take[k].second;
+            lds = std::max(lds, r.lds);
+            r.next_tile += take[k].second;
+        }
+        const uint64_t tiles = h_prefix[take.size()];
+        check(hipMemcpyAsync(d_jobs.p, h_jobs, take.size() * sizeof(SymDevParams), hipMemcpyHostToDevice, stream),
+              "upload work list");
+        check(hipMemcpyAsync(d_prefix.p, h_prefix, (take.size() + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream),
+              "upload work list");
+        const uint32_t g = grid(lds, tiles);
+        check(launch_sym_batch(d_jobs.as<SymDevParams>(), d_prefix.as<uint64_t>(), (uint32_t)take.size(),
+                               d_counter.as<uint32_t>(), g, lds, 0, stream), "serving step");
+        check(hipStreamSynchronize(stream), "serving step");
+        steps++;
+        const double t = now_s();
+        std::vector<uint64_t> still;
+        for (uint64_t id : active) {
+            Req &r = reqs.at(id);
+            if (r.next_tile == r.tiles) {
+                r.state = DONE;
+                r.t_done = t;
+                finished.push_back(id);
+            } else {
+                still.push_back(id);
+            }
+        }
+        active.swap(still);
+        return true;
+    }
+
+    Req &get(uint64_t id) {
+        auto it = reqs.find(id);
+        if (it == reqs.end()) throw std::out_of_range("unknown request id");
+        return it->second;
+    }
+
+    void release(uint64_t id) {
+        Req &r = get(id);
+        if (r.state == ACTIVE) throw std::invalid_argument("request is still running");
+        release_memory(r);
+        if (r.state == QUEUED) {
+            for (auto it = queue.begin(); it != queue.end(); ++it) {
+                if (*it == id) {
+                    queue.erase(it);
+                    break;
+                }
+            }
+        }
+        reqs.erase(id);
+    }
+};
+
+extern "C" {
+
+int mfs_server_create(uint64_t budget_bytes, double step_cost_bytes, mfs_server **out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out is null");
+        if (!budget_bytes) throw std::invalid_argument("budget must be positive");
+        *out = new mfs_server(budget_bytes, step_cost_bytes > 0 ? step_cost_bytes : 2e9);
+    });
+}
+
+void mfs_server_destroy(mfs_server *v) { delete v; }
+
+int mfs_server_submit(mfs_server *v, mfs_sampler *s, uint64_t shots, uint64_t seed, uint64_t *out_id) {
+    return guarded([&] {
+        const uint64_t id = v->submit(s, shots, seed);
+        if (out_id) *out_id = id;
+    });
+}
+
+int mfs_server_step(mfs_server *v, int *did_work) {
+    return guarded([&] {
+        const bool w = v->step();
+        if (did_work) *did_work = w ? 1 : 0;
+    });
+}
+
+int mfs_server_poll(mfs_server *v, uint64_t *ids, uint32_t cap, uint32_t *n) {
+    return guarded([&] {
+        const uint32_t k = (uint32_t)std::min<size_t>(cap, v->finished.size());
+        for (uint32_t i = 0; i < k; i++) ids[i] = v->finished[i];
+        v->finished.erase(v->finished.begin(), v->finished.begin() + k);
+        if (n) *n = k;
+    });
+}
+
+int mfs_server_result(mfs_server *v, uint64_t id, uint8_t **dets, size_t *dets_stride, uint8_t **obs,
+                      size_t *obs_stride, uint64_t *shots, double times[3]) {
+    return guarded([&] {
+        auto &r = v->get(id);
+        if (r.state != mfs_server::DONE) throw std::invalid_argument("request has not finished");
+        if (dets) *dets = r.dets;
+        if (dets_stride) *dets_stride = r.det_row;
+        if (obs) *obs = r.obs_row ? r.obs : nullptr;
+        if (obs_stride) *obs_stride = r.obs_row;
+        if (shots) *shots = r.shots;
+        if (times) {
+            times[0] = r.t_submit;
+            times[1] = r.t_admit;
+            times[2] = r.t_done;
+        }
+    });
+}
+
+int mfs_server_copy_result(mfs_server *v, uint64_t id, uint8_t *dets, size_t dets_stride, uint8_t *obs,
+                           size_t obs_stride) {
+    return guarded([&] {
+        auto &r = v->get(id);
+        if (r.state != mfs_server::DONE) throw std::invalid_argument("request has not finished");
+        if (dets && r.det_row)
+            check(hipMemcpy2DAsync(dets, dets_stride, r.dets, r.det_row, r.det_row, r.shots, hipMemcpyDeviceToHost,
+                                   v->stream), "copy result");
+        if (obs && r.obs_row)
+            check(hipMemcpy2DAsync(obs, obs_stride, r.obs, r.obs_row, r.obs_row, r.shots, hipMemcpyDeviceToHost,
+                                   v->stream), "copy result");
+        check(hipStreamSynchronize(v->stream), "copy result");
+    });
+}
+
+int mfs_server_release(mfs_server *v, uint64_t id) {
+    return guarded([&] { v->release(id); });
+}
+
+int mfs_server_stats(mfs_server *v, uint64_t out[6]) {
+    return guarded([&] {
+        out[0] = v->steps;
+        out[1] = v->queue.size();
+        out[2] = v->active.size();
+        out[3] = v->in_use;
+        out[4] = v->peak_in_use;
+        out[5] = v->budget;
+    });
+}
+
+int mfs_server_replay(mfs_server *v, uint64_t n, mfs_sampler *const *samplers, const uint64_t *shots,
+                      const uint64_t *seeds, const double *arrival_s, double *latency_ms, double *wall_s) {
+    return guarded([&] {
+        for (uint64_t i = 1; i < n; i++) {
+            if (arrival_s[i] < arrival_s[i - 1]) throw std::invalid_argument("arrival times must be sorted");
+        }
+        std::unordered_map<uint64_t, uint64_t> index;   // request id -> trace position
+        std::vector<double> arrive_at(n);
+        const double t0 = now_s();
+        uint64_t next = 0, done = 0;
+        std::vector<uint64_t> ids(256);
+        while (done < n) {
+            const double t = now_s() - t0;
+            while (next < n && arrival_s[next] <= t) {
+                const uint64_t id = v->submit(samplers[next], shots[next], seeds[next]);
+                index[id] = next;
+                arrive_at[next] = t0 + arrival_s[next];
+                next++;
+            }
+            const bool worked = v->step();
+            for (uint64_t id : v->finished) {
+                const uint64_t k = index.at(id);
+                latency_ms[k] = (v->reqs.at(id).t_done - arrive_at[k]) * 1e3;
+                v->release(id);
+                done++;
+            }
+            v->finished.clear();
+            if (!worked && next < n) {
+                const double wait = arrival_s[next] - (now_s() - t0);
+                if (wait > 2e-4) std::this_thread::sleep_for(std::chrono::duration<double>(wait - 1e-4));
+            }
+        }
+        if (wall_s) *wall_s = now_s() - t0;
+    });
+}
+
+}  // extern "C"

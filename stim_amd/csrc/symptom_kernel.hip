@@ -483,29 +483,41 @@ __global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
 }
 
 // Batched form: many launches (jobs, possibly of different circuits) in one persistent kernel.
-// Workgroups take global tile numbers from a counter in order; job j owns tiles
-// [prefix[j], prefix[j+1]). Every workgroup leaves once the counter passes the last tile.
+// Workgroups claim chunks of BATCH_CHUNK consecutive global tile numbers from a counter; job j owns
+// tiles [prefix[j], prefix[j+1]). The job of the current tile is cached, so the search only runs when
+// a chunk crosses into another job. Every workgroup leaves once the counter passes the last tile.
+constexpr uint32_t BATCH_CHUNK = 8;
+
 __global__ __launch_bounds__(NT) void sym_batch_kernel(const SymDevParams *__restrict__ jobs,
                                                        const uint64_t *__restrict__ prefix, uint32_t njobs,
                                                        uint32_t *counter) {
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ uint32_t next;
     const uint64_t total = prefix[njobs];
+    uint32_t j = 0;
+    uint64_t lo = 0, hi = 0;   // tile range of job j (empty until the first search)
     for (;;) {
         __syncthreads();
-        if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+        if (threadIdx.x == 0) next = atomicAdd(counter, BATCH_CHUNK);
         __syncthreads();
-        const uint32_t t = next;
-        if (t >= total) break;
-        uint32_t lo = 0, hi = njobs;
-        while (hi - lo > 1) {
-            const uint32_t mid = (lo + hi) / 2;
-            if (prefix[mid] <= t) lo = mid;
-            else hi = mid;
+        const uint64_t t0 = next;
+        if (t0 >= total) break;
+        const uint64_t t1 = t0 + BATCH_CHUNK < total ? t0 + BATCH_CHUNK : total;
+        for (uint64_t t = t0; t < t1; t++) {
+            if (t < lo || t >= hi) {
+                uint32_t a = 0, b = njobs;
+                while (b - a > 1) {
+                    const uint32_t mid = (a + b) / 2;
+                    if (prefix[mid] <= t) a = mid;
+                    else b = mid;
+                }
+                j = __builtin_amdgcn_readfirstlane(a);
+                lo = prefix[j];
+                hi = prefix[j + 1];
+            }
+            if (t != t0) __syncthreads();   // the previous tile's rows are written before the image is reused
+            scan_tile(jobs[j], (uint32_t)(t - lo), lds);
         }
-        const uint32_t j = __builtin_amdgcn_readfirstlane(lo);
-        const SymDevParams P = jobs[j];
-        scan_tile(P, (uint32_t)(t - prefix[j]), lds);
     }
 }
 
@@ -613,12 +625,14 @@ hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hi
     return hipGetLastError();
 }
 
+// The dynamic LDS attribute is set by sym_batch_blocks_per_cu (before any graph capture).
 hipError_t launch_sym_batch(SymDevParams *jobs, const uint64_t *prefix, uint32_t njobs, uint32_t *counter,
-                            uint32_t grid, size_t lds, hipStream_t stream) {
-    (void)hipFuncSetAttribute((const void *)sym_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                            uint32_t grid, size_t lds, int advance, hipStream_t stream) {
     hipLaunchKernelGGL(sym_batch_reset_kernel, dim3(1), dim3(64), 0, stream, counter);
     hipLaunchKernelGGL(sym_batch_kernel, dim3(grid), dim3(NT), lds, stream, jobs, prefix, njobs, counter);
-    hipLaunchKernelGGL(sym_batch_advance_kernel, dim3((njobs + 255) / 256), dim3(256), 0, stream, jobs, prefix, njobs);
+    if (advance)
+        hipLaunchKernelGGL(sym_batch_advance_kernel, dim3((njobs + 255) / 256), dim3(256), 0, stream, jobs, prefix,
+                           njobs);
     return hipGetLastError();
 }
 

@@ -264,11 +264,11 @@ class CompiledDetectorSampler:
         N.check(N.lib().mfs_set_symptom_tile_kb(self._h, int(kb)))
 
     def engine_info(self):
-        out = (ctypes.c_uint64 * 6)()
+        out = (ctypes.c_uint64 * 8)()
         N.check(N.lib().mfs_engine_info(self._h, out))
         v = list(out)
         return {'engine': {1: 'frame', 2: 'symptom'}[v[0]], 'classes': v[1], 'columns': v[2], 'entries': v[3],
-                'build_ms': v[4] / 1000.0, 'sites': v[5]}
+                'build_ms': v[4] / 1000.0, 'sites': v[5], 'tile_shots': v[6], 'lds_bytes_per_tile': v[7]}
 
     def bench_device(self, shots: int, warmup: int = 2, iters: int = 5, split: bool = False):
         """Average device time (ms) of one sample of `shots` into HBM-resident packed outputs.
