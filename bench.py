@@ -201,7 +201,7 @@ def batch64(args):
 
     # Autotuner on its own samplers (so the compared samplers start at the same stream position).
     tuner = BatchSampler(list(zip(tune_s, shots)))
-    kb, tune_ms = tuner.autotune(candidates=(8, 16, 32, 48), iters=1)
+    kb, tune_ms = tuner.autotune(candidates=(12, 16, 20, 24, 32), iters=1)
     del tuner
     for s in batch_s + plain_s:
         s.set_symptom_tile_kb(kb)

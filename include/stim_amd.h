@@ -128,11 +128,15 @@ int mfs_batch_run(mfs_batch *b, int use_graph);
 /* Runs warmup + iters times; out_ms[0] = average device time per run (HIP events on the batch's
  * stream), out_ms[1] = average wall time per run. */
 int mfs_batch_time(mfs_batch *b, int use_graph, int warmup, int iters, double out_ms[2]);
-/* Autotuner: times iters runs for each candidate tile size (KB) and keeps the fastest, also setting it
- * on every job's sampler so plain calls stay comparable. */
+/* Schedule of the batch kernel: 1 (default) = persistent workgroups (one per resident slot) claiming
+ * chunks of tiles from a work counter; 0 = one workgroup per tile. Results do not depend on it. */
+int mfs_batch_set_schedule(mfs_batch *b, int persistent);
+/* Autotuner: times iters runs for each candidate tile size (KB) under both schedules and keeps the
+ * fastest pair, also setting the tile size on every job's sampler so plain calls stay comparable. */
 int mfs_batch_autotune(mfs_batch *b, const int *candidates, int n, int iters, int *best_kb, double *best_ms);
-/* out[0..5] = jobs, persistent-kernel jobs, separately launched jobs, tiles, workgroups, LDS bytes. */
-int mfs_batch_info(mfs_batch *b, uint64_t out[6]);
+/* out[0..7] = jobs, kernel jobs, separately launched jobs, tiles, workgroups, LDS bytes, persistent
+ * schedule (1/0), tile size in KB (0 = each sampler's own). */
+int mfs_batch_info(mfs_batch *b, uint64_t out[8]);
 
 /* ---- Serving: resident circuits, memory-budget admission, continuous batching -------------------
  * No reference counterpart. A server takes requests (sampler, shots, request seed) for circuits with

@@ -26,7 +26,7 @@ EXPORTED = [
     'mfs_sample_with_faults', 'mfs_bench_device', 'mfs_set_tile_words', 'mfs_get_info', 'mfs_last_error',
     'mfs_set_engine', 'mfs_engine_info', 'mfs_set_symptom_tile_kb',
     'mfs_batch_create', 'mfs_batch_destroy', 'mfs_batch_add', 'mfs_batch_set_tile_kb', 'mfs_batch_run',
-    'mfs_batch_time', 'mfs_batch_autotune', 'mfs_batch_info',
+    'mfs_batch_time', 'mfs_batch_autotune', 'mfs_batch_info', 'mfs_batch_set_schedule',
     'mfs_server_create', 'mfs_server_destroy', 'mfs_server_submit', 'mfs_server_step', 'mfs_server_poll',
     'mfs_server_result', 'mfs_server_copy_result', 'mfs_server_release', 'mfs_server_stats', 'mfs_server_replay',
 ]
@@ -75,6 +75,7 @@ def lib():
     L.mfs_batch_autotune.argtypes = [p, ctypes.POINTER(ctypes.c_int), ctypes.c_int, ctypes.c_int,
                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]
     L.mfs_batch_info.argtypes = [p, ctypes.POINTER(u64)]
+    L.mfs_batch_set_schedule.argtypes = [p, ctypes.c_int]
     dbl = ctypes.c_double
     L.mfs_server_create.argtypes = [u64, dbl, ctypes.POINTER(p)]
     L.mfs_server_destroy.argtypes = [p]

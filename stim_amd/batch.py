@@ -58,16 +58,21 @@ class BatchSampler:
         N.check(N.lib().mfs_batch_time(self._h, 1 if use_graph else 0, int(warmup), int(iters), out))
         return out[0], out[1]
 
-    def autotune(self, candidates=(8, 16, 24, 32, 48, 64), iters: int = 2):
-        """Picks the fastest LDS output-image size per workgroup (KB); returns (kb, ms per run)."""
+    def autotune(self, candidates=(12, 16, 20, 24, 32, 48), iters: int = 2):
+        """Picks the fastest LDS output-image size per workgroup (KB) and schedule (persistent or one
+        workgroup per tile); returns (kb, ms per run)."""
         arr = (ctypes.c_int * len(candidates))(*[int(c) for c in candidates])
         kb, ms = ctypes.c_int(), ctypes.c_double()
         N.check(N.lib().mfs_batch_autotune(self._h, arr, len(candidates), int(iters), ctypes.byref(kb),
                                            ctypes.byref(ms)))
         return kb.value, ms.value
 
+    def set_schedule(self, persistent: bool):
+        """Persistent workgroups claiming tiles (True) or one workgroup per tile (False)."""
+        N.check(N.lib().mfs_batch_set_schedule(self._h, 1 if persistent else 0))
+
     def info(self):
-        out = (ctypes.c_uint64 * 6)()
+        out = (ctypes.c_uint64 * 8)()
         N.check(N.lib().mfs_batch_info(self._h, out))
-        keys = ['jobs', 'kernel_jobs', 'separate_jobs', 'tiles', 'workgroups', 'lds_bytes']
+        keys = ['jobs', 'kernel_jobs', 'separate_jobs', 'tiles', 'workgroups', 'lds_bytes', 'persistent', 'tile_kb']
         return dict(zip(keys, list(out)))

@@ -15,6 +15,7 @@ struct mfs_batch {
     };
     std::vector<Job> jobs;
     int tile_kb = 0;                 // LDS output image per workgroup (0: each sampler's own setting)
+    int persistent = 1;              // 1: persistent workgroups claiming tiles; 0: one workgroup per tile
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     DevBuf d_jobs, d_prefix, d_counter;
@@ -136,7 +137,7 @@ struct mfs_batch {
             check(hipGetDevice(&dev), "device");
             check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device");
             const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)sym_batch_blocks_per_cu(lds);
-            grid = (uint32_t)std::min<uint64_t>(resident, total_tiles);
+            grid = persistent ? (uint32_t)std::min<uint64_t>(resident, total_tiles) : (uint32_t)total_tiles;
         } else {
             grid = 0;
         }
@@ -257,24 +258,29 @@ int mfs_batch_autotune(mfs_batch *b, const int *candidates, int n, int iters, in
     return guarded([&] {
         if (!candidates || n < 1) throw std::invalid_argument("no candidate tile sizes");
         double best = 1e300;
-        int pick = candidates[0];
+        int pick = candidates[0], pick_sched = 1;
         for (int i = 0; i < n; i++) {
             if (candidates[i] < 1 || candidates[i] > 128) throw std::invalid_argument("candidate tile sizes must be 1..128 KB");
-            b->tile_kb = candidates[i];
-            b->invalidate();
-            b->run_once(false);
-            check(hipStreamSynchronize(b->stream), "autotune");
-            const auto t0 = std::chrono::steady_clock::now();
-            for (int k = 0; k < std::max(iters, 1); k++) b->run_once(false);
-            check(hipStreamSynchronize(b->stream), "autotune");
-            const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
-                              std::max(iters, 1);
-            if (ms < best) {
-                best = ms;
-                pick = candidates[i];
+            for (int sched = 1; sched >= 0; sched--) {
+                b->tile_kb = candidates[i];
+                b->persistent = sched;
+                b->invalidate();
+                b->run_once(false);
+                check(hipStreamSynchronize(b->stream), "autotune");
+                const auto t0 = std::chrono::steady_clock::now();
+                for (int k = 0; k < std::max(iters, 1); k++) b->run_once(false);
+                check(hipStreamSynchronize(b->stream), "autotune");
+                const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() /
+                                  std::max(iters, 1);
+                if (ms < best) {
+                    best = ms;
+                    pick = candidates[i];
+                    pick_sched = sched;
+                }
             }
         }
         b->tile_kb = pick;
+        b->persistent = pick_sched;
         b->invalidate();
         // Plain calls on these samplers use the chosen tiles too, so they stay comparable to the batch.
         for (auto &jb : b->jobs) jb.s->sym_tile_kb = pick;
@@ -283,7 +289,14 @@ int mfs_batch_autotune(mfs_batch *b, const int *candidates, int n, int iters, in
     });
 }
 
-int mfs_batch_info(mfs_batch *b, uint64_t out[6]) {
+int mfs_batch_set_schedule(mfs_batch *b, int persistent) {
+    return guarded([&] {
+        b->persistent = persistent ? 1 : 0;
+        b->invalidate();
+    });
+}
+
+int mfs_batch_info(mfs_batch *b, uint64_t out[8]) {
     return guarded([&] {
         b->sync_counters();
         out[0] = b->jobs.size();
@@ -292,6 +305,8 @@ int mfs_batch_info(mfs_batch *b, uint64_t out[6]) {
         out[3] = b->total_tiles;
         out[4] = b->grid;
         out[5] = b->lds;
+        out[6] = (uint64_t)b->persistent;
+        out[7] = (uint64_t)b->tile_kb;
     });
 }
 

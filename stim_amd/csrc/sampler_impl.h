@@ -131,7 +131,8 @@ struct mfs_sampler {
     SymPlan plan;
     uint64_t sym_entries = 0;
     double sym_build_ms = 0;
-    DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols;
+    DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols, d_recs32, d_ovf32;
+    bool sym_compact = false;
     int sym_tile_kb = 32;        // target LDS image size per symptom tile
     int sym_scan = 1;            // scan kernel (0: per-lane-walk queue kernel)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -464,6 +465,33 @@ struct mfs_sampler {
         }
         upload(d_recs, recs, "symptom records");
         upload(d_ovf, ovf, "symptom overflow entries");
+        // Compact records by slot, when every entry index fits 15 bits.
+        sym_compact = prog.num_detectors + prog.num_observables <= SYM_COMPACT_LIMIT;
+        if (sym_compact) {
+            std::vector<uint32_t> r32(std::max<uint64_t>(plan.total_slots, 1), 0);
+            std::vector<uint32_t> o32;
+            for (const SymClass &cl : plan.classes) {
+                for (uint64_t st = 0; st < cl.nsites; st++) {
+                    for (uint32_t j = 0; j < cl.basis; j++) {
+                        const uint64_t c = cl.col0 + st * cl.basis + j;
+                        const uint32_t n = offs[c + 1] - offs[c];
+                        const uint32_t *e = ents.data() + offs[c];
+                        uint32_t rec;
+                        if (n <= 2) {
+                            rec = (n << 30) | (n >= 1 ? e[0] : 0u) | (n == 2 ? e[1] << 15 : 0u);
+                        } else {
+                            if (o32.size() >= (1u << 30)) throw std::invalid_argument("symptom table too large");
+                            rec = (3u << 30) | (uint32_t)o32.size();
+                            o32.push_back(n);
+                            o32.insert(o32.end(), e, e + n);
+                        }
+                        r32[cl.rec0 + st * cl.rstride + j] = rec;
+                    }
+                }
+            }
+            upload(d_recs32, r32, "compact symptom records");
+            upload(d_ovf32, o32, "compact symptom overflow entries");
+        }
         sym_state = 1;
         return true;
     }
@@ -500,6 +528,8 @@ struct mfs_sampler {
         P.otab = d_sotab.as<uint32_t>();
         P.recs = d_recs.as<uint2>();
         P.ovf = d_ovf.as<uint32_t>();
+        P.recs32 = sym_compact ? d_recs32.as<uint32_t>() : nullptr;
+        P.ovf32 = sym_compact ? d_ovf32.as<uint32_t>() : nullptr;
         P.nclasses = (uint32_t)plan.classes.size();
         P.otab_words = (uint32_t)plan.otab.size();
         P.D = prog.num_detectors;

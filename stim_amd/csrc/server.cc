@@ -2,10 +2,12 @@
 // batching.
 //
 // A request is (sampler, shots, request seed). Its output rows (bit-packed detection events and
-// observables) live in device memory from admission until the caller releases them; admission waits
-// until the request's rows fit the memory budget. Every step builds one work list from all admitted
-// requests -- a fair share of shot tiles from each, so a small request that arrives behind a large one
-// finishes within a step or two -- and runs it in one launch of the persistent symptom kernel. A
+// observables) live in device memory from admission until the caller releases them. Admission takes
+// queued requests first-fit in arrival order (within a window of the oldest ones) while their rows fit
+// the memory budget and fewer than max_active requests run. Every step builds one work list from the
+// running requests and runs it in one launch of the symptom batch kernel: three quarters of the step's
+// work go to the requests with the least work left (so a small request that arrives behind large ones
+// finishes within a step or two), the rest to the oldest request (so large ones keep moving). A
 // request's random stream is keyed by its sampler and its request seed, and a tile's randomness by the
 // tile's index inside the request, so a request's result does not depend on what it was batched with
 // or how its tiles were split across steps.
@@ -19,53 +21,68 @@
 
 namespace {
 
-// Device memory in size classes (eighths of a power of two), kept for reuse after release.
-struct Pool {
-    std::map<size_t, std::vector<void *>> free_lists;
-    size_t cached = 0;
+// Request rows are carved out of device segments that are allocated on demand (8 GB, or the request's
+// size if larger) up to the budget and kept until the server is destroyed: first-fit over each
+// segment's free list of (offset, size) extents, merging neighbours on release, so steady-state
+// serving calls neither hipMalloc nor hipFree (hipFree synchronizes the device).
+struct Arena {
+    struct Segment {
+        uint8_t *base;
+        size_t cap;
+        std::map<size_t, size_t> free_ext;   // offset -> size
+    };
+    std::vector<Segment> segs;
+    size_t total = 0;
+    static constexpr size_t SEGMENT = (size_t)8 << 30;
 
-    static size_t size_class(size_t n) {
-        n = std::max<size_t>(n, 1 << 16);
-        size_t p = 1;
-        while (p < n) p <<= 1;
-        const size_t step = std::max<size_t>(p >> 3, 1 << 16);
-        return (n + step - 1) / step * step;
-    }
+    static size_t round(size_t n) { return (std::max<size_t>(n, 1) + 255) / 256 * 256; }
 
-    void *get(size_t cls) {
-        auto it = free_lists.find(cls);
-        if (it != free_lists.end() && !it->second.empty()) {
-            void *p = it->second.back();
-            it->second.pop_back();
-            cached -= cls;
-            return p;
-        }
-        void *p = nullptr;
-        if (hipMalloc(&p, cls) != hipSuccess) {
-            (void)hipGetLastError();
-            trim(0);
-            check(hipMalloc(&p, cls), "hipMalloc (request output)");
-        }
-        return p;
-    }
-
-    void put(void *p, size_t cls) {
-        free_lists[cls].push_back(p);
-        cached += cls;
-    }
-
-    // Frees cached blocks until at most `keep` bytes stay cached.
-    void trim(size_t keep) {
-        for (auto &kv : free_lists) {
-            while (cached > keep && !kv.second.empty()) {
-                (void)hipFree(kv.second.back());
-                kv.second.pop_back();
-                cached -= kv.first;
+    // Returns null when nothing fits and no segment can be added within `limit` bytes in total.
+    uint8_t *get(size_t n, size_t limit) {
+        for (Segment &g : segs) {
+            for (auto it = g.free_ext.begin(); it != g.free_ext.end(); ++it) {
+                if (it->second >= n) {
+                    const size_t off = it->first, sz = it->second;
+                    g.free_ext.erase(it);
+                    if (sz > n) g.free_ext[off + n] = sz - n;
+                    return g.base + off;
+                }
             }
         }
+        const size_t want = std::max(n, std::min(SEGMENT, limit > total ? limit - total : 0));
+        if (total + want > limit || want < n) return nullptr;
+        Segment g{nullptr, want, {}};
+        check(hipMalloc(&g.base, want), "hipMalloc (serving arena)");
+        total += want;
+        if (want > n) g.free_ext[n] = want - n;
+        segs.push_back(std::move(g));
+        return segs.back().base;
     }
 
-    ~Pool() { trim(0); }
+    void put(uint8_t *p, size_t n) {
+        for (Segment &g : segs) {
+            if (p < g.base || p >= g.base + g.cap) continue;
+            size_t off = (size_t)(p - g.base), sz = n;
+            auto next = g.free_ext.lower_bound(off);
+            if (next != g.free_ext.end() && off + sz == next->first) {
+                sz += next->second;
+                next = g.free_ext.erase(next);
+            }
+            if (next != g.free_ext.begin()) {
+                auto prev = std::prev(next);
+                if (prev->first + prev->second == off) {
+                    prev->second += sz;
+                    return;
+                }
+            }
+            g.free_ext[off] = sz;
+            return;
+        }
+    }
+
+    ~Arena() {
+        for (Segment &g : segs) (void)hipFree(g.base);
+    }
 };
 
 double now_s() {
@@ -86,17 +103,19 @@ struct mfs_server {
         double tile_cost = 0;   // estimated work per tile (bytes written + 16 bytes per hit)
         uint8_t *dets = nullptr, *obs = nullptr;
         size_t det_row = 0, obs_row = 0;
-        size_t bytes = 0;       // size class held against the budget
+        size_t bytes = 0;       // arena bytes held against the budget
         double t_submit = 0, t_admit = 0, t_done = 0;
     };
 
     uint64_t budget;
     double step_cost;           // work per step (same unit as tile_cost)
+    size_t max_active = 64;     // requests running at once
+    size_t admit_window = 64;   // queued requests considered for first-fit admission
     uint64_t in_use = 0, peak_in_use = 0, steps = 0, next_id = 1;
     std::unordered_map<uint64_t, Req> reqs;
     std::deque<uint64_t> queue;
     std::vector<uint64_t> active, finished;
-    Pool pool;
+    Arena arena;
     hipStream_t stream = nullptr;
     DevBuf d_jobs, d_prefix, d_counter;
     SymDevParams *h_jobs = nullptr;   // pinned staging for the work list
@@ -118,7 +137,7 @@ struct mfs_server {
     }
 
     void release_memory(Req &r) {
-        if (r.dets) pool.put(r.dets, r.bytes);
+        if (r.dets) arena.put(r.dets, r.bytes);
         r.dets = r.obs = nullptr;
         if (r.state != QUEUED) in_use -= r.bytes;
     }
@@ -138,7 +157,7 @@ struct mfs_server {
         r.obs_row = (s->prog.num_observables + 7) / 8;
         const size_t det_bytes = ((size_t)shots * r.det_row + 255) / 256 * 256;
         const size_t obs_bytes = (size_t)shots * r.obs_row;
-        r.bytes = Pool::size_class(det_bytes + obs_bytes);
+        r.bytes = Arena::round(det_bytes + obs_bytes);
         if (r.bytes > budget) throw std::invalid_argument("request output exceeds the server's memory budget");
         static uint8_t probe;
         const auto L = s->layout(MFS_BIT_PACKED | MFS_DEVICE_OUTPUT | (r.obs_row ? MFS_SEPARATE_OBSERVABLES : 0));
@@ -159,13 +178,21 @@ struct mfs_server {
         return id;
     }
 
-    // Admits queued requests, in arrival order, while their rows fit the budget.
+    // Admits queued requests first-fit in arrival order while their rows fit the budget.
     void admit() {
-        while (!queue.empty()) {
-            Req &r = reqs.at(queue.front());
-            if (in_use + r.bytes > budget) break;
-            if (in_use + r.bytes + pool.cached > budget) pool.trim(budget - in_use - r.bytes);
-            uint8_t *base = static_cast<uint8_t *>(pool.get(r.bytes));
+        size_t looked = 0;
+        for (auto it = queue.begin(); it != queue.end() && active.size() < max_active && looked < admit_window;) {
+            looked++;
+            Req &r = reqs.at(*it);
+            if (in_use + r.bytes > budget) {
+                ++it;
+                continue;
+            }
+            uint8_t *base = arena.get(r.bytes, budget);
+            if (!base) {   // fragmented: wait for releases
+                ++it;
+                continue;
+            }
             r.dets = base;
             r.obs = base + ((size_t)r.shots * r.det_row + 255) / 256 * 256;
             r.P.dets.ptr = r.dets;
@@ -176,8 +203,8 @@ struct mfs_server {
             peak_in_use = std::max(peak_in_use, in_use);
             r.state = ACTIVE;
             r.t_admit = now_s();
-            active.push_back(queue.front());
-            queue.pop_front();
+            active.push_back(*it);
+            it = queue.erase(it);
         }
     }
 
@@ -197,24 +224,39 @@ struct mfs_server {
     bool step() {
         admit();
         if (active.empty()) return false;
-        // Shares: each active request gets up to step_cost / n of work; what is left goes in order.
+        // Shares: three quarters of the step by least remaining work, the rest to the oldest request.
         std::vector<std::pair<uint64_t, uint64_t>> take;   // (request id, tiles this step)
-        double left = step_cost;
-        const double fair = step_cost / (double)active.size();
-        for (uint64_t id : active) {
+        std::vector<uint64_t> order(active);
+        std::stable_sort(order.begin(), order.end(), [&](uint64_t a, uint64_t b) {
+            const Req &x = reqs.at(a), &y = reqs.at(b);
+            return (double)(x.tiles - x.next_tile) * x.tile_cost < (double)(y.tiles - y.next_tile) * y.tile_cost;
+        });
+        double left = 0.75 * step_cost;
+        auto give = [&](uint64_t id, double budget_left) -> double {
             Req &r = reqs.at(id);
-            const uint64_t want = r.tiles - r.next_tile;
-            const uint64_t share = std::max<uint64_t>(1, (uint64_t)(fair / r.tile_cost));
-            const uint64_t n = std::min(want, share);
-            take.push_back({id, n});
-            left -= (double)n * r.tile_cost;
-        }
-        for (auto &t : take) {
+            uint64_t already = 0;
+            for (auto &t : take)
+                if (t.first == id) already = t.second;
+            const uint64_t want = r.tiles - r.next_tile - already;
+            if (!want) return 0;
+            const uint64_t n = std::min<uint64_t>(want, std::max<uint64_t>(1, (uint64_t)(budget_left / r.tile_cost)));
+            bool found = false;
+            for (auto &t : take)
+                if (t.first == id) {
+                    t.second += n;
+                    found = true;
+                }
+            if (!found) take.push_back({id, n});
+            return (double)n * r.tile_cost;
+        };
+        for (uint64_t id : order) {
             if (left <= 0) break;
-            Req &r = reqs.at(t.first);
-            const uint64_t more = std::min<uint64_t>(r.tiles - r.next_tile - t.second, (uint64_t)(left / r.tile_cost));
-            t.second += more;
-            left -= (double)more * r.tile_cost;
+            left -= give(id, left);
+        }
+        left += 0.25 * step_cost;
+        for (uint64_t id : active) {   // oldest first
+            if (left <= 0) break;
+            left -= give(id, left);
         }
         // Work list: one job per request chunk.
         if (take.size() > h_cap) {

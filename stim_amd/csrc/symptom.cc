@@ -2,6 +2,7 @@
 // outcome table, and lays out one symptom column per basis fault. See symptom.h.
 #include "symptom.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -168,7 +169,17 @@ SymPlan plan_symptoms(const Program &prog) {
         plan.classes.push_back(sc);
     }
     plan.total_sites = sites;
-    if (plan.columns.size() >= (1ull << 31)) {
+    // Compact record slots: each class starts at a multiple of its stride, sites are stride apart.
+    uint64_t slot = 0;
+    for (SymClass &sc : plan.classes) {
+        sc.rstride = sc.basis <= 2 ? sc.basis : sc.basis <= 4 ? 4 : sc.basis;
+        const uint32_t align = sc.rstride <= 4 ? std::max<uint32_t>(sc.rstride, 1) : 1;
+        slot = (slot + align - 1) / align * align;
+        sc.rec0 = (uint32_t)slot;
+        slot += (uint64_t)sc.nsites * sc.rstride;
+    }
+    plan.total_slots = slot;
+    if (plan.columns.size() >= (1ull << 31) || plan.total_slots >= (1ull << 31)) {
         plan.supported = false;
         plan.why_unsupported = "too many noise sites for the symptom table";
     }

@@ -34,9 +34,12 @@ struct SymClass {
     uint32_t woff;   // outcome table offset ((threshold, code-bitmask) pairs)
     uint32_t nout;   // outcomes | SYM_UNIFORM when all outcomes are equally likely
     uint32_t site0;  // cumulative sites of earlier classes
+    uint32_t rec0;     // first slot of the class in the compact record table (a multiple of rstride <= 4)
+    uint32_t rstride;  // compact slots per site: basis rounded up to 1, 2 or 4 (basis itself above 4)
+    uint32_t pad0, pad1;
 };
 constexpr uint32_t SYM_UNIFORM = 1u << 31;
-static_assert(sizeof(SymClass) == 32, "SymClass layout");
+static_assert(sizeof(SymClass) == 48, "SymClass layout");
 
 // Basis fault of one column, as a fixed-fault entry for the frame kernel.
 struct SymColumn {
@@ -48,6 +51,7 @@ struct SymPlan {
     std::vector<uint32_t> otab;        // outcome tables (threshold, basis-bitmask code)
     std::vector<SymColumn> columns;    // column -> basis fault
     uint64_t total_sites = 0;
+    uint64_t total_slots = 0;          // compact record slots (columns plus alignment padding)
     // op index -> (class, first class-site of the op's sites) for fixed-fault XOR; chain links map
     // to (class, link index) via chain_link.
     std::vector<uint32_t> op_class, op_site0, op_link;
@@ -60,6 +64,11 @@ SymPlan plan_symptoms(const Program &prog);
 // Column record (8 bytes): x bits 28..31 = entry count c; c <= 2: entries inline (x bits 0..27,
 // then y); c == 3: x bits 0..27 = offset into the overflow entries, y = entry count.
 constexpr uint32_t SYM_INLINE_MAX = 2;
+// Compact column record (4 bytes), used when every detector and observable index is below 2^15:
+// bits 30..31 = entry count c; c <= 2: entries in bits 0..14 and 15..29; c == 3: bits 0..29 = offset
+// of the entry count in the compact overflow list, followed by the entries. A class's records start at
+// a slot aligned to its stride, so one 4-, 8- or 16-byte load reads every basis record of a site.
+constexpr uint32_t SYM_COMPACT_LIMIT = 1u << 15;
 constexpr uint32_t SYM_OTAB_LDS_WORDS = 1024;  // outcome tables up to this size are read from LDS
 // LDS hit queue of the sampling kernel: 256 lanes x 8 hits x 8 bytes, plus a counter.
 constexpr size_t SYM_QUEUE_BYTES = 256 * 8 * 8 + 16;
@@ -69,6 +78,8 @@ struct SymDevParams {
     const uint32_t *otab;
     const uint2 *recs;             // one record per column
     const uint32_t *ovf;           // overflow entries
+    const uint32_t *recs32;        // compact records by slot (null when not available)
+    const uint32_t *ovf32;         // compact overflow list
     uint32_t nclasses;
     uint32_t otab_words;           // outcome-table words (copied to LDS when small)
     uint32_t D, O;

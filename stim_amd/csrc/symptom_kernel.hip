@@ -394,6 +394,55 @@ __device__ __forceinline__ void sink_hit(const SymDevParams &P, const Sink &k, u
     }
 }
 
+// Compact records (4 bytes; see symptom.h).
+__device__ __forceinline__ void sink_record32(const Sink &k, const uint32_t *ovf32, uint32_t rec, uint32_t dbase,
+                                              uint32_t obase) {
+    const uint32_t c = rec >> 30;
+    if (c == 3) {
+        const uint32_t off = rec & 0x3FFFFFFFu;
+        const uint32_t n = ovf32[off];
+        for (uint32_t e = 0; e < n; e++) sink_bit(k, ovf32[off + 1 + e], dbase, obase);
+        return;
+    }
+    if (c >= 1) sink_bit(k, rec & 0x7FFFu, dbase, obase);
+    if (c == 2) sink_bit(k, (rec >> 15) & 0x7FFFu, dbase, obase);
+}
+
+// Applies one hit with compact records: a single 16-, 8- or 4-byte load fetches every basis record
+// of the site (classes with more than 4 basis columns load the records they use one by one).
+__device__ __forceinline__ void sink_hit32(const SymDevParams &P, const Sink &k, const SymClass &cl, uint32_t site,
+                                           uint32_t code, uint32_t shot) {
+    const uint32_t dbase = shot * k.det_bits + k.det_col0;
+    const uint32_t obase = k.obs_base + shot * k.obs_bits + k.obs_col0 - k.D;
+    const uint32_t slot = cl.rec0 + site * cl.rstride;
+    if (cl.rstride <= 4) {
+        uint32_t r[4] = {0, 0, 0, 0};
+        if (cl.rstride == 4) {
+            const uint4 v = *reinterpret_cast<const uint4 *>(P.recs32 + slot);
+            r[0] = v.x;
+            r[1] = v.y;
+            r[2] = v.z;
+            r[3] = v.w;
+        } else if (cl.rstride == 2) {
+            const uint2 v = *reinterpret_cast<const uint2 *>(P.recs32 + slot);
+            r[0] = v.x;
+            r[1] = v.y;
+        } else {
+            r[0] = P.recs32[slot];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if ((code >> j) & 1) sink_record32(k, P.ovf32, r[j], dbase, obase);
+        }
+    } else {
+        while (code) {
+            const uint32_t j = __ffs(code) - 1;
+            code &= code - 1;
+            sink_record32(k, P.ovf32, P.recs32[slot + j], dbase, obase);
+        }
+    }
+}
+
 // One tile of the scan sampler: tile `btile` of the launch described by P, in the workgroup's LDS.
 __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile, uint32_t *lds) {
     const uint32_t S = P.shots_per_tile;
@@ -456,8 +505,10 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
                     pos += g[k];
                     if (pos < N) {
                         const uint32_t l = (uint32_t)pos;
-                        sink_hit(P, sink, cl.col0 + (l >> P.tile_log2) * cl.basis,
-                                 outcome(otab, cl.woff, cl.nout, oc[k]), l & (S - 1), cl.basis);
+                        const uint32_t site = l >> P.tile_log2;
+                        const uint32_t code = outcome(otab, cl.woff, cl.nout, oc[k]);
+                        if (P.recs32) sink_hit32(P, sink, cl, site, code, l & (S - 1));
+                        else sink_hit(P, sink, cl.col0 + site * cl.basis, code, l & (S - 1), cl.basis);
                     }
                     pos += 1;
                 }
@@ -482,10 +533,11 @@ __global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
     scan_tile(P, blockIdx.x, lds);
 }
 
-// Batched form: many launches (jobs, possibly of different circuits) in one persistent kernel.
-// Workgroups claim chunks of BATCH_CHUNK consecutive global tile numbers from a counter; job j owns
-// tiles [prefix[j], prefix[j+1]). The job of the current tile is cached, so the search only runs when
-// a chunk crosses into another job. Every workgroup leaves once the counter passes the last tile.
+// Batched form: many launches (jobs, possibly of different circuits) in one kernel; job j owns global
+// tiles [prefix[j], prefix[j+1]). Persistent schedule (grid smaller than the tile count): workgroups
+// claim chunks of BATCH_CHUNK consecutive tiles from a counter, caching the current job so the search
+// only runs when a chunk crosses into another job, and leave once the counter passes the last tile.
+// With a grid of one workgroup per tile, each workgroup samples its own tile.
 constexpr uint32_t BATCH_CHUNK = 8;
 
 __global__ __launch_bounds__(NT) void sym_batch_kernel(const SymDevParams *__restrict__ jobs,
@@ -494,6 +546,22 @@ __global__ __launch_bounds__(NT) void sym_batch_kernel(const SymDevParams *__res
     extern __shared__ __align__(16) uint32_t lds[];
     __shared__ uint32_t next;
     const uint64_t total = prefix[njobs];
+    if ((uint64_t)gridDim.x >= total) {
+        // One tile per workgroup (the dispatcher schedules the tiles).
+        const uint64_t t = blockIdx.x;
+        uint32_t a = 0, b = njobs;
+        while (b - a > 1) {
+            const uint32_t mid = (a + b) / 2;
+            if (prefix[mid] <= t) a = mid;
+            else b = mid;
+        }
+        const uint32_t j = __builtin_amdgcn_readfirstlane(a);
+        if (t < total) {
+            const SymDevParams P = jobs[j];
+            scan_tile(P, (uint32_t)(t - prefix[j]), lds);
+        }
+        return;
+    }
     uint32_t j = 0;
     uint64_t lo = 0, hi = 0;   // tile range of job j (empty until the first search)
     for (;;) {
@@ -516,7 +584,8 @@ __global__ __launch_bounds__(NT) void sym_batch_kernel(const SymDevParams *__res
                 hi = prefix[j + 1];
             }
             if (t != t0) __syncthreads();   // the previous tile's rows are written before the image is reused
-            scan_tile(jobs[j], (uint32_t)(t - lo), lds);
+            const SymDevParams P = jobs[j];
+            scan_tile(P, (uint32_t)(t - lo), lds);
         }
     }
 }

@@ -59,6 +59,16 @@ def test_batch_graph_replays_continue_the_stream():
         _check(got, [_plain(s, n) for s, n in zip(ref, SHOTS)])
 
 
+def test_one_workgroup_per_tile_schedule_equals_plain_calls():
+    from stim_amd.batch import BatchSampler
+    batch = BatchSampler(list(zip(_samplers(), SHOTS)))
+    batch.set_schedule(False)
+    ref = _samplers()
+    for _ in range(2):
+        _check(batch.run(use_graph=True), [_plain(s, n) for s, n in zip(ref, SHOTS)])
+    assert batch.info()['persistent'] == 0
+
+
 def test_same_sampler_twice_in_one_batch():
     import stim_amd
     from stim_amd.batch import BatchSampler
@@ -90,11 +100,12 @@ def test_autotuner_choice_keeps_batch_equal_to_plain_calls():
     kb, ms = batch.autotune(candidates=(8, 16, 32), iters=1)
     assert kb in (8, 16, 32) and ms > 0
     ref = _samplers()
-    # Bring the reference samplers to the same stream position: the autotuner ran 2 runs per candidate.
+    # Bring the reference samplers to the same stream position: the autotuner ran 2 runs per candidate
+    # tile size and schedule (the schedule does not change the results).
     for s, n in zip(ref, SHOTS):
         for cand in (8, 16, 32):
             s.set_symptom_tile_kb(cand)
-            for _ in range(2):
+            for _ in range(4):
                 _plain(s, n)
         s.set_symptom_tile_kb(kb)
     got = batch.run()

@@ -325,6 +325,24 @@ def test_consecutive_calls_draw_fresh_randomness(engine):
     assert repeats <= 2, repeats
 
 
+def test_wide_symptom_records_agree_with_frame_engine():
+    """More than 2^15 detectors: the symptom engine uses its 8-byte records; rates must agree with the
+    frame engine's."""
+    from stim_amd import gen
+    text = gen.generate('repetition_code:memory', distance=185, rounds=185, after_clifford_depolarization=0.01,
+                        after_reset_flip_probability=0.01, before_measure_flip_probability=0.01)
+    shots = 1 << 15
+    a = _sampler(text, seed=31, engine='frame').sample(shots, bit_packed=True, append_observables=True)
+    b = _sampler(text, seed=32, engine='symptom').sample(shots, bit_packed=True, append_observables=True)
+    n = 184 * 186 + 1
+    assert n > 1 << 15
+    ra = np.unpackbits(a, axis=1, bitorder='little')[:, :n].mean(axis=0)
+    rb = np.unpackbits(b, axis=1, bitorder='little')[:, :n].mean(axis=0)
+    p = (ra + rb) / 2
+    sd = np.sqrt(np.maximum(p * (1 - p), 1e-12) * 2 / shots)
+    assert np.max(np.abs((ra - rb) / sd)) < _zcrit(n)
+
+
 def test_engines_agree_statistically_on_zoo():
     """Frame and symptom engines sample the same distribution on an all-gate circuit."""
     text = zoo.gate_zoo(7, n_qubits=10, layers=12)
