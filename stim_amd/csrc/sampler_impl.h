@@ -131,7 +131,7 @@ struct mfs_sampler {
     SymPlan plan;
     uint64_t sym_entries = 0;
     double sym_build_ms = 0;
-    DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols, d_recs32, d_ovf32;
+    DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols, d_recs32, d_ovf32, d_groups;
     bool sym_compact = false;
     int sym_tile_kb = 32;        // target LDS image size per symptom tile
     int sym_scan = 1;            // scan kernel (0: per-lane-walk queue kernel)
@@ -491,7 +491,28 @@ struct mfs_sampler {
             }
             upload(d_recs32, r32, "compact symptom records");
             upload(d_ovf32, o32, "compact symptom overflow entries");
+        } else {
+            // Without compact records a hit needs its class's column layout: one group per class.
+            plan.groups.clear();
+            for (uint32_t c = 0; c < plan.classes.size(); c++) {
+                const SymClass &sc = plan.classes[c];
+                SymGroup g{};
+                g.p = sc.p;
+                g.inv_log = sc.inv_log;
+                g.site0 = sc.site0;
+                g.nsites = sc.nsites;
+                g.rec0 = sc.rec0;
+                g.rstride = sc.rstride;
+                g.nmem = 1;
+                g.c0 = c;
+                for (int k = 0; k < 4; k++) g.bound[k] = 0xFFFFFFFFu;
+                g.bound[0] = 0;
+                g.nout[0] = sc.nout;
+                g.woff[0] = sc.woff;
+                plan.groups.push_back(g);
+            }
         }
+        upload(d_groups, plan.groups, "symptom groups");
         sym_state = 1;
         return true;
     }
@@ -525,6 +546,8 @@ struct mfs_sampler {
     SymDevParams sym_params(uint32_t S) {
         SymDevParams P{};
         P.classes = d_classes.as<SymClass>();
+        P.groups = d_groups.as<SymGroup>();
+        P.ngroups = (uint32_t)plan.groups.size();
         P.otab = d_sotab.as<uint32_t>();
         P.recs = d_recs.as<uint2>();
         P.ovf = d_ovf.as<uint32_t>();

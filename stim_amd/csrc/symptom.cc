@@ -102,10 +102,25 @@ SymPlan plan_symptoms(const Program &prog) {
     }
     add_chain(chain);
 
-    // Lay out classes, sites and columns.
+    // Lay out classes, sites and columns, in order of firing probability so that classes with equal
+    // probability are adjacent (the sampler draws them as one process).
+    auto class_p = [&](uint32_t c) -> float {
+        if (!class_is_chain[c]) return prog.ops[class_ops[c][0]].p;
+        double none = 1.0;
+        for (uint32_t l : class_ops[c]) none *= 1.0 - (double)prog.ops[l].p;
+        return (float)(1.0 - none);
+    };
+    std::vector<uint32_t> order(class_ops.size());
+    for (uint32_t c = 0; c < order.size(); c++) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return class_p(a) < class_p(b); });
+    std::vector<uint32_t> new_index(class_ops.size());
+    for (uint32_t k = 0; k < order.size(); k++) new_index[order[k]] = k;
+    for (uint32_t &c : plan.op_class) {
+        if (c != 0xFFFFFFFFu) c = new_index[c];
+    }
     uint32_t col = 0;
     uint64_t sites = 0;
-    for (uint32_t c = 0; c < class_ops.size(); c++) {
+    for (uint32_t c : order) {
         SymClass sc{};
         sc.col0 = col;
         sc.site0 = (uint32_t)sites;
@@ -152,6 +167,15 @@ SymPlan plan_symptoms(const Program &prog) {
                     if (std::fabs((double)prog.otab[first.woff + 2 * k] - want) > 2.0) uniform = false;
                 }
                 if (uniform) sc.nout |= SYM_UNIFORM;
+                // Uniform over the codes 1..nout: the kernel computes the code without the table.
+                bool seq = uniform;
+                std::vector<bool> seen(sc.nout & 0xFFFF, false);
+                for (uint32_t k = 0; seq && k < (sc.nout & 0xFFFF); k++) {
+                    const uint32_t code = prog.otab[first.woff + 2 * k + 1];
+                    if (code < 1 || code > (sc.nout & 0xFFFF) || seen[code - 1]) seq = false;
+                    else seen[code - 1] = true;
+                }
+                if (seq) sc.nout |= SYM_SEQ;
             }
             uint32_t n = 0;
             for (uint32_t oi : class_ops[c]) {
@@ -172,13 +196,42 @@ SymPlan plan_symptoms(const Program &prog) {
     // Compact record slots: each class starts at a multiple of its stride, sites are stride apart.
     uint64_t slot = 0;
     for (SymClass &sc : plan.classes) {
-        sc.rstride = sc.basis <= 2 ? sc.basis : sc.basis <= 4 ? 4 : sc.basis;
-        const uint32_t align = sc.rstride <= 4 ? std::max<uint32_t>(sc.rstride, 1) : 1;
+        sc.rstride = sc.basis <= 4 ? 4 : sc.basis;
+        const uint32_t align = sc.rstride == 4 ? 4 : 1;
         slot = (slot + align - 1) / align * align;
         sc.rec0 = (uint32_t)slot;
         slot += (uint64_t)sc.nsites * sc.rstride;
     }
     plan.total_slots = slot;
+    // Groups: runs of up to 4 adjacent classes with equal probability and at most 4 basis columns,
+    // sampled as one Bernoulli process over their sites; every other class is a group of its own.
+    for (uint32_t c = 0; c < plan.classes.size(); c++) {
+        const SymClass &sc = plan.classes[c];
+        SymGroup *g = plan.groups.empty() ? nullptr : &plan.groups.back();
+        const bool joinable = sc.basis <= 4;
+        if (g && joinable && g->nmem < 4 && g->rstride == 4 && g->p == sc.p && g->inv_log == sc.inv_log) {
+            g->bound[g->nmem] = sc.site0 - g->site0;
+            g->nout[g->nmem] = sc.nout;
+            g->woff[g->nmem] = sc.woff;
+            g->nmem++;
+            g->nsites += sc.nsites;
+            continue;
+        }
+        SymGroup ng{};
+        ng.p = sc.p;
+        ng.inv_log = sc.inv_log;
+        ng.site0 = sc.site0;
+        ng.nsites = sc.nsites;
+        ng.rec0 = sc.rec0;
+        ng.rstride = joinable ? 4 : sc.rstride;
+        ng.nmem = 1;
+        ng.c0 = c;
+        for (int k = 0; k < 4; k++) ng.bound[k] = 0xFFFFFFFFu;
+        ng.bound[0] = 0;
+        ng.nout[0] = sc.nout;
+        ng.woff[0] = sc.woff;
+        plan.groups.push_back(ng);
+    }
     if (plan.columns.size() >= (1ull << 31) || plan.total_slots >= (1ull << 31)) {
         plan.supported = false;
         plan.why_unsupported = "too many noise sites for the symptom table";

@@ -39,7 +39,22 @@ struct SymClass {
     uint32_t pad0, pad1;
 };
 constexpr uint32_t SYM_UNIFORM = 1u << 31;
+constexpr uint32_t SYM_SEQ = 1u << 30;   // uniform over the codes 1..nout (no table lookup needed)
 static_assert(sizeof(SymClass) == 48, "SymClass layout");
+
+// A group of classes sampled as one Bernoulli process: classes with equal probability, adjacent in
+// site order, whose compact records are 4 slots per site, so a hit's record slot is
+// rec0 + (site - site0) * rstride whatever its class; member k starts at group-relative site bound[k].
+struct SymGroup {
+    float p, inv_log;
+    uint32_t site0, nsites;
+    uint32_t rec0, rstride;
+    uint32_t nmem, c0;
+    uint32_t bound[4];   // unused members: 0xFFFFFFFF
+    uint32_t nout[4];
+    uint32_t woff[4];
+};
+static_assert(sizeof(SymGroup) == 80, "SymGroup layout");
 
 // Basis fault of one column, as a fixed-fault entry for the frame kernel.
 struct SymColumn {
@@ -50,6 +65,7 @@ struct SymPlan {
     std::vector<SymClass> classes;
     std::vector<uint32_t> otab;        // outcome tables (threshold, basis-bitmask code)
     std::vector<SymColumn> columns;    // column -> basis fault
+    std::vector<SymGroup> groups;      // sampling groups over the classes
     uint64_t total_sites = 0;
     uint64_t total_slots = 0;          // compact record slots (columns plus alignment padding)
     // op index -> (class, first class-site of the op's sites) for fixed-fault XOR; chain links map
@@ -75,6 +91,8 @@ constexpr size_t SYM_QUEUE_BYTES = 256 * 8 * 8 + 16;
 
 struct SymDevParams {
     const SymClass *classes;
+    const SymGroup *groups;
+    uint32_t ngroups;
     const uint32_t *otab;
     const uint2 *recs;             // one record per column
     const uint32_t *ovf;           // overflow entries

@@ -278,13 +278,14 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
                    P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
 }
 
-// Scan formulation of the sparse draw (the workhorse sampler). For each site class, the tile's
-// N = sites * S positions carry a Bernoulli(p) process; its hits are the partial sums of iid
-// geometric gaps (the RareErrorIterator recurrence, probability_util.h:33-71). Every lane draws the
-// same number m of gaps per round (no divergence), a workgroup prefix scan over the lanes' gap sums
-// lays the concatenated gap sequence along the positions, hits that land inside [0, N) are applied,
-// and rounds repeat until the sequence passes N. m is chosen so one round covers N with high
-// probability. Streams are keyed by (tile, class, round, lane).
+// Scan formulation of the sparse draw (the workhorse sampler). For each sampling group (classes of
+// equal probability, adjacent in site order), the tile's N = sites * S positions carry a Bernoulli(p)
+// process; its hits are the partial sums of iid geometric gaps (the RareErrorIterator recurrence,
+// probability_util.h:33-71). Every lane draws the same number m of gaps per round (no divergence), a
+// workgroup prefix scan over the lanes' gap sums lays the concatenated gap sequence along the
+// positions, hits that land inside [0, N) are applied, and rounds repeat until the sequence passes N.
+// m is chosen so one round covers N with high probability. Streams are keyed by (tile, group, round,
+// lane).
 constexpr uint32_t SCAN_M_MAX = 8;
 
 __device__ __forceinline__ uint32_t gap32(uint32_t r, float inv_log2_1mp) {
@@ -408,32 +409,19 @@ __device__ __forceinline__ void sink_record32(const Sink &k, const uint32_t *ovf
     if (c == 2) sink_bit(k, (rec >> 15) & 0x7FFFu, dbase, obase);
 }
 
-// Applies one hit with compact records: a single 16-, 8- or 4-byte load fetches every basis record
-// of the site (classes with more than 4 basis columns load the records they use one by one).
-__device__ __forceinline__ void sink_hit32(const SymDevParams &P, const Sink &k, const SymClass &cl, uint32_t site,
-                                           uint32_t code, uint32_t shot) {
+// Applies one hit with compact records: the site's basis records start at `slot`, `rstride` apart
+// sites; a single 16-byte load fetches all of them when rstride is 4 (classes with more than 4 basis
+// columns load the records they use one by one).
+__device__ __forceinline__ void sink_slot32(const SymDevParams &P, const Sink &k, uint32_t slot, uint32_t rstride,
+                                            uint32_t code, uint32_t shot) {
     const uint32_t dbase = shot * k.det_bits + k.det_col0;
     const uint32_t obase = k.obs_base + shot * k.obs_bits + k.obs_col0 - k.D;
-    const uint32_t slot = cl.rec0 + site * cl.rstride;
-    if (cl.rstride <= 4) {
-        uint32_t r[4] = {0, 0, 0, 0};
-        if (cl.rstride == 4) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(P.recs32 + slot);
-            r[0] = v.x;
-            r[1] = v.y;
-            r[2] = v.z;
-            r[3] = v.w;
-        } else if (cl.rstride == 2) {
-            const uint2 v = *reinterpret_cast<const uint2 *>(P.recs32 + slot);
-            r[0] = v.x;
-            r[1] = v.y;
-        } else {
-            r[0] = P.recs32[slot];
-        }
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if ((code >> j) & 1) sink_record32(k, P.ovf32, r[j], dbase, obase);
-        }
+    if (rstride == 4) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(P.recs32 + slot);
+        if (code & 1) sink_record32(k, P.ovf32, v.x, dbase, obase);
+        if (code & 2) sink_record32(k, P.ovf32, v.y, dbase, obase);
+        if (code & 4) sink_record32(k, P.ovf32, v.z, dbase, obase);
+        if (code & 8) sink_record32(k, P.ovf32, v.w, dbase, obase);
     } else {
         while (code) {
             const uint32_t j = __ffs(code) - 1;
@@ -468,13 +456,14 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
     const uint64_t s0 = (uint64_t)btile * S;
     const uint32_t tile = (uint32_t)(P.tile0 + btile);
     const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
-    for (uint32_t c = 0; c < P.nclasses; c++) {
-        const SymClass cl = P.classes[c];
-        if (!(cl.p > 0.0f)) continue;
-        const uint32_t N = cl.nsites << P.tile_log2;
-        // Gaps per lane per round: cover the expected hits plus six standard deviations.
-        const float mu = (float)N * cl.p;
-        uint32_t m = (uint32_t)((mu + 6.0f * sqrtf(mu) + 8.0f) / NT) + 1;
+    for (uint32_t gi = 0; gi < P.ngroups; gi++) {
+        const SymGroup G = P.groups[gi];
+        if (!(G.p > 0.0f)) continue;
+        const uint32_t N = G.nsites << P.tile_log2;
+        // Gaps per lane per round: cover the expected hits plus three standard deviations (a short
+        // extra round follows in the rare tiles that need it).
+        const float mu = (float)N * G.p;
+        uint32_t m = (uint32_t)((mu + 3.0f * sqrtf(mu) + 4.0f) / NT) + 1;
         if (m > SCAN_M_MAX) m = SCAN_M_MAX;
         uint32_t base = 0;
         for (uint32_t round = 0; base < N; round++) {
@@ -483,10 +472,10 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
 #pragma unroll
             for (uint32_t k = 0; k < SCAN_M_MAX / 2; k++) {
                 if (2 * k < m) {
-                    const uint4 r = philox10(make_uint4(round * (SCAN_M_MAX / 2) + k, tid, tile, TAG_SCAN | c), k0, k1);
-                    g[2 * k] = gap32(r.x, cl.inv_log);
+                    const uint4 r = philox10(make_uint4(round * (SCAN_M_MAX / 2) + k, tid, tile, TAG_SCAN | gi), k0, k1);
+                    g[2 * k] = gap32(r.x, G.inv_log);
                     oc[2 * k] = r.y;
-                    g[2 * k + 1] = gap32(r.z, cl.inv_log);
+                    g[2 * k + 1] = gap32(r.z, G.inv_log);
                     oc[2 * k + 1] = r.w;
                 } else {
                     g[2 * k] = g[2 * k + 1] = 0;
@@ -498,23 +487,41 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
                 if (k < m) sum = sat_add(sum, g[k] + 1);
             }
             const uint2 sc = block_scan(sum, scratch);
-            uint64_t pos = (uint64_t)base + sc.x;
+            // Positions stay at most N < 2^31: the start and every gap are clamped at N.
+            uint32_t pos = sat_add(base, sc.x);
+            pos = pos < N ? pos : N;
 #pragma unroll
             for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
                 if (k < m) {
-                    pos += g[k];
+                    pos += g[k] < N ? g[k] : N;
+                    pos = pos < N ? pos : N;
                     if (pos < N) {
-                        const uint32_t l = (uint32_t)pos;
-                        const uint32_t site = l >> P.tile_log2;
-                        const uint32_t code = outcome(otab, cl.woff, cl.nout, oc[k]);
-                        if (P.recs32) sink_hit32(P, sink, cl, site, code, l & (S - 1));
-                        else sink_hit(P, sink, cl.col0 + site * cl.basis, code, l & (S - 1), cl.basis);
+                        const uint32_t srel = pos >> P.tile_log2;
+                        const uint32_t shot = pos & (S - 1);
+                        // Member class of the site (a group holds up to 4 classes in site order).
+                        const uint32_t mk = (srel >= G.bound[1] ? 1u : 0u) + (srel >= G.bound[2] ? 1u : 0u) +
+                                            (srel >= G.bound[3] ? 1u : 0u);
+                        const uint32_t nf = mk == 0 ? G.nout[0] : mk == 1 ? G.nout[1] : mk == 2 ? G.nout[2] : G.nout[3];
+                        uint32_t code;
+                        if (nf & SYM_SEQ) {
+                            code = __umulhi(oc[k], nf & 0xFFFFu) + 1;
+                        } else {
+                            const uint32_t wo =
+                                mk == 0 ? G.woff[0] : mk == 1 ? G.woff[1] : mk == 2 ? G.woff[2] : G.woff[3];
+                            code = outcome(otab, wo, nf, oc[k]);
+                        }
+                        if (P.recs32) {
+                            sink_slot32(P, sink, G.rec0 + srel * G.rstride, G.rstride, code, shot);
+                        } else {
+                            const SymClass cl = P.classes[G.c0];
+                            sink_hit(P, sink, cl.col0 + srel * cl.basis, code, shot, cl.basis);
+                        }
                     }
                     pos += 1;
                 }
             }
-            const uint64_t nb = (uint64_t)base + sc.y;
-            base = nb >= N ? N : (uint32_t)nb;
+            base = sat_add(base, sc.y);
+            base = base < N ? base : N;
         }
     }
     __syncthreads();
