@@ -18,16 +18,25 @@ struct mfs_batch {
     int persistent = 1;              // 1: persistent workgroups claiming tiles; 0: one workgroup per tile
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    DevBuf d_jobs, d_prefix, d_counter;
+    DevBuf d_jobs, d_prefix, d_counter, d_advance;
+    // Occupancy levels: jobs grouped by the workgroups per CU their own LDS allows, one persistent
+    // kernel per level (each at the occupancy a plain launch of its jobs would get).
+    struct Level {
+        size_t lds;
+        uint32_t grid;
+        uint32_t first, count;   // jobs [first, first + count) of the uploaded list
+        uint64_t tiles;
+    };
+    std::vector<Level> levels;
     // Prepared state.
     bool prepared = false;
     std::vector<uint32_t> sym_jobs;      // job indices run by the persistent kernel, in launch order
     std::vector<uint32_t> frame_jobs;    // job indices run as their own launches
-    std::vector<std::pair<mfs_sampler *, uint64_t>> advance;  // RNG tiles each sampler uses per run
+    std::vector<std::pair<mfs_sampler *, uint64_t>> advance;  // RNG tiles each sampler uses per run (all its jobs)
     std::vector<std::pair<mfs_sampler *, uint64_t>> expect;   // sampler tile counters the params assume
     uint64_t total_tiles = 0;
-    size_t lds = 0;
-    uint32_t grid = 0;
+    size_t lds = 0;      // largest level's LDS
+    uint32_t grid = 0;   // workgroups over all levels
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 
@@ -70,9 +79,11 @@ struct mfs_batch {
             uint64_t tiles;
             double cost;
             uint32_t job;
+            size_t lds;
+            int per_cu;
+            mfs_sampler *s;
         };
         std::vector<Item> items;
-        size_t max_lds = 0;
         for (uint32_t j = 0; j < jobs.size(); j++) {
             Job &jb = jobs[j];
             mfs_sampler *s = jb.s;
@@ -89,6 +100,7 @@ struct mfs_batch {
                 frame_jobs.push_back(j);
                 continue;
             }
+            if (r.lds_scan + 64 > MAX_LDS) throw std::invalid_argument("a batch job's tile does not fit in LDS");
             uint64_t &t0 = slot(s);
             r.P.tile0 = t0;
             t0 += r.tiles;
@@ -97,8 +109,8 @@ struct mfs_batch {
             const double S = r.P.shots_per_tile;
             // Per-tile cost estimate: hits applied plus output bytes written, plus a fixed overhead.
             const double cost = S * (hits * 16.0 + r.P.det_row_bytes + r.P.obs_row_bytes) + 4096.0;
-            items.push_back({r.P, r.tiles, cost, j});
-            max_lds = std::max(max_lds, r.lds_scan);
+            items.push_back({r.P, r.tiles, cost, j, std::max<size_t>(r.lds_scan, 64),
+                             sym_batch_blocks_per_cu(std::max<size_t>(r.lds_scan, 64)), s});
         }
         for (auto &kv : next_tile) {
             uint64_t start = 0;
@@ -106,40 +118,66 @@ struct mfs_batch {
                 if (e.first == kv.first) start = e.second;
             advance.push_back({kv.first, kv.second - start});
         }
-        // Heaviest tiles first, so the tail of the run is made of light tiles.
-        std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+        // Levels by occupancy (highest first); inside a level, heaviest tiles first so the tail of the
+        // run is made of light tiles.
+        std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) {
+            return a.per_cu != b.per_cu ? a.per_cu > b.per_cu : a.cost > b.cost;
+        });
         std::vector<SymDevParams> params;
-        std::vector<uint64_t> prefix{0};
-        for (const Item &it : items) {
-            params.push_back(it.P);
-            prefix.push_back(prefix.back() + it.tiles);
-            sym_jobs.push_back(it.job);
+        std::vector<uint64_t> prefix, adv;
+        levels.clear();
+        total_tiles = 0;
+        lds = 64;
+        for (size_t i = 0; i < items.size();) {
+            size_t e = i;
+            Level lv{0, 0, (uint32_t)i, 0, 0};
+            std::vector<uint64_t> pre{0};
+            while (e < items.size() && items[e].per_cu == items[i].per_cu) {
+                params.push_back(items[e].P);
+                pre.push_back(pre.back() + items[e].tiles);
+                uint64_t a = 0;
+                for (auto &kv : advance)
+                    if (kv.first == items[e].s) a = kv.second;
+                adv.push_back(a);
+                sym_jobs.push_back(items[e].job);
+                lv.lds = std::max(lv.lds, items[e].lds);
+                e++;
+            }
+            lv.count = (uint32_t)(e - i);
+            lv.tiles = pre.back();
+            if (lv.tiles >= (1ull << 32)) throw std::invalid_argument("too many tiles in one batch");
+            prefix.insert(prefix.end(), pre.begin(), pre.end());
+            total_tiles += lv.tiles;
+            lds = std::max(lds, lv.lds);
+            levels.push_back(lv);
+            i = e;
         }
-        total_tiles = prefix.back();
-        if (total_tiles >= (1ull << 32)) throw std::invalid_argument("too many tiles in one batch");
-        if (max_lds > MAX_LDS) throw std::invalid_argument("a batch job's tile does not fit in LDS");
-        lds = std::max<size_t>(max_lds, 64);
         if (!stream) {
             check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
             check(hipEventCreate(&ev0), "event");
             check(hipEventCreate(&ev1), "event");
         }
+        grid = 0;
         if (!params.empty()) {
             d_jobs.ensure(params.size() * sizeof(SymDevParams));
             d_prefix.ensure(prefix.size() * sizeof(uint64_t));
-            d_counter.ensure(4);
+            d_advance.ensure(adv.size() * sizeof(uint64_t));
+            d_counter.ensure(levels.size() * 4);
             check(hipMemcpyAsync(d_jobs.p, params.data(), params.size() * sizeof(SymDevParams), hipMemcpyHostToDevice,
                                  stream), "upload batch");
             check(hipMemcpyAsync(d_prefix.p, prefix.data(), prefix.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
+                                 stream), "upload batch");
+            check(hipMemcpyAsync(d_advance.p, adv.data(), adv.size() * sizeof(uint64_t), hipMemcpyHostToDevice,
                                  stream), "upload batch");
             check(hipStreamSynchronize(stream), "upload batch");
             int dev = 0, cus = 0;
             check(hipGetDevice(&dev), "device");
             check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev), "device");
-            const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)sym_batch_blocks_per_cu(lds);
-            grid = persistent ? (uint32_t)std::min<uint64_t>(resident, total_tiles) : (uint32_t)total_tiles;
-        } else {
-            grid = 0;
+            for (Level &lv : levels) {
+                const uint64_t resident = (uint64_t)std::max(cus, 1) * (uint64_t)sym_batch_blocks_per_cu(lv.lds);
+                lv.grid = persistent ? (uint32_t)std::min<uint64_t>(resident, lv.tiles) : (uint32_t)lv.tiles;
+                grid += lv.grid;
+            }
         }
         prepared = true;
     }
@@ -159,9 +197,14 @@ struct mfs_batch {
     }
 
     void enqueue_sym() {
-        if (!grid) return;
-        check(launch_sym_batch(d_jobs.as<SymDevParams>(), d_prefix.as<uint64_t>(), (uint32_t)sym_jobs.size(),
-                               d_counter.as<uint32_t>(), grid, lds, 1, stream), "batch launch");
+        // Level l's jobs start at levels[l].first; its prefix array (count + 1 entries) at first + l.
+        for (size_t l = 0; l < levels.size(); l++) {
+            const Level &lv = levels[l];
+            check(launch_sym_batch(d_jobs.as<SymDevParams>() + lv.first, d_prefix.as<uint64_t>() + lv.first + l,
+                                   d_advance.as<uint64_t>() + lv.first, lv.count, d_counter.as<uint32_t>() + l,
+                                   lv.grid, lv.lds, stream),
+                  "batch launch");
+        }
     }
 
     // One run: the persistent kernel (plain launches or a graph replay), then the frame-engine jobs.
@@ -303,7 +346,7 @@ int mfs_batch_info(mfs_batch *b, uint64_t out[8]) {
         out[1] = b->sym_jobs.size();
         out[2] = b->frame_jobs.size();
         out[3] = b->total_tiles;
-        out[4] = b->grid;
+        out[4] = b->grid;   // workgroups over all occupancy levels
         out[5] = b->lds;
         out[6] = (uint64_t)b->persistent;
         out[7] = (uint64_t)b->tile_kb;

@@ -265,7 +265,7 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
             check(hipEventCreate(&s->ev1), "event");
         }
         const uint64_t tiles = (shots + s->W() - 1) / s->W();
-        const bool sym = s->use_symptoms(false);
+        const bool sym = s->use_symptoms(false, shots);
         auto one = [&]() {
             if (sym) {
                 s->tiles_used += s->sym_launch(L, shots, dd.as<uint8_t>(), L.det_row, od.as<uint8_t>(), L.obs_row,

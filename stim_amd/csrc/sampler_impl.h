@@ -34,8 +34,8 @@ hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, 
 hipError_t launch_sym_fixed(const SymDevParams &P, const uint64_t *off, const uint32_t *cols, uint32_t tiles,
                             size_t lds, hipStream_t stream);
 hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream);
-hipError_t launch_sym_batch(SymDevParams *jobs, const uint64_t *prefix, uint32_t njobs, uint32_t *counter,
-                            uint32_t grid, size_t lds, int advance, hipStream_t stream);
+hipError_t launch_sym_batch(SymDevParams *jobs, const uint64_t *prefix, const uint64_t *advance, uint32_t njobs,
+                            uint32_t *counter, uint32_t grid, size_t lds, hipStream_t stream);
 int sym_batch_blocks_per_cu(size_t lds);
 hipError_t launch_sym_count(const uint8_t *det, uint32_t det_row, const uint8_t *obs, uint32_t obs_row, uint64_t rows,
                             uint32_t *counts, hipStream_t stream);
@@ -133,6 +133,8 @@ struct mfs_sampler {
     double sym_build_ms = 0;
     DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols, d_recs32, d_ovf32, d_groups;
     bool sym_compact = false;
+    bool plan_counted = false;
+    uint64_t column_count = 0;
     int sym_tile_kb = 32;        // target LDS image size per symptom tile
     int sym_scan = 1;            // scan kernel (0: per-lane-walk queue kernel)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -269,7 +271,7 @@ struct mfs_sampler {
         }
         const bool device_out = flags & MFS_DEVICE_OUTPUT;
         if (shots == 0) return;
-        if (use_symptoms(faults != nullptr)) {
+        if (use_symptoms(faults != nullptr, shots)) {
             sample_symptom(L, shots, device_out, dets, dets_stride, obs, obs_stride, fault_offsets, faults, base_char,
                            newline);
             return;
@@ -517,7 +519,12 @@ struct mfs_sampler {
         return true;
     }
 
-    bool use_symptoms(bool fixed) {
+    // Whether a call of `shots` shots runs on the symptom engine. In automatic mode the symptom table
+    // is built only when it is small next to the work it saves: its build simulates one shot per basis
+    // fault, so a call of fewer than half as many shots as there are columns samples on the frame
+    // engine instead (until a larger call builds the table). ANY_SHOTS skips that test.
+    static constexpr uint64_t ANY_SHOTS = ~0ull;
+    bool use_symptoms(bool fixed, uint64_t shots = ANY_SHOTS) {
         if (engine_mode == 1) return false;
         if ((prog.num_detectors + 2ull * prog.num_observables) / 8 > 64 * 1024) {
             if (engine_mode == 2) throw std::invalid_argument("rows too wide for the symptom engine's LDS image");
@@ -529,12 +536,18 @@ struct mfs_sampler {
                 "the symptom engine needs deterministic detectors and observables; this circuit has random ones");
             return false;
         }
+        if (engine_mode == 0 && sym_state == 0 && shots != ANY_SHOTS) {
+            if (!plan_counted) {
+                column_count = plan_symptoms(prog).columns.size();
+                plan_counted = true;
+            }
+            if (column_count > 2 * shots) return false;
+        }
         const bool ok = ensure_symptoms();
         if (!ok && engine_mode == 2) throw std::invalid_argument("symptom engine unavailable: " + sym_why);
         return ok;
     }
 
-    // Shots per symptom tile: LDS image of about 32 KB.
     // Shots per symptom tile: the largest power of two (up to max_shots) whose output rows fit about
     // kb KB of LDS.
     uint32_t sym_tile_shots(size_t row_bytes, int kb, uint32_t max_shots) const {

@@ -290,8 +290,8 @@ struct mfs_server {
         check(hipMemcpyAsync(d_prefix.p, h_prefix, (take.size() + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream),
               "upload work list");
         const uint32_t g = grid(lds, tiles);
-        check(launch_sym_batch(d_jobs.as<SymDevParams>(), d_prefix.as<uint64_t>(), (uint32_t)take.size(),
-                               d_counter.as<uint32_t>(), g, lds, 0, stream), "serving step");
+        check(launch_sym_batch(d_jobs.as<SymDevParams>(), d_prefix.as<uint64_t>(), nullptr, (uint32_t)take.size(),
+                               d_counter.as<uint32_t>(), g, lds, stream), "serving step");
         check(hipStreamSynchronize(stream), "serving step");
         steps++;
         const double t = now_s();

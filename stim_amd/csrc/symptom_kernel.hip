@@ -535,7 +535,9 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
                    P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
 }
 
-__global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
+// Both scan kernels ask for 6 waves per SIMD (at most 80 VGPRs, no spills): the sampler is bound by
+// instruction issue and latency, so resident waves matter more than registers.
+__global__ __launch_bounds__(NT, 6) void sym_scan_kernel(SymDevParams P) {
     extern __shared__ __align__(16) uint32_t lds[];
     scan_tile(P, blockIdx.x, lds);
 }
@@ -547,7 +549,7 @@ __global__ __launch_bounds__(NT) void sym_scan_kernel(SymDevParams P) {
 // With a grid of one workgroup per tile, each workgroup samples its own tile.
 constexpr uint32_t BATCH_CHUNK = 8;
 
-__global__ __launch_bounds__(NT) void sym_batch_kernel(const SymDevParams *__restrict__ jobs,
+__global__ __launch_bounds__(NT, 6) void sym_batch_kernel(const SymDevParams *__restrict__ jobs,
                                                        const uint64_t *__restrict__ prefix, uint32_t njobs,
                                                        uint32_t *counter) {
     extern __shared__ __align__(16) uint32_t lds[];
@@ -598,15 +600,15 @@ __global__ __launch_bounds__(NT) void sym_batch_kernel(const SymDevParams *__res
 }
 
 // Batch bookkeeping kernels (kernels rather than memory operations, so the batch can be captured in a
-// graph): reset the tile counter before a run; advance every job's RNG tile base by its tile count
-// after it.
+// graph): reset the tile counter before a run; after it, advance every job's RNG tile base by the
+// tiles its sampler used in the run (all of that sampler's jobs), as a plain call would.
 __global__ void sym_batch_reset_kernel(uint32_t *counter) {
     if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0;
 }
 
-__global__ void sym_batch_advance_kernel(SymDevParams *jobs, const uint64_t *prefix, uint32_t njobs) {
+__global__ void sym_batch_advance_kernel(SymDevParams *jobs, const uint64_t *advance, uint32_t njobs) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j < njobs) jobs[j].tile0 += prefix[j + 1] - prefix[j];
+    if (j < njobs) jobs[j].tile0 += advance[j];
 }
 
 // Fixed-fault mode of the symptom engine: shot s gets the XOR of the columns listed in
@@ -702,18 +704,25 @@ hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hi
 }
 
 // The dynamic LDS attribute is set by sym_batch_blocks_per_cu (before any graph capture).
-hipError_t launch_sym_batch(SymDevParams *jobs, const uint64_t *prefix, uint32_t njobs, uint32_t *counter,
-                            uint32_t grid, size_t lds, int advance, hipStream_t stream) {
+// advance == null: no tile-base update (callers that set the bases themselves).
+hipError_t launch_sym_batch(SymDevParams *jobs, const uint64_t *prefix, const uint64_t *advance, uint32_t njobs,
+                            uint32_t *counter, uint32_t grid, size_t lds, hipStream_t stream) {
     hipLaunchKernelGGL(sym_batch_reset_kernel, dim3(1), dim3(64), 0, stream, counter);
     hipLaunchKernelGGL(sym_batch_kernel, dim3(grid), dim3(NT), lds, stream, jobs, prefix, njobs, counter);
     if (advance)
-        hipLaunchKernelGGL(sym_batch_advance_kernel, dim3((njobs + 255) / 256), dim3(256), 0, stream, jobs, prefix,
+        hipLaunchKernelGGL(sym_batch_advance_kernel, dim3((njobs + 255) / 256), dim3(256), 0, stream, jobs, advance,
                            njobs);
     return hipGetLastError();
 }
 
+// Raises (never lowers) the batch kernel's dynamic-LDS limit to lds, then returns how many of its
+// workgroups with that much LDS fit on one CU. Every launch size is announced through here first.
 int sym_batch_blocks_per_cu(size_t lds) {
-    (void)hipFuncSetAttribute((const void *)sym_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static size_t reserved = 0;
+    if (lds > reserved) {
+        (void)hipFuncSetAttribute((const void *)sym_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        reserved = lds;
+    }
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sym_batch_kernel, NT, lds) != hipSuccess) return 1;
     return n > 0 ? n : 1;

@@ -319,7 +319,9 @@ def _fmt(p):
 
 
 def random_clifford(n_qubits=10000, layers=1000, seed=0, px=3e-5, py=3e-5, pz=3e-5, p2=1e-4, block=10,
-                    n_products=500, measured_fraction=0.1):
+                    n_products=500, measured_fraction=0.1, mpp_flip=0.0):
+    """mpp_flip > 0 makes the product measurements noisy (MPP(p)); without it the comparing detectors
+    are noiseless, since nothing acts between the two measurements of a product."""
     if n_qubits < 8 or n_qubits % 2:
         raise ValueError('n_qubits must be an even number >= 8')
     rng = random.Random(seed)
@@ -352,8 +354,9 @@ def random_clifford(n_qubits=10000, layers=1000, seed=0, px=3e-5, py=3e-5, pz=3e
             prods = []
             for k in range(n_products):
                 prods.append('*'.join(rng.choice('XYZ') + str(q) for q in chosen[4 * k:4 * k + 4]))
-            out.append('MPP ' + ' '.join(prods))
-            out.append('MPP ' + ' '.join(prods))
+            mpp = f'MPP({_fmt(mpp_flip)}) ' if mpp_flip > 0 else 'MPP '
+            out.append(mpp + ' '.join(prods))
+            out.append(mpp + ' '.join(prods))
             for k in range(n_products):
                 out.append(f'DETECTOR rec[-{n_products - k}] rec[-{2 * n_products - k}]')
             if not observable_done:

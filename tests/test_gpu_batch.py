@@ -27,7 +27,10 @@ SHOTS = [3000, 70001, 12345, 200000, 777, 4096]
 
 def _samplers(seed0=100):
     import stim_amd
-    return [stim_amd.Circuit(t).compile_detector_sampler(seed=seed0 + k) for k, t in enumerate(_circuits())]
+    out = [stim_amd.Circuit(t).compile_detector_sampler(seed=seed0 + k) for k, t in enumerate(_circuits())]
+    for s in out:
+        s.engine_info()   # builds the symptom tables, so small plain calls use them too
+    return out
 
 
 def _plain(s, shots):
@@ -77,9 +80,27 @@ def test_same_sampler_twice_in_one_batch():
     batch = BatchSampler([(s, 5000), (s, 3000)])
     got = batch.run()
     ref = stim_amd.Circuit(text).compile_detector_sampler(seed=9)
+    ref.engine_info()
     _check(got, [_plain(ref, 5000), _plain(ref, 3000)])
     # A plain call after the batch continues the stream.
     np.testing.assert_array_equal(_plain(s, 1000)[0], _plain(ref, 1000)[0])
+
+
+def test_same_sampler_twice_graph_replays_continue_the_stream():
+    """A sampler with two jobs in one batch advances by both jobs' tiles on every replay."""
+    import stim_amd
+    from stim_amd.batch import BatchSampler
+    text = _circuits()[2]
+    s = stim_amd.Circuit(text).compile_detector_sampler(seed=13)
+    other = stim_amd.Circuit(_circuits()[0]).compile_detector_sampler(seed=14)
+    batch = BatchSampler([(s, 6000), (other, 2000), (s, 3001)])
+    ref = stim_amd.Circuit(text).compile_detector_sampler(seed=13)
+    ref_o = stim_amd.Circuit(_circuits()[0]).compile_detector_sampler(seed=14)
+    ref.engine_info()
+    ref_o.engine_info()
+    for _ in range(3):
+        got = batch.run(use_graph=True)
+        _check(got, [_plain(ref, 6000), _plain(ref_o, 2000), _plain(ref, 3001)])
 
 
 def test_plain_call_between_runs_resyncs_the_batch():
@@ -88,6 +109,7 @@ def test_plain_call_between_runs_resyncs_the_batch():
     text = _circuits()[0]
     s = stim_amd.Circuit(text).compile_detector_sampler(seed=21)
     ref = stim_amd.Circuit(text).compile_detector_sampler(seed=21)
+    ref.engine_info()
     batch = BatchSampler([(s, 4000)])
     _check(batch.run(use_graph=True), [_plain(ref, 4000)])
     np.testing.assert_array_equal(_plain(s, 999)[0], _plain(ref, 999)[0])

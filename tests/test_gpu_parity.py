@@ -356,6 +356,22 @@ def test_engines_agree_statistically_on_zoo():
     assert np.max(np.abs(z)) < _zcrit(len(z))
 
 
+def test_random_clifford_engines_agree():
+    """The random-Clifford stress family (noisy MPP products, feedback, PAULI_CHANNEL_1, DEPOLARIZE2):
+    both engines sample the same detector rates."""
+    from stim_amd import gen
+    text = gen.random_clifford(n_qubits=400, layers=30, seed=3, px=2e-3, py=2e-3, pz=2e-3, p2=5e-3, n_products=60,
+                               mpp_flip=0.01)
+    shots = 1 << 16
+    a = _sampler(text, seed=41, engine='frame').sample(shots, append_observables=True)
+    b = _sampler(text, seed=42, engine='symptom').sample(shots, append_observables=True)
+    ra, rb = a.mean(axis=0), b.mean(axis=0)
+    assert ra.mean() > 0.005
+    p = (ra + rb) / 2
+    sd = np.sqrt(np.maximum(p * (1 - p), 1e-12) * 2 / shots)
+    assert np.max(np.abs((ra - rb) / sd)) < _zcrit(len(ra))
+
+
 def test_auto_engine_is_symptom_for_deterministic_circuits(golden):
     s = _sampler(golden.circuit('surface_d5_r5'))
     info = s.engine_info()
