@@ -636,40 +636,66 @@ __global__ void sym_fixed_kernel(SymDevParams P, const uint64_t *__restrict__ of
 }
 
 // Counts set bits of each column's dets row + obs row.
-__global__ void sym_count_kernel(const uint8_t *__restrict__ det, uint32_t det_row, const uint8_t *__restrict__ obs,
-                                 uint32_t obs_row, uint64_t rows, uint32_t *__restrict__ counts) {
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    uint32_t c = 0;
-    const uint8_t *d = det + r * det_row;
-    for (uint32_t i = 0; i < det_row; i++) c += __popc(d[i]);
-    const uint8_t *o = obs + r * obs_row;
-    for (uint32_t i = 0; i < obs_row; i++) c += __popc(o[i]);
-    counts[r] = c;
+// Symptom-table extraction from the fixed-fault rows (one row per basis fault), one wave per row: lanes
+// read the row's bytes 64 at a time (coalesced); the count kernel sums their popcounts, the fill kernel
+// places each lane's set bits after a wave prefix sum of the counts, so a row's entries come out in
+// increasing bit order.
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
 }
 
-__global__ void sym_fill_kernel(const uint8_t *__restrict__ det, uint32_t det_row, const uint8_t *__restrict__ obs,
-                                uint32_t obs_row, uint64_t rows, uint32_t D, const uint32_t *__restrict__ offs,
-                                uint32_t *__restrict__ ent) {
-    const uint64_t r = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ uint32_t wave_exclusive(uint32_t v, uint32_t lane) {
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x += y;
+    }
+    return x - v;
+}
+
+__global__ __launch_bounds__(256) void sym_count_kernel(const uint8_t *__restrict__ det, uint32_t det_row,
+                                                        const uint8_t *__restrict__ obs, uint32_t obs_row,
+                                                        uint64_t rows, uint32_t *__restrict__ counts) {
+    const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (r >= rows) return;   // whole waves leave together
+    uint32_t c = 0;
+    const uint8_t *d = det + r * det_row;
+    for (uint32_t i = lane; i < det_row; i += 64) c += __popc(d[i]);
+    const uint8_t *o = obs + r * obs_row;
+    for (uint32_t i = lane; i < obs_row; i += 64) c += __popc(o[i]);
+    c = wave_sum(c);
+    if (lane == 0) counts[r] = c;
+}
+
+__global__ __launch_bounds__(256) void sym_fill_kernel(const uint8_t *__restrict__ det, uint32_t det_row,
+                                                       const uint8_t *__restrict__ obs, uint32_t obs_row, uint64_t rows,
+                                                       uint32_t D, const uint32_t *__restrict__ offs,
+                                                       uint32_t *__restrict__ ent) {
+    const uint64_t r = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
     if (r >= rows) return;
     uint32_t k = offs[r];
-    const uint8_t *d = det + r * det_row;
-    for (uint32_t i = 0; i < det_row; i++) {
-        uint32_t v = d[i];
-        while (v) {
-            const uint32_t b = __ffs(v) - 1;
-            v &= v - 1;
-            ent[k++] = 8 * i + b;
-        }
-    }
-    const uint8_t *o = obs + r * obs_row;
-    for (uint32_t i = 0; i < obs_row; i++) {
-        uint32_t v = o[i];
-        while (v) {
-            const uint32_t b = __ffs(v) - 1;
-            v &= v - 1;
-            ent[k++] = D + 8 * i + b;
+    for (int part = 0; part < 2; part++) {
+        const uint8_t *src = part == 0 ? det + r * det_row : obs + r * obs_row;
+        const uint32_t len = part == 0 ? det_row : obs_row;
+        const uint32_t base_bit = part == 0 ? 0u : D;
+        for (uint32_t c0 = 0; c0 < len; c0 += 64) {
+            const uint32_t i = c0 + lane;
+            uint32_t v = i < len ? src[i] : 0u;
+            const uint32_t n = __popc(v);
+            const uint32_t before = wave_exclusive(n, lane);
+            const uint32_t total = __shfl(before + n, 63, 64);
+            uint32_t at = k + before;
+            while (v) {
+                const uint32_t b = __ffs(v) - 1;
+                v &= v - 1;
+                ent[at++] = base_bit + 8 * i + b;
+            }
+            k += total;
         }
     }
 }
@@ -739,7 +765,7 @@ hipError_t launch_sym_fixed(const SymDevParams &P, const uint64_t *off, const ui
 hipError_t launch_sym_count(const uint8_t *det, uint32_t det_row, const uint8_t *obs, uint32_t obs_row, uint64_t rows,
                             uint32_t *counts, hipStream_t stream) {
     if (!rows) return hipSuccess;
-    hipLaunchKernelGGL(sym_count_kernel, dim3((uint32_t)((rows + 255) / 256)), dim3(256), 0, stream, det, det_row, obs,
+    hipLaunchKernelGGL(sym_count_kernel, dim3((uint32_t)((rows + 3) / 4)), dim3(256), 0, stream, det, det_row, obs,
                        obs_row, rows, counts);
     return hipGetLastError();
 }
@@ -747,7 +773,7 @@ hipError_t launch_sym_count(const uint8_t *det, uint32_t det_row, const uint8_t 
 hipError_t launch_sym_fill(const uint8_t *det, uint32_t det_row, const uint8_t *obs, uint32_t obs_row, uint64_t rows,
                            uint32_t D, const uint32_t *offs, uint32_t *ent, hipStream_t stream) {
     if (!rows) return hipSuccess;
-    hipLaunchKernelGGL(sym_fill_kernel, dim3((uint32_t)((rows + 255) / 256)), dim3(256), 0, stream, det, det_row, obs,
+    hipLaunchKernelGGL(sym_fill_kernel, dim3((uint32_t)((rows + 3) / 4)), dim3(256), 0, stream, det, det_row, obs,
                        obs_row, rows, D, offs, ent);
     return hipGetLastError();
 }
