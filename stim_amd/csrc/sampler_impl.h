@@ -133,6 +133,7 @@ struct mfs_sampler {
     double sym_build_ms = 0;
     DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols, d_recs32, d_ovf32, d_groups;
     bool sym_compact = false;
+    bool sym_ovf32 = false;      // some compact record overflows
     bool plan_counted = false;
     uint64_t column_count = 0;
     int sym_tile_kb = 32;        // target LDS image size per symptom tile
@@ -468,7 +469,7 @@ struct mfs_sampler {
         upload(d_recs, recs, "symptom records");
         upload(d_ovf, ovf, "symptom overflow entries");
         // Compact records by slot, when every entry index fits 15 bits.
-        sym_compact = prog.num_detectors + prog.num_observables <= SYM_COMPACT_LIMIT;
+        sym_compact = prog.num_detectors <= SYM_COMPACT_DETECTORS && prog.num_observables <= SYM_COMPACT_OBSERVABLES;
         if (sym_compact) {
             std::vector<uint32_t> r32(std::max<uint64_t>(plan.total_slots, 1), 0);
             std::vector<uint32_t> o32;
@@ -478,12 +479,23 @@ struct mfs_sampler {
                         const uint64_t c = cl.col0 + st * cl.basis + j;
                         const uint32_t n = offs[c + 1] - offs[c];
                         const uint32_t *e = ents.data() + offs[c];
+                        uint32_t dets[2], nd = 0, obs_bits = 0;
+                        bool spill = false;
+                        for (uint32_t t = 0; t < n; t++) {
+                            if (e[t] >= prog.num_detectors) {
+                                obs_bits |= 1u << (30 + (e[t] - prog.num_detectors));
+                            } else if (nd < 2) {
+                                dets[nd++] = e[t];
+                            } else {
+                                spill = true;
+                            }
+                        }
                         uint32_t rec;
-                        if (n <= 2) {
-                            rec = (n << 30) | (n >= 1 ? e[0] : 0u) | (n == 2 ? e[1] << 15 : 0u);
+                        if (!spill) {
+                            rec = obs_bits | (nd << 28) | (nd >= 1 ? dets[0] : 0u) | (nd == 2 ? dets[1] << 14 : 0u);
                         } else {
-                            if (o32.size() >= (1u << 30)) throw std::invalid_argument("symptom table too large");
-                            rec = (3u << 30) | (uint32_t)o32.size();
+                            if (o32.size() >= (1u << 28)) throw std::invalid_argument("symptom table too large");
+                            rec = (3u << 28) | (uint32_t)o32.size();
                             o32.push_back(n);
                             o32.insert(o32.end(), e, e + n);
                         }
@@ -491,6 +503,7 @@ struct mfs_sampler {
                     }
                 }
             }
+            sym_ovf32 = !o32.empty();
             upload(d_recs32, r32, "compact symptom records");
             upload(d_ovf32, o32, "compact symptom overflow entries");
         } else {
@@ -566,6 +579,7 @@ struct mfs_sampler {
         P.ovf = d_ovf.as<uint32_t>();
         P.recs32 = sym_compact ? d_recs32.as<uint32_t>() : nullptr;
         P.ovf32 = sym_compact ? d_ovf32.as<uint32_t>() : nullptr;
+        P.ovf32_any = sym_ovf32 ? 1u : 0u;
         P.nclasses = (uint32_t)plan.classes.size();
         P.otab_words = (uint32_t)plan.otab.size();
         P.D = prog.num_detectors;

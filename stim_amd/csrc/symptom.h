@@ -80,11 +80,14 @@ SymPlan plan_symptoms(const Program &prog);
 // Column record (8 bytes): x bits 28..31 = entry count c; c <= 2: entries inline (x bits 0..27,
 // then y); c == 3: x bits 0..27 = offset into the overflow entries, y = entry count.
 constexpr uint32_t SYM_INLINE_MAX = 2;
-// Compact column record (4 bytes), used when every detector and observable index is below 2^15:
-// bits 30..31 = entry count c; c <= 2: entries in bits 0..14 and 15..29; c == 3: bits 0..29 = offset
-// of the entry count in the compact overflow list, followed by the entries. A class's records start at
-// a slot aligned to its stride, so one 4-, 8- or 16-byte load reads every basis record of a site.
-constexpr uint32_t SYM_COMPACT_LIMIT = 1u << 15;
+// Compact column record (4 bytes), used when there are at most 2^14 detectors and 2 observables:
+// bits 0..13 and 14..27 = up to two detector indices, bits 28..29 = their count c (0..2), bit 30 / 31 =
+// the fault flips observable 0 / 1. c == 3 marks a symptom with more than two detectors: bits 0..27
+// are then the offset of its entry count in the compact overflow list, followed by its entries
+// (detector m, observable k as D + k); the observable bits stay zero. Classes with at most 4 basis
+// columns take 4 slots per site, so one 16-byte load reads every basis record of a site.
+constexpr uint32_t SYM_COMPACT_DETECTORS = 1u << 14;
+constexpr uint32_t SYM_COMPACT_OBSERVABLES = 2;
 constexpr uint32_t SYM_OTAB_LDS_WORDS = 1024;  // outcome tables up to this size are read from LDS
 // LDS hit queue of the sampling kernel: 256 lanes x 8 hits x 8 bytes, plus a counter.
 constexpr size_t SYM_QUEUE_BYTES = 256 * 8 * 8 + 16;
@@ -98,6 +101,7 @@ struct SymDevParams {
     const uint32_t *ovf;           // overflow entries
     const uint32_t *recs32;        // compact records by slot (null when not available)
     const uint32_t *ovf32;         // compact overflow list
+    uint32_t ovf32_any;            // some compact record overflows (count 3)
     uint32_t nclasses;
     uint32_t otab_words;           // outcome-table words (copied to LDS when small)
     uint32_t D, O;

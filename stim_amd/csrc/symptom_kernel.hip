@@ -395,18 +395,27 @@ __device__ __forceinline__ void sink_hit(const SymDevParams &P, const Sink &k, u
     }
 }
 
-// Compact records (4 bytes; see symptom.h).
-__device__ __forceinline__ void sink_record32(const Sink &k, const uint32_t *ovf32, uint32_t rec, uint32_t dbase,
+// Compact records (4 bytes; see symptom.h): detector entries go straight to dbase + m; the observable
+// flags to observable bits obase + D + k (skipped when observables are not written).
+__device__ __forceinline__ void xor_bit(const Sink &k, uint32_t bit) {
+    atomicXor(&k.img[bit >> 5], 1u << (bit & 31));
+}
+
+__device__ __forceinline__ void sink_record32(const SymDevParams &P, const Sink &k, uint32_t rec, uint32_t dbase,
                                               uint32_t obase) {
-    const uint32_t c = rec >> 30;
-    if (c == 3) {
-        const uint32_t off = rec & 0x3FFFFFFFu;
-        const uint32_t n = ovf32[off];
-        for (uint32_t e = 0; e < n; e++) sink_bit(k, ovf32[off + 1 + e], dbase, obase);
+    const uint32_t c = (rec >> 28) & 3u;
+    if (P.ovf32_any && c == 3) {
+        const uint32_t off = rec & 0x0FFFFFFFu;
+        const uint32_t n = P.ovf32[off];
+        for (uint32_t e = 0; e < n; e++) sink_bit(k, P.ovf32[off + 1 + e], dbase, obase);
         return;
     }
-    if (c >= 1) sink_bit(k, rec & 0x7FFFu, dbase, obase);
-    if (c == 2) sink_bit(k, (rec >> 15) & 0x7FFFu, dbase, obase);
+    if (c >= 1) xor_bit(k, dbase + (rec & 0x3FFFu));
+    if (c == 2) xor_bit(k, dbase + ((rec >> 14) & 0x3FFFu));
+    if (k.obs_on && (rec >> 30)) {
+        if (rec & (1u << 30)) xor_bit(k, obase + k.D);
+        if (rec & (1u << 31)) xor_bit(k, obase + k.D + 1);
+    }
 }
 
 // Applies one hit with compact records: the site's basis records start at `slot`, `rstride` apart
@@ -418,15 +427,15 @@ __device__ __forceinline__ void sink_slot32(const SymDevParams &P, const Sink &k
     const uint32_t obase = k.obs_base + shot * k.obs_bits + k.obs_col0 - k.D;
     if (rstride == 4) {
         const uint4 v = *reinterpret_cast<const uint4 *>(P.recs32 + slot);
-        if (code & 1) sink_record32(k, P.ovf32, v.x, dbase, obase);
-        if (code & 2) sink_record32(k, P.ovf32, v.y, dbase, obase);
-        if (code & 4) sink_record32(k, P.ovf32, v.z, dbase, obase);
-        if (code & 8) sink_record32(k, P.ovf32, v.w, dbase, obase);
+        if (code & 1) sink_record32(P, k, v.x, dbase, obase);
+        if (code & 2) sink_record32(P, k, v.y, dbase, obase);
+        if (code & 4) sink_record32(P, k, v.z, dbase, obase);
+        if (code & 8) sink_record32(P, k, v.w, dbase, obase);
     } else {
         while (code) {
             const uint32_t j = __ffs(code) - 1;
             code &= code - 1;
-            sink_record32(k, P.ovf32, P.recs32[slot + j], dbase, obase);
+            sink_record32(P, k, P.recs32[slot + j], dbase, obase);
         }
     }
 }
