@@ -105,6 +105,18 @@ int mfs_engine_info(mfs_sampler *s, uint64_t out[8]);
  * prefix-scan kernel, with image size -kb. */
 int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb);
 
+/* ---- Detector error model sampler -------------------------------------------------------------
+ * Replaces stim.DetectorErrorModel.compile_sampler(seed=...) and CompiledDemSampler.sample / sample_write
+ * (src/stim/dem/detector_error_model.pybind.cc, src/stim/simulators/dem_sampler.pybind.cc:40-79, 119-309;
+ * the sampler itself src/stim/simulators/dem_sampler.inl:50-77). The model crosses as .dem text. The
+ * returned handle works with mfs_sample, mfs_sample_write, mfs_bench_device, mfs_circuit_stats
+ * (qubits, measurements and lookback are 0) and mfs_destroy. mfs_sample_with_faults replays recorded
+ * errors (recorded_errors_to_replay): a fault (e, 0, 1) fires mechanism e, counted over error
+ * instructions in REPEAT-unrolled order. Every mechanism fires independently with its own probability. */
+int mfs_dem_compile(const char *dem_text, size_t len, uint64_t seed, int has_seed, mfs_sampler **out);
+/* Parses on the host only: out[0..2] = detectors, observables, error mechanisms. */
+int mfs_dem_parse_stats(const char *dem_text, size_t len, uint64_t out[3]);
+
 /* ---- Batches: many sampling jobs in one persistent kernel ----------------------------------------
  * No reference counterpart (the reference samples one circuit per call). A batch holds jobs, each a
  * sampler, a shot count and device-resident bit-packed output rows (dets: shots rows of >= ceil(D/8)

@@ -6,5 +6,6 @@
 """
 
 from .sampler import Circuit, CompiledDetectorSampler  # noqa: F401
+from .dem import DetectorErrorModel, CompiledDemSampler  # noqa: F401
 
-__all__ = ['Circuit', 'CompiledDetectorSampler']
+__all__ = ['Circuit', 'CompiledDetectorSampler', 'DetectorErrorModel', 'CompiledDemSampler']

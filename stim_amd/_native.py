@@ -29,6 +29,7 @@ EXPORTED = [
     'mfs_batch_time', 'mfs_batch_autotune', 'mfs_batch_info', 'mfs_batch_set_schedule',
     'mfs_server_create', 'mfs_server_destroy', 'mfs_server_submit', 'mfs_server_step', 'mfs_server_poll',
     'mfs_server_result', 'mfs_server_copy_result', 'mfs_server_release', 'mfs_server_stats', 'mfs_server_replay',
+    'mfs_dem_compile', 'mfs_dem_parse_stats',
 ]
 
 _lib = None
@@ -90,6 +91,8 @@ def lib():
     L.mfs_server_stats.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_server_replay.argtypes = [p, u64, ctypes.POINTER(p), ctypes.POINTER(u64), ctypes.POINTER(u64),
                                     ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
+    L.mfs_dem_compile.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.POINTER(p)]
+    L.mfs_dem_parse_stats.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(u64)]
     L.mfs_last_error.argtypes = []
     L.mfs_last_error.restype = ctypes.c_char_p
     for name in EXPORTED:

@@ -158,6 +158,37 @@ int mfs_compile(const char *stim_text, size_t len, uint64_t seed, int has_seed, 
     });
 }
 
+int mfs_dem_compile(const char *dem_text, size_t len, uint64_t seed, int has_seed, mfs_sampler **out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out is null");
+        *out = nullptr;
+        const DemModel model = parse_dem(dem_text, dem_text ? len : 0);
+        auto s = std::make_unique<mfs_sampler>();
+        if (has_seed) {
+            s->key = splitmix64(seed ^ 0xDEADBEEF124Cull);
+        } else {
+            std::random_device rd;
+            s->key = ((uint64_t)rd() << 32) ^ rd();
+        }
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            throw HipError("no HIP device available: the stim_amd sampler runs only on the GPU");
+        }
+        check(hipStreamCreate(&s->stream), "stream");
+        s->init_dem(model);
+        *out = s.release();
+    });
+}
+
+int mfs_dem_parse_stats(const char *dem_text, size_t len, uint64_t out[3]) {
+    return guarded([&] {
+        const DemModel model = parse_dem(dem_text, dem_text ? len : 0);
+        out[0] = model.num_detectors;
+        out[1] = model.num_observables;
+        out[2] = model.errors.size();
+    });
+}
+
 void mfs_destroy(mfs_sampler *s) {
     delete s;
 }
@@ -300,6 +331,7 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
 
 int mfs_set_tile_words(mfs_sampler *s, int words) {
     int rc = guarded([&] {
+        if (s->dem_mode) throw std::invalid_argument("a detector error model sampler has no frame engine");
         if (words != 0 && words != 1 && words != 2 && words != 4) throw std::invalid_argument("tile words must be 0, 1, 2 or 4");
         s->pick_tile_words(words);
     });
@@ -310,6 +342,10 @@ int mfs_set_tile_words(mfs_sampler *s, int words) {
 int mfs_set_engine(mfs_sampler *s, int engine) {
     return guarded([&] {
         if (engine < 0 || engine > 2) throw std::invalid_argument("engine must be 0 (auto), 1 (frame) or 2 (symptom)");
+        if (s->dem_mode) {
+            if (engine == 1) throw std::invalid_argument("a detector error model sampler has no frame engine");
+            return;
+        }
         s->engine_mode = engine;
         if (engine == 2 && !s->use_symptoms(false)) throw std::invalid_argument("symptom engine unavailable");
     });
@@ -340,6 +376,8 @@ int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb) {
     return guarded([&] {
         if (kb == 0 || kb < -128 || kb > 128) throw std::invalid_argument("symptom tile size must be 1..128 KB");
         // A negative size selects the per-lane-walk queue kernel instead of the scan kernel.
+        if (kb < 0 && s->dem_mode)
+            throw std::invalid_argument("a detector error model sampler samples only with the scan kernel");
         s->sym_scan = kb > 0;
         s->sym_tile_kb = kb > 0 ? kb : -kb;
     });

@@ -23,6 +23,7 @@
 #include "circuit.h"
 #include "compiler.h"
 #include "program.h"
+#include "dem.h"
 #include "symptom.h"
 
 namespace sa {
@@ -131,7 +132,10 @@ struct mfs_sampler {
     SymPlan plan;
     uint64_t sym_entries = 0;
     double sym_build_ms = 0;
-    DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols, d_recs32, d_ovf32, d_groups;
+    DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols, d_recs32, d_ovf32, d_groups, d_thr;
+    // Detector-error-model samplers: no circuit; the symptom tables come from the model's mechanisms.
+    bool dem_mode = false;
+    std::vector<uint32_t> dem_col;   // mechanism index (model order) -> symptom column
     bool sym_compact = false;
     bool sym_ovf32 = false;      // some compact record overflows
     bool plan_counted = false;
@@ -532,6 +536,127 @@ struct mfs_sampler {
         return true;
     }
 
+    // Builds the symptom tables of a detector error model (reference DemSampler,
+    // src/stim/simulators/dem_sampler.inl:50-77: each mechanism fires independently with its
+    // probability and XORs its targets into the outputs). Mechanisms are ordered by probability and
+    // cut into groups whose probabilities lie within a factor of 2 of the group's largest; each group
+    // is drawn at that largest probability and thinned per mechanism (SYM_THIN).
+    void init_dem(const DemModel &m) {
+        dem_mode = true;
+        engine_mode = 2;
+        randomize = false;
+        if (m.num_detectors >= (1ull << 31) || m.num_observables >= (1ull << 31))
+            throw std::invalid_argument("detector error model too large");
+        prog.num_detectors = (uint32_t)m.num_detectors;
+        prog.num_observables = (uint32_t)m.num_observables;
+        const uint64_t E = m.errors.size();
+        if (E >= (1ull << 31)) throw std::invalid_argument("too many error mechanisms");
+        std::vector<uint32_t> order(E);
+        for (uint32_t e = 0; e < E; e++) order[e] = e;
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return m.errors[a].p > m.errors[b].p; });
+        plan = SymPlan{};
+        dem_col.assign(E, 0);
+        sym_compact = prog.num_detectors <= SYM_COMPACT_DETECTORS && prog.num_observables <= SYM_COMPACT_OBSERVABLES;
+        const uint32_t D = prog.num_detectors;
+        std::vector<uint2> recs(E);
+        std::vector<uint32_t> ovf, r32, o32, thr(std::max<uint64_t>(E, 1), 0xFFFFFFFFu);
+        // Outcome table shared by every class: one outcome, code 1.
+        plan.otab = {0xFFFFFFFFu, 1u};
+        uint64_t slot = 0;
+        for (uint64_t i = 0; i < E;) {
+            const double pmax = m.errors[order[i]].p;
+            uint64_t e = i;
+            while (e < E && (pmax == 0 ? m.errors[order[e]].p == 0 : m.errors[order[e]].p > pmax / 2)) e++;
+            SymClass sc{};
+            sc.p = (float)pmax;
+            sc.inv_log = (pmax > 0 && pmax < 1) ? (float)(1.0 / std::log2(1.0 - pmax)) : 0.0f;
+            sc.nsites = (uint32_t)(e - i);
+            sc.basis = 1;
+            sc.col0 = (uint32_t)i;
+            sc.woff = 0;
+            sc.nout = 1u | SYM_UNIFORM | SYM_SEQ;
+            sc.site0 = (uint32_t)i;
+            sc.rec0 = (uint32_t)slot;
+            sc.rstride = 4;
+            SymGroup g{};
+            g.p = sc.p;
+            g.inv_log = sc.inv_log;
+            g.site0 = sc.site0;
+            g.nsites = sc.nsites;
+            g.rec0 = sc.rec0;
+            g.rstride = 4;
+            g.nmem = 1;
+            g.c0 = (uint32_t)plan.classes.size();
+            for (int k = 0; k < 4; k++) g.bound[k] = 0xFFFFFFFFu;
+            g.bound[0] = 0;
+            g.nout[0] = sc.nout;
+            g.woff[0] = 0;
+            for (uint64_t j = i; j < e; j++) {
+                const DemError &err = m.errors[order[j]];
+                dem_col[order[j]] = (uint32_t)j;
+                plan.columns.push_back({order[j], 0, 1});
+                if (pmax > 0) {
+                    const double t = std::floor(err.p / pmax * 4294967296.0 + 0.5) - 1.0;
+                    thr[j] = t >= 4294967295.0 ? 0xFFFFFFFFu : t < 0 ? 0u : (uint32_t)t;
+                    if (thr[j] != 0xFFFFFFFFu) g.flags = SYM_THIN;
+                }
+            }
+            plan.classes.push_back(sc);
+            plan.groups.push_back(g);
+            slot += sc.nsites * 4ull;
+            i = e;
+        }
+        plan.total_sites = E;
+        plan.total_slots = slot;
+        // Records: 8-byte per column (entries: detectors, then observables as D + k), compact by slot.
+        if (sym_compact) r32.assign(std::max<uint64_t>(slot, 1), 0);
+        uint32_t c = 0;   // class of mechanism j (classes cover increasing, contiguous site ranges)
+        for (uint64_t j = 0; j < E; j++) {
+            const DemError &err = m.errors[plan.columns[j].op];
+            std::vector<uint32_t> ent(err.dets.begin(), err.dets.end());
+            for (uint32_t k : err.obs) ent.push_back(D + k);
+            const uint32_t n = (uint32_t)ent.size();
+            if (n <= SYM_INLINE_MAX) {
+                recs[j] = make_uint2((n << 28) | (n >= 1 ? ent[0] : 0u), n == 2 ? ent[1] : 0u);
+            } else {
+                if (ovf.size() >= (1u << 28)) throw std::invalid_argument("detector error model too large");
+                recs[j] = make_uint2((3u << 28) | (uint32_t)ovf.size(), n);
+                ovf.insert(ovf.end(), ent.begin(), ent.end());
+            }
+            if (sym_compact) {
+                uint32_t rec;
+                if (err.dets.size() <= 2) {
+                    rec = ((uint32_t)err.dets.size() << 28) | (err.dets.size() >= 1 ? err.dets[0] : 0u) |
+                          (err.dets.size() == 2 ? err.dets[1] << 14 : 0u);
+                    for (uint32_t k : err.obs) rec |= 1u << (30 + k);
+                } else {
+                    rec = (3u << 28) | (uint32_t)o32.size();
+                    o32.push_back(n);
+                    o32.insert(o32.end(), ent.begin(), ent.end());
+                }
+                // Mechanism j is site j: its class's records start at rec0 with 4 slots per site.
+                while (c + 1 < plan.classes.size() && plan.classes[c + 1].site0 <= j) c++;
+                const uint64_t at = plan.classes[c].rec0 + (j - plan.classes[c].site0) * 4ull;
+                r32[at] = rec;
+                r32[at + 1] = thr[j];
+            }
+        }
+        sym_entries = 0;
+        for (const DemError &err : m.errors) sym_entries += err.dets.size() + err.obs.size();
+        upload(d_classes, plan.classes, "classes");
+        upload(d_sotab, plan.otab, "symptom outcome table");
+        upload(d_recs, recs, "symptom records");
+        upload(d_ovf, ovf, "symptom overflow entries");
+        upload(d_groups, plan.groups, "symptom groups");
+        upload(d_thr, thr, "thinning thresholds");
+        if (sym_compact) {
+            sym_ovf32 = !o32.empty();
+            upload(d_recs32, r32, "compact symptom records");
+            upload(d_ovf32, o32, "compact symptom overflow entries");
+        }
+        sym_state = 1;
+    }
+
     // Whether a call of `shots` shots runs on the symptom engine. In automatic mode the symptom table
     // is built only when it is small next to the work it saves: its build simulates one shot per basis
     // fault, so a call of fewer than half as many shots as there are columns samples on the frame
@@ -580,6 +705,7 @@ struct mfs_sampler {
         P.recs32 = sym_compact ? d_recs32.as<uint32_t>() : nullptr;
         P.ovf32 = sym_compact ? d_ovf32.as<uint32_t>() : nullptr;
         P.ovf32_any = sym_ovf32 ? 1u : 0u;
+        P.thr = d_thr.as<uint32_t>();
         P.nclasses = (uint32_t)plan.classes.size();
         P.otab_words = (uint32_t)plan.otab.size();
         P.D = prog.num_detectors;
@@ -619,7 +745,15 @@ struct mfs_sampler {
             while ((1u << lg) < v) lg++;
             return lg;
         };
-        bool scan_ok = scan && (plan.total_sites << log2_of(S)) < (1ull << 31);
+        // The scan kernel addresses (site, shot) positions of one group in 32 bits.
+        uint64_t group_sites = 0;
+        for (const SymGroup &g : plan.groups) group_sites = std::max<uint64_t>(group_sites, g.nsites);
+        if (dem_mode && scan) {
+            // Thinned buckets are sampled only by the scan kernel: narrow the tile until it fits.
+            while (S > 1 && (group_sites << log2_of(S)) >= (1ull << 31)) S /= 2;
+        }
+        bool scan_ok = scan && (group_sites << log2_of(S)) < (1ull << 31);
+        if (dem_mode && scan && !scan_ok) throw std::invalid_argument("detector error model too large to sample");
         if (!scan_ok) S = std::min<uint32_t>(S, 1024);
         SymLaunch r{};
         SymDevParams &P = r.P;
@@ -737,6 +871,15 @@ struct mfs_sampler {
         std::vector<uint32_t> cols;
         for (uint64_t s = 0; s < n; s++) {
             for (uint64_t i = off[s0 + s]; i < off[s0 + s + 1]; i++) {
+                if (dem_mode) {
+                    // (mechanism index, 0, 1): the mechanism fires in this shot.
+                    const uint32_t e = faults[3 * i];
+                    if (e >= dem_col.size() || faults[3 * i + 1] != 0 || faults[3 * i + 2] > 1) {
+                        throw std::out_of_range("error mechanism out of range");
+                    }
+                    if (faults[3 * i + 2]) cols.push_back(dem_col[e]);
+                    continue;
+                }
                 uint32_t op, site;
                 resolve_fault(prog, faults[3 * i], faults[3 * i + 1], &op, &site);
                 const uint32_t c = plan.op_class[op];

@@ -230,6 +230,7 @@ SymPlan plan_symptoms(const Program &prog) {
         ng.bound[0] = 0;
         ng.nout[0] = sc.nout;
         ng.woff[0] = sc.woff;
+        ng.flags = 0;
         plan.groups.push_back(ng);
     }
     if (plan.columns.size() >= (1ull << 31) || plan.total_slots >= (1ull << 31)) {

@@ -53,8 +53,15 @@ struct SymGroup {
     uint32_t bound[4];   // unused members: 0xFFFFFFFF
     uint32_t nout[4];
     uint32_t woff[4];
+    uint32_t flags;      // SYM_THIN: sites fire with their own probability <= p (see below)
+    uint32_t pad[3];
 };
-static_assert(sizeof(SymGroup) == 80, "SymGroup layout");
+static_assert(sizeof(SymGroup) == 96, "SymGroup layout");
+// Thinned group (detector error models, whose mechanisms have many distinct probabilities): the group
+// is drawn at its largest probability p and a hit at a site with probability p_i is kept when the
+// hit's 32-bit random word is at most the site's threshold round(p_i / p * 2^32) - 1. The threshold
+// sits in slot 1 of the site's compact records (thr[] for 8-byte records).
+constexpr uint32_t SYM_THIN = 1u;
 
 // Basis fault of one column, as a fixed-fault entry for the frame kernel.
 struct SymColumn {
@@ -102,6 +109,7 @@ struct SymDevParams {
     const uint32_t *recs32;        // compact records by slot (null when not available)
     const uint32_t *ovf32;         // compact overflow list
     uint32_t ovf32_any;            // some compact record overflows (count 3)
+    const uint32_t *thr;           // per-site thresholds of thinned groups (8-byte-record path)
     uint32_t nclasses;
     uint32_t otab_words;           // outcome-table words (copied to LDS when small)
     uint32_t D, O;

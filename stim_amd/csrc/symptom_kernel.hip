@@ -421,12 +421,14 @@ __device__ __forceinline__ void sink_record32(const SymDevParams &P, const Sink 
 // Applies one hit with compact records: the site's basis records start at `slot`, `rstride` apart
 // sites; a single 16-byte load fetches all of them when rstride is 4 (classes with more than 4 basis
 // columns load the records they use one by one).
+// thin: the group is thinned (the site's threshold is slot 1; the hit is dropped when r exceeds it).
 __device__ __forceinline__ void sink_slot32(const SymDevParams &P, const Sink &k, uint32_t slot, uint32_t rstride,
-                                            uint32_t code, uint32_t shot) {
+                                            uint32_t code, uint32_t shot, bool thin, uint32_t r) {
     const uint32_t dbase = shot * k.det_bits + k.det_col0;
     const uint32_t obase = k.obs_base + shot * k.obs_bits + k.obs_col0 - k.D;
     if (rstride == 4) {
         const uint4 v = *reinterpret_cast<const uint4 *>(P.recs32 + slot);
+        if (thin && r > v.y) return;
         if (code & 1) sink_record32(P, k, v.x, dbase, obase);
         if (code & 2) sink_record32(P, k, v.y, dbase, obase);
         if (code & 4) sink_record32(P, k, v.z, dbase, obase);
@@ -519,9 +521,10 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
                                 mk == 0 ? G.woff[0] : mk == 1 ? G.woff[1] : mk == 2 ? G.woff[2] : G.woff[3];
                             code = outcome(otab, wo, nf, oc[k]);
                         }
+                        const bool thin = (G.flags & SYM_THIN) != 0;
                         if (P.recs32) {
-                            sink_slot32(P, sink, G.rec0 + srel * G.rstride, G.rstride, code, shot);
-                        } else {
+                            sink_slot32(P, sink, G.rec0 + srel * G.rstride, G.rstride, code, shot, thin, oc[k]);
+                        } else if (!thin || oc[k] <= P.thr[G.site0 + srel]) {
                             const SymClass cl = P.classes[G.c0];
                             sink_hit(P, sink, cl.col0 + srel * cl.basis, code, shot, cl.basis);
                         }

@@ -210,9 +210,57 @@ def write_gen_pins():
     json.dump(pins, open(os.path.join(HERE, 'gen_pins.json'), 'w'), indent=0)
 
 
+def write_dem_fixtures():
+    """Detector error models of fixture circuits (`stim analyze_errors`, unfolded and with REPEAT
+    blocks folded) and replay vectors: random error masks fed to `stim sample_dem --replay_err_in`."""
+    ddir = os.path.join(HERE, 'dems')
+    os.makedirs(ddir, exist_ok=True)
+    rng = random.Random(20261203)
+    cases = []
+    for name in ['surface_d3_r3', 'surface_d5_r5', 'rep_d7_r5', 'color_d5_r3']:
+        path = os.path.join(HERE, 'circuits', name + '.stim')
+        for folded in (False, True):
+            args = ['analyze_errors', '--in', path, '--approximate_disjoint_errors']
+            if folded:
+                args.append('--fold_loops')
+            text = run(args)
+            dname = name + ('_folded' if folded else '')
+            with open(os.path.join(ddir, dname + '.dem'), 'w') as f:
+                f.write(text)
+            # Replay: 40 shots, each with 0..6 random mechanisms (masks in 01 format).
+            ne = None
+            with tempfile.TemporaryDirectory() as td:
+                model = os.path.join(ddir, dname + '.dem')
+                # Count mechanisms via an empty replay of zero errors per shot is not possible without
+                # knowing E; sample once with --err_out to learn the row width.
+                run(['sample_dem', '--shots', '1', '--in', model, '--out', os.path.join(td, 'd'),
+                     '--obs_out', os.path.join(td, 'o'), '--err_out', os.path.join(td, 'e'), '--seed', '1'])
+                ne = len(open(os.path.join(td, 'e')).read().strip())
+                masks = []
+                for _ in range(40):
+                    k = rng.randint(0, 6)
+                    masks.append(sorted(rng.sample(range(ne), k)))
+                with open(os.path.join(td, 'err.01'), 'w') as f:
+                    for m in masks:
+                        row = ['0'] * ne
+                        for e in m:
+                            row[e] = '1'
+                        f.write(''.join(row) + '\n')
+                run(['sample_dem', '--shots', '40', '--in', model, '--out', os.path.join(td, 'd.01'),
+                     '--obs_out', os.path.join(td, 'o.01'), '--replay_err_in', os.path.join(td, 'err.01'),
+                     '--replay_err_in_format', '01'])
+                dets = open(os.path.join(td, 'd.01')).read().split()
+                obs = open(os.path.join(td, 'o.01')).read().split()
+            cases.append({'dem': dname, 'num_errors': ne, 'errors': masks, 'dets': dets, 'obs': obs})
+    json.dump(cases, open(os.path.join(HERE, 'dem_replay.json'), 'w'))
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == '--gen-pins-only':
         write_gen_pins()
+        return
+    if len(sys.argv) > 1 and sys.argv[1] == '--dem-only':
+        write_dem_fixtures()
         return
     if not os.path.isdir('/root/reference') or not os.path.exists(STIM):
         sys.exit('make_fixtures.py needs /root/reference and oracle/_ref/stim (make -C oracle ref)')
@@ -235,6 +283,7 @@ def main():
             f.write(text)
 
     write_gen_pins()
+    write_dem_fixtures()
 
     # Fixed-fault vectors from the reference sampler.
     fixtures = []
