@@ -9,6 +9,7 @@ separate observables) into HBM-resident output rows.
     python bench.py --workload batch64      # 64 heterogeneous surface codes: one persistent kernel vs 64 calls
     python bench.py --workload serve        # request loop with continuous batching, p50/p99 latency
     python bench.py --workload clifford     # 10,000-qubit, 1,000-layer random-Clifford circuit, 2^16 shots
+    python bench.py --workload dem          # detector-error-model sampler, the d=25 r=25 model, 2^20 shots
 """
 
 import argparse
@@ -28,6 +29,7 @@ HBM_PEAK_GBS = 8000.0
 CIRCUIT = os.path.join(ROOT, 'tests', 'golden', 'circuits', 'surface_d25_r25.stim')
 REF_STIM = os.path.join(ROOT, 'oracle', '_ref', 'stim')
 TRAFFIC = os.path.join(ROOT, 'profiles', 'sym_scan_kernel_d25_r25_2p20_traffic.md')
+DEM_D25 = os.path.join(ROOT, 'tests', 'golden', 'dems', 'surface_d25_r25_folded.dem.gz')
 
 
 def cpu_baseline(circuit_path):
@@ -315,13 +317,105 @@ def clifford(args):
     }
 
 
+def dem_cpu_baseline(text):
+    """The reference CLI `stim sample_dem`, one process per core, on a bounded sample of the same model."""
+    import tempfile
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    if not (os.path.exists(REF_STIM) and os.access(REF_STIM, os.X_OK)):
+        return None
+    shots = 1 << 18
+    with tempfile.NamedTemporaryFile('w', suffix='.dem', delete=False) as f:
+        f.write(text)
+        path = f.name
+    try:
+        procs = []
+        t0 = time.perf_counter()
+        for k in range(cores):
+            procs.append(subprocess.Popen([REF_STIM, 'sample_dem', '--shots', str(shots), '--in', path,
+                                           '--out_format', 'b8', '--out', '/dev/null', '--obs_out', '/dev/null',
+                                           '--obs_out_format', 'b8', '--seed', str(k)],
+                                          stdout=subprocess.DEVNULL, stderr=subprocess.PIPE))
+        ok = all(p.wait() == 0 for p in procs)
+        wall = time.perf_counter() - t0
+    finally:
+        os.unlink(path)
+    if not ok:
+        return None
+    return {'value': cores * shots / wall, 'unit': 'shots/s', 'cores': cores, 'kind': 'reference',
+            'sample': f'stim sample_dem b8 to /dev/null, {cores} processes x {shots} shots '
+                      f'(d=25 r=25 p=1e-3 model), wall {wall:.2f}s'}
+
+
+def dem(args):
+    """Detector-error-model sampler on the d=25, 25-round surface code's model with folded loops
+    (356,321 mechanisms once unrolled)."""
+    import gzip
+    import torch
+    import stim_amd
+    from stim_amd import _native as N
+    with gzip.open(DEM_D25, 'rt') as f:
+        text = f.read()
+    t0 = time.perf_counter()
+    sampler = stim_amd.DetectorErrorModel(text).compile_sampler(seed=2026)
+    compile_s = time.perf_counter() - t0
+    D, O = sampler.num_detectors, sampler.num_observables
+    S = args.shots
+    dets, obs, drow, orow = device_buffers(torch, S, D, O)
+    flags = N.MFS_BIT_PACKED | N.MFS_SEPARATE_OBSERVABLES | N.MFS_DEVICE_OUTPUT
+    L = N.lib()
+
+    def step():
+        N.check(L.mfs_sample(sampler._h, S, flags, dets.data_ptr(), dets.stride(0), obs.data_ptr(), obs.stride(0)))
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = sampler.bench_device(S, warmup=5, iters=max(10, min(args.steps, 50)))
+    engine = sampler.engine_info()
+    b_alg = S * (drow + orow)
+    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    return {
+        'metric': 'DEM sampler detector shots/s @ d=25 r=25 p=1e-3 model',
+        'value': args.steps * S / wall,
+        'unit': 'shots/s',
+        'n_gpus': 1,
+        'steps': args.steps,
+        'warmup': args.warmup,
+        'ms_per_step': wall / args.steps * 1e3,
+        'higher_is_better': True,
+        'scaling': 'weak',
+        'vs_baseline': None,
+        'dtype': 'u32',
+        'data': 'the reference analyze_errors model (folded loops) of the d=25 r=25 p=1e-3 rotated memory-Z '
+                'circuit, seeded Philox draws',
+        'config': {'workload': 'dem_d25_r25', 'mechanisms': sampler.num_errors, 'detectors': D, 'observables': O,
+                   'shots_per_step': S, 'tile_shots': engine['tile_shots'],
+                   'lds_bytes_per_tile': engine['lds_bytes_per_tile'], 'probability_buckets': engine['classes'],
+                   'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'kernel': 'sym_scan_kernel',
+                     'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
+        'compile_s': compile_s,
+        'cpu_baseline': None if args.no_cpu else dem_cpu_baseline(text),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=50)
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--shots', type=int, default=1 << 20)
-    ap.add_argument('--workload', choices=['surface_d25', 'batch64', 'serve', 'clifford'], default='surface_d25')
+    ap.add_argument('--workload', choices=['surface_d25', 'batch64', 'serve', 'clifford', 'dem'], default='surface_d25')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--tile-words', type=int, default=0)
     ap.add_argument('--calibrate-copy', action='store_true',
@@ -340,6 +434,8 @@ def main():
         line = batch64(args)
     elif args.workload == 'clifford':
         line = clifford(args)
+    elif args.workload == 'dem':
+        line = dem(args)
     else:
         from stim_amd import serve
         line = serve.bench(qps=args.qps, requests=args.requests)

@@ -539,8 +539,9 @@ struct mfs_sampler {
     // Builds the symptom tables of a detector error model (reference DemSampler,
     // src/stim/simulators/dem_sampler.inl:50-77: each mechanism fires independently with its
     // probability and XORs its targets into the outputs). Mechanisms are ordered by probability and
-    // cut into groups whose probabilities lie within a factor of 2 of the group's largest; each group
-    // is drawn at that largest probability and thinned per mechanism (SYM_THIN).
+    // cut into groups whose probabilities lie within a factor DEM_BUCKET_RATIO of the group's largest;
+    // each group is drawn at that largest probability and thinned per mechanism (SYM_THIN).
+    static constexpr double DEM_BUCKET_RATIO = 0.5;
     void init_dem(const DemModel &m) {
         dem_mode = true;
         engine_mode = 2;
@@ -566,7 +567,8 @@ struct mfs_sampler {
         for (uint64_t i = 0; i < E;) {
             const double pmax = m.errors[order[i]].p;
             uint64_t e = i;
-            while (e < E && (pmax == 0 ? m.errors[order[e]].p == 0 : m.errors[order[e]].p > pmax / 2)) e++;
+            auto joins = [&](double p) { return p == pmax || (pmax > 0 && p > pmax * DEM_BUCKET_RATIO); };
+            while (e < E && joins(m.errors[order[e]].p)) e++;
             SymClass sc{};
             sc.p = (float)pmax;
             sc.inv_log = (pmax > 0 && pmax < 1) ? (float)(1.0 / std::log2(1.0 - pmax)) : 0.0f;
@@ -591,6 +593,7 @@ struct mfs_sampler {
             g.bound[0] = 0;
             g.nout[0] = sc.nout;
             g.woff[0] = 0;
+            if (sym_compact) g.flags = SYM_DEM_SLOTS;
             for (uint64_t j = i; j < e; j++) {
                 const DemError &err = m.errors[order[j]];
                 dem_col[order[j]] = (uint32_t)j;
@@ -598,7 +601,7 @@ struct mfs_sampler {
                 if (pmax > 0) {
                     const double t = std::floor(err.p / pmax * 4294967296.0 + 0.5) - 1.0;
                     thr[j] = t >= 4294967295.0 ? 0xFFFFFFFFu : t < 0 ? 0u : (uint32_t)t;
-                    if (thr[j] != 0xFFFFFFFFu) g.flags = SYM_THIN;
+                    if (thr[j] != 0xFFFFFFFFu) g.flags |= SYM_THIN;
                 }
             }
             plan.classes.push_back(sc);
@@ -624,10 +627,17 @@ struct mfs_sampler {
                 ovf.insert(ovf.end(), ent.begin(), ent.end());
             }
             if (sym_compact) {
-                uint32_t rec;
-                if (err.dets.size() <= 2) {
-                    rec = ((uint32_t)err.dets.size() << 28) | (err.dets.size() >= 1 ? err.dets[0] : 0u) |
-                          (err.dets.size() == 2 ? err.dets[1] << 14 : 0u);
+                // Up to six detectors inline, two per record in slots 0, 2 and 3 (SYM_DEM_SLOTS).
+                const size_t nd = err.dets.size();
+                auto pair = [&](size_t a) -> uint32_t {
+                    const size_t c = nd > a ? std::min<size_t>(2, nd - a) : 0;
+                    return ((uint32_t)c << 28) | (c >= 1 ? err.dets[a] : 0u) | (c == 2 ? err.dets[a + 1] << 14 : 0u);
+                };
+                uint32_t rec, rec2 = 0, rec3 = 0;
+                if (nd <= 6) {
+                    rec = pair(0);
+                    rec2 = pair(2);
+                    rec3 = pair(4);
                     for (uint32_t k : err.obs) rec |= 1u << (30 + k);
                 } else {
                     rec = (3u << 28) | (uint32_t)o32.size();
@@ -639,6 +649,8 @@ struct mfs_sampler {
                 const uint64_t at = plan.classes[c].rec0 + (j - plan.classes[c].site0) * 4ull;
                 r32[at] = rec;
                 r32[at + 1] = thr[j];
+                r32[at + 2] = rec2;
+                r32[at + 3] = rec3;
             }
         }
         sym_entries = 0;

@@ -53,7 +53,7 @@ struct SymGroup {
     uint32_t bound[4];   // unused members: 0xFFFFFFFF
     uint32_t nout[4];
     uint32_t woff[4];
-    uint32_t flags;      // SYM_THIN: sites fire with their own probability <= p (see below)
+    uint32_t flags;      // SYM_THIN, SYM_DEM_SLOTS (see below)
     uint32_t pad[3];
 };
 static_assert(sizeof(SymGroup) == 96, "SymGroup layout");
@@ -62,6 +62,11 @@ static_assert(sizeof(SymGroup) == 96, "SymGroup layout");
 // hit's 32-bit random word is at most the site's threshold round(p_i / p * 2^32) - 1. The threshold
 // sits in slot 1 of the site's compact records (thr[] for 8-byte records).
 constexpr uint32_t SYM_THIN = 1u;
+// Detector-error-model sites with compact records: a mechanism keeps up to six detectors inline, two
+// per record in slots 0, 2 and 3 (slot 0 also carries the observable flags; unused records have count
+// 0), and every hit applies those three records.
+constexpr uint32_t SYM_DEM_SLOTS = 2u;
+constexpr uint32_t SYM_DEM_SLOT_CODE = 0xDu;   // slots 0, 2 and 3
 
 // Basis fault of one column, as a fixed-fault entry for the frame kernel.
 struct SymColumn {

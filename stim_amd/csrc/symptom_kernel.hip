@@ -521,6 +521,7 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
                                 mk == 0 ? G.woff[0] : mk == 1 ? G.woff[1] : mk == 2 ? G.woff[2] : G.woff[3];
                             code = outcome(otab, wo, nf, oc[k]);
                         }
+                        if (G.flags & SYM_DEM_SLOTS) code = SYM_DEM_SLOT_CODE;
                         const bool thin = (G.flags & SYM_THIN) != 0;
                         if (P.recs32) {
                             sink_slot32(P, sink, G.rec0 + srel * G.rstride, G.rstride, code, shot, thin, oc[k]);

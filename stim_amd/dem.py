@@ -13,6 +13,7 @@ import os
 import numpy as np
 
 from . import _native as N
+from .sampler import CompiledDetectorSampler as _CDS
 
 
 def _text_bytes(text):
@@ -62,6 +63,10 @@ class CompiledDemSampler:
         if h is not None and h.value:
             N.lib().mfs_destroy(h)
             self._h = None
+
+    # Device-side introspection and timing, shared with the circuit sampler (they read only the handle).
+    engine_info = _CDS.engine_info
+    bench_device = _CDS.bench_device
 
     def _replay_faults(self, errs, shots):
         """(shot offsets, fault triples) for a recorded-error array: bool (shots, E) or bit-packed uint8

@@ -11,6 +11,7 @@ for them, per-detector rates derived from `stim analyze_errors`, seeded firing c
     make -C oracle -j8 ref && python tests/golden/make_fixtures.py
 """
 
+import gzip
 import json
 import os
 import random
@@ -253,6 +254,11 @@ def write_dem_fixtures():
                 obs = open(os.path.join(td, 'o.01')).read().split()
             cases.append({'dem': dname, 'num_errors': ne, 'errors': masks, 'dets': dets, 'obs': obs})
     json.dump(cases, open(os.path.join(HERE, 'dem_replay.json'), 'w'))
+    # The d=25, 25-round surface code's model with folded loops: the full-size DEM workload.
+    text = run(['analyze_errors', '--in', os.path.join(HERE, 'circuits', 'surface_d25_r25.stim'),
+                '--approximate_disjoint_errors', '--fold_loops'])
+    with gzip.open(os.path.join(ddir, 'surface_d25_r25_folded.dem.gz'), 'wt', compresslevel=9) as f:
+        f.write(text)
 
 
 def main():

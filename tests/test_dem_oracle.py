@@ -99,7 +99,7 @@ def _stats(text):
 
 
 def test_library_parse_counts_match_oracle(golden):
-    names = sorted(f[:-4] for f in os.listdir(os.path.join(golden.dir, 'dems')))
+    names = sorted(f[:-4] for f in os.listdir(os.path.join(golden.dir, 'dems')) if f.endswith('.dem'))
     assert len(names) == 8
     for name in names:
         text = _dem(golden, name)
@@ -122,3 +122,14 @@ def test_library_rejects_malformed_models(bad):
     rc, _ = _stats(bad)
     assert rc == N.MFS_ERR_INVALID_ARGUMENT, bad
     assert b'detector error model line' in lib.mfs_last_error()
+
+
+def test_library_parse_counts_match_oracle_d25(golden):
+    import gzip
+    with gzip.open(os.path.join(golden.dir, 'dems', 'surface_d25_r25_folded.dem.gz'), 'rt') as f:
+        text = f.read()
+    assert 'repeat' in text
+    nd, no, errors = do.parse(text)
+    assert (nd, no, len(errors)) == (15600, 1, 356321)
+    rc, out = _stats(text)
+    assert rc == 0 and out == [nd, no, len(errors)]

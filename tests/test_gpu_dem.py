@@ -233,3 +233,35 @@ def test_sample_write_matches_sample(golden, tmp_path):
                    replay_err_in_file=str(tmp_path / 'e.b8'), replay_err_in_format='b8')
     assert open(tmp_path / 'rd.01').read().split() == c['dets']
     assert open(tmp_path / 'ro.01').read().split() == c['obs']
+
+
+def _d25_text(golden):
+    import gzip
+    with gzip.open(os.path.join(golden.dir, 'dems', 'surface_d25_r25_folded.dem.gz'), 'rt') as f:
+        return f.read()
+
+
+def test_d25_model_full_size_rates(golden):
+    """The d=25, 25-round model with folded loops (356,321 mechanisms once unrolled, 15,600 detectors)
+    at 2^19 shots: per-detector rates against the exact rates, counted from the packed rows a bit plane
+    at a time."""
+    text = _d25_text(golden)
+    m = do.parse(text)
+    s = _sampler(text, seed=31)
+    assert (s.num_detectors, s.num_observables, s.num_errors) == (15600, 1, 356321)
+    n = m[0] + m[1]
+    counts = np.zeros(((n + 7) // 8) * 8, dtype=np.int64)
+    shots = 0
+    for _ in range(2):
+        d, o, _ = s.sample(1 << 18, bit_packed=True)
+        for b in range(8):
+            counts[b:m[0]:8] += ((d >> b) & 1).sum(axis=0, dtype=np.int64)[: len(range(b, m[0], 8))]
+        counts[m[0]] += int((o[:, 0] & 1).sum())
+        shots += len(d)
+    assert not (d[:, -1] >> (m[0] % 8 or 8)).any()   # padding bits of the last byte stay zero
+    _check_rates_counts(counts[:n], do.exact_rates(m), shots)
+
+
+def _check_rates_counts(counts, rates, shots):
+    z = _z(counts, shots, rates)
+    assert np.max(np.abs(z)) < _zcrit(len(z)), np.argsort(-np.abs(z))[:5]
