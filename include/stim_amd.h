@@ -58,6 +58,12 @@ int mfs_circuit_stats(const mfs_sampler *s, uint64_t out[5]);
  * Reference: Circuit.compute_stats / num_detectors etc. (src/stim/circuit/circuit_instruction.cc:44-100). */
 int mfs_parse_stats(const char *stim_text, size_t len, uint64_t out[10]);
 
+/* Reference sample of a circuit (host only, no GPU): the noiseless run with every random measurement
+ * result taken as 0, written bit-packed little-endian into out[0 .. ceil(num_measurements / 8)).
+ * Reference: Circuit.reference_sample (src/stim/circuit/circuit.pybind.cc:664-671) and
+ * TableauSimulator::reference_sample_circuit (src/stim/simulators/tableau_simulator.inl:1435-1438). */
+int mfs_reference_sample(const char *stim_text, size_t len, uint8_t *out, size_t out_len);
+
 /* Samples detection events. Row strides are in bytes. dets must hold shots rows of
  * (packed ? ceil(n/8) : n) bytes where n = D + O*(prepend+append); obs (when
  * MFS_SEPARATE_OBSERVABLES) holds shots rows of (packed ? ceil(O/8) : O) bytes.

@@ -233,6 +233,8 @@ void validate(const Instr &in, const Cursor &cur) {
                     cur.fail(std::string(gi.name) + " only takes Pauli targets");
                 if (in.gate == Gate::MPP && (t.kind == TK::REC || t.kind == TK::SWEEP))
                     cur.fail("MPP cannot target measurement records");
+                if ((in.gate == Gate::SPP || in.gate == Gate::SPP_DAG) && (t.kind == TK::REC || t.kind == TK::SWEEP))
+                    cur.fail(std::string(gi.name) + " only takes Pauli targets ('X2', 'Y3', 'Z5', etc).");
                 break;
             case Shape::RECORDS:
                 if (t.kind != TK::REC) cur.fail("DETECTOR only takes measurement record targets like rec[-1]");

@@ -1,6 +1,7 @@
 // C-ABI entry points of the stim_amd sampler: compile, sample, sample_write, fixed-fault sampling,
 // and the result-format writers.
 #include "sampler_impl.h"
+#include "tableau.h"
 
 namespace {
 
@@ -209,6 +210,21 @@ int mfs_parse_stats(const char *stim_text, size_t len, uint64_t out[10]) {
         out[7] = prog.num_noise_ops;
         out[8] = prog.ring;
         out[9] = lds_bytes(prog, 2);
+    });
+}
+
+int mfs_reference_sample(const char *stim_text, size_t len, uint8_t *out, size_t out_len) {
+    return guarded([&] {
+        std::string text(stim_text ? stim_text : "", stim_text ? len : 0);
+        Block b = parse_circuit(text.data(), text.size());
+        const std::vector<uint8_t> ref = reference_sample(flatten(b));
+        const size_t need = (ref.size() + 7) / 8;
+        if (need && !out) throw std::invalid_argument("out is null");
+        if (out_len < need) throw std::invalid_argument("output buffer too small for the reference sample");
+        std::memset(out, 0, need);
+        for (size_t m = 0; m < ref.size(); m++) {
+            if (ref[m]) out[m >> 3] |= (uint8_t)(1u << (m & 7));
+        }
     });
 }
 

@@ -82,6 +82,18 @@ class Circuit:
     def compile_detector_sampler(self, *, seed=None) -> 'CompiledDetectorSampler':
         return CompiledDetectorSampler(self, seed=seed)
 
+    def reference_sample(self, *, bit_packed: bool = False) -> np.ndarray:
+        """The noiseless run with every random measurement result taken as 0 (host tableau).
+
+        Reference: stim.Circuit.reference_sample (src/stim/circuit/circuit.pybind.cc:664-671)."""
+        n = self.num_measurements
+        packed = np.zeros((n + 7) // 8, dtype=np.uint8)
+        t = _text_bytes(self._text)
+        N.check(N.lib().mfs_reference_sample(t, len(t), packed.ctypes.data if packed.size else None, packed.size))
+        if bit_packed:
+            return packed
+        return np.unpackbits(packed, bitorder='little')[:n].astype(np.bool_)
+
 
 def _check_out_buffer(buf, shape, bit_packed):
     """Validates a caller-provided output buffer (numpy.pybind.cc:28-44, 72-88)."""

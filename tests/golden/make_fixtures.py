@@ -261,7 +261,51 @@ def write_dem_fixtures():
         f.write(text)
 
 
+def ref_sample_cli(text):
+    """The reference CLI's reference sample of a circuit: `stim m2d` converts an all-zero measurement
+    shot of a copy of the circuit whose detectors are its single measurements; each detector is then
+    the measurement's reference bit (measurements_to_detection_events.inl:75-88)."""
+    body = [l for l in text.split('\n')
+            if not l.strip().startswith('DETECTOR') and not l.strip().startswith('OBSERVABLE_INCLUDE')]
+    m = st.stats(st.parse_flat(text))['num_measurements']
+    if m == 0:
+        return ''
+    body += ['DETECTOR rec[-%d]' % (m - j) for j in range(m)]
+    with tempfile.TemporaryDirectory() as td:
+        cpath = os.path.join(td, 'c.stim')
+        ipath = os.path.join(td, 'm.01')
+        with open(cpath, 'w') as f:
+            f.write('\n'.join(body) + '\n')
+        with open(ipath, 'w') as f:
+            f.write('0' * m + '\n')
+        out = run(['m2d', '--circuit', cpath, '--in', ipath, '--in_format', '01', '--out_format', '01'], stdin='')
+    line = out.strip()
+    assert len(line) == m, (len(line), m)
+    return line
+
+
+def write_ref_samples():
+    """Reference samples (tableau, random results taken as 0) from the reference CLI for the fixture
+    circuits, gate-zoo circuits and sign-sensitive random circuits."""
+    cases = []
+    for name in ['surface_d3_r3', 'surface_d5_r5', 'surface_x_d5_r3', 'rep_d7_r5', 'color_d5_r3', 'unrotated_d3_r2',
+                 'surface_d25_r25']:
+        text = open(os.path.join(HERE, 'circuits', name + '.stim')).read()
+        cases.append({'name': name, 'ref': ref_sample_cli(text)})
+    for seed in range(40):
+        text = zoo.gate_zoo(seed)
+        cases.append({'name': 'gate_zoo_%d' % seed, 'circuit': text, 'ref': ref_sample_cli(text)})
+    for seed in range(500):
+        text = zoo.ref_zoo(seed, n_qubits=6 + seed % 5)
+        cases.append({'name': 'ref_zoo_%d' % seed, 'circuit': text, 'ref': ref_sample_cli(text)})
+    json.dump(cases, open(os.path.join(HERE, 'ref_samples.json'), 'w'))
+    return cases
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == '--ref-samples-only':
+        write_ref_samples()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == '--gen-pins-only':
         write_gen_pins()
         return

@@ -139,3 +139,79 @@ def gate_zoo(seed, n_qubits=8, layers=10, nobs=2, repeat=True):
     out.append(f'RX {q}')
     out.append(f'OBSERVABLE_INCLUDE({nobs - 1 if nobs else 0}) X{q}')
     return '\n'.join(out) + '\n'
+
+
+def ref_zoo(seed, n_qubits=6, layers=14):
+    """Seeded circuits whose reference sample depends on the sign conventions of every gate:
+    random Clifford layers, measurements and resets in every basis (some results inverted),
+    Pauli-product measurements and phases (repeated factors, inverted factors, measurement bits in
+    SPP products), pair measurements, feedback from measurement records, MPAD, and noise (which the
+    reference sample ignores; heralded errors record 0). Ends by measuring every qubit."""
+    rng = random.Random(seed)
+    qs = list(range(n_qubits))
+    lines = ['RX 0', 'RY 1']
+    nmeas = 0
+
+    def sub(k=None):
+        return rng.sample(qs, k=k or rng.randint(1, len(qs)))
+
+    def inv(q):
+        return ('!' if rng.random() < 0.3 else '') + str(q)
+
+    def product():
+        ps = rng.sample(qs, k=rng.randint(1, 3))
+        fs = [rng.choice('XYZ') + str(q) for q in ps]
+        if rng.random() < 0.3:
+            fs.append(fs[0])            # a repeated factor cancels
+        if rng.random() < 0.2:
+            q = ps[0]
+            fs += ['X%d' % q, 'Z%d' % q, 'X%d' % q, 'Z%d' % q]   # XZXZ = -I
+        return '*'.join(('!' if rng.random() < 0.25 else '') + f for f in fs)
+
+    for _ in range(layers):
+        k = rng.randrange(12)
+        if k < 3:
+            lines.append(rng.choice(ONE_Q) + ' ' + ' '.join(map(str, sub())))
+        elif k < 5:
+            ps = sub(2 * rng.randint(1, len(qs) // 2))
+            lines.append(rng.choice(TWO_Q) + ' ' + ' '.join(map(str, ps)))
+        elif k == 5:
+            ts = sub()
+            name = rng.choice(['M', 'MX', 'MY', 'MR', 'MRX', 'MRY'])
+            arg = '(0.05)' if rng.random() < 0.3 else ''
+            lines.append(name + arg + ' ' + ' '.join(inv(q) for q in ts))
+            nmeas += len(ts)
+        elif k == 6:
+            lines.append(rng.choice(['R', 'RX', 'RY']) + ' ' + ' '.join(map(str, sub())))
+        elif k == 7:
+            n = rng.randint(1, 3)
+            lines.append('MPP ' + ' '.join(product() for _ in range(n)))
+            nmeas += n
+        elif k == 8:
+            lines.append(rng.choice(['SPP', 'SPP_DAG']) + ' ' + ' '.join(product() for _ in range(rng.randint(1, 2))))
+        elif k == 9 and nmeas:
+            q = rng.choice(qs)
+            r = 'rec[-%d]' % rng.randint(1, min(nmeas, 6))
+            lines.append(rng.choice(['CX %s %d', 'CY %s %d', 'CZ %s %d']) % (r, q) if rng.random() < 0.6
+                         else rng.choice(['XCZ %d %s', 'YCZ %d %s', 'CZ %d %s']) % (q, r))
+        elif k == 10:
+            pairs = []
+            for _ in range(rng.randint(1, 3)):
+                a, b = rng.sample(qs, 2)
+                pairs.append('%s %s' % (inv(a), inv(b)))
+            lines.append('%s %s' % (rng.choice(['MXX', 'MYY', 'MZZ']), ' '.join(pairs)))
+            nmeas += len(pairs)
+        else:
+            kind = rng.randrange(4)
+            if kind == 0:
+                lines.append('MPAD %d %d' % (rng.randint(0, 1), rng.randint(0, 1)))
+                nmeas += 2
+            elif kind == 1:
+                lines.append('HERALDED_ERASE(0.2) %d' % rng.choice(qs))
+                nmeas += 1
+            elif kind == 2:
+                lines.append('DEPOLARIZE1(0.3) ' + ' '.join(map(str, sub())))
+            else:
+                lines.append('X_ERROR(0.4) ' + ' '.join(map(str, sub())))
+    lines.append(rng.choice(['M', 'MX', 'MY']) + ' ' + ' '.join(inv(q) for q in qs))
+    return '\n'.join(lines) + '\n'
