@@ -83,7 +83,8 @@ def test_parser_accepts_reference_syntax():
         DETECTOR(1, 2) rec[-1] rec[-2]
     }
     OBSERVABLE_INCLUDE(2) rec[-1] X1
-    SPP X0*rec[-1]
+    SPP X0*!Z1 Y2
+    SPP_DAG Z0
     MXX 0 1
     MYY 1 2
     MZZ 0 2
