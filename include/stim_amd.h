@@ -58,6 +58,18 @@ int mfs_circuit_stats(const mfs_sampler *s, uint64_t out[5]);
  * Reference: Circuit.compute_stats / num_detectors etc. (src/stim/circuit/circuit_instruction.cc:44-100). */
 int mfs_parse_stats(const char *stim_text, size_t len, uint64_t out[10]);
 
+/* Compiles a measurement sampler: sampling returns every measurement result (record order) instead of
+ * detection events, as the reference sample XOR the frame simulator's flips. The reference sample is
+ * computed on the host (skip_reference_sample = 0, reference_sample_bits = NULL), taken as all zeros
+ * (skip_reference_sample = 1), or given bit-packed little-endian in reference_sample_bits (at least
+ * ceil(num_measurements / 8) bytes). mfs_sample / mfs_sample_with_faults / mfs_sample_write then take
+ * ncols = num_measurements and no observables. Runs on the frame engine.
+ * Reference: Circuit.compile_sampler -> CompiledMeasurementSampler (src/stim/circuit/circuit.pybind.cc:588-594,
+ * src/stim/py/compiled_measurement_sampler.pybind.cc:27-80) and sample_batch_measurements
+ * (src/stim/simulators/frame_simulator_util.inl:290-372). */
+int mfs_compile_measurements(const char *stim_text, size_t len, uint64_t seed, int has_seed, int skip_reference_sample,
+                             const uint8_t *reference_sample_bits, size_t reference_len, mfs_sampler **out);
+
 /* Reference sample of a circuit (host only, no GPU): the noiseless run with every random measurement
  * result taken as 0, written bit-packed little-endian into out[0 .. ceil(num_measurements / 8)).
  * Reference: Circuit.reference_sample (src/stim/circuit/circuit.pybind.cc:664-671) and

@@ -5,7 +5,8 @@
     dets, obs = c.compile_detector_sampler(seed=5).sample(2**20, bit_packed=True, separate_observables=True)
 """
 
-from .sampler import Circuit, CompiledDetectorSampler  # noqa: F401
+from .sampler import Circuit, CompiledDetectorSampler, CompiledMeasurementSampler  # noqa: F401
 from .dem import DetectorErrorModel, CompiledDemSampler  # noqa: F401
 
-__all__ = ['Circuit', 'CompiledDetectorSampler', 'DetectorErrorModel', 'CompiledDemSampler']
+__all__ = ['Circuit', 'CompiledDetectorSampler', 'CompiledMeasurementSampler', 'DetectorErrorModel',
+           'CompiledDemSampler']

@@ -29,7 +29,7 @@ EXPORTED = [
     'mfs_batch_time', 'mfs_batch_autotune', 'mfs_batch_info', 'mfs_batch_set_schedule',
     'mfs_server_create', 'mfs_server_destroy', 'mfs_server_submit', 'mfs_server_step', 'mfs_server_poll',
     'mfs_server_result', 'mfs_server_copy_result', 'mfs_server_release', 'mfs_server_stats', 'mfs_server_replay',
-    'mfs_dem_compile', 'mfs_dem_parse_stats', 'mfs_reference_sample',
+    'mfs_dem_compile', 'mfs_dem_parse_stats', 'mfs_reference_sample', 'mfs_compile_measurements',
 ]
 
 _lib = None
@@ -57,6 +57,8 @@ def lib():
     L.mfs_circuit_stats.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_parse_stats.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(u64)]
     L.mfs_reference_sample.argtypes = [ctypes.c_char_p, sz, p, sz]
+    L.mfs_compile_measurements.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.c_int, p, sz,
+                                           ctypes.POINTER(p)]
     L.mfs_sample.argtypes = [p, u64, u32, p, sz, p, sz]
     L.mfs_sample_write.argtypes = [p, u64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_char_p, ctypes.c_char_p]

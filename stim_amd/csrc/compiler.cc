@@ -475,8 +475,20 @@ struct Builder {
     }
 
     // ------------------------------------------------------------------ main dispatch
+    // Measurement outputs: the circuit's detectors and observables are dropped and every measurement
+    // result becomes an output row of its own (a one-term detector right after the instruction).
+    bool meas_outputs = false;
+
     void run(const std::vector<const Instr *> &flat) {
         P.stats = compute_stats(flat);
+        if (meas_outputs) {
+            // The ring keeps every result of one instruction until its output rows are read.
+            uint64_t widest = 0;
+            for (const Instr *in : flat) widest = std::max<uint64_t>(widest, num_results(*in));
+            P.stats.max_lookback = std::max<uint64_t>(P.stats.max_lookback, widest);
+            P.stats.num_detectors = P.stats.num_measurements;
+            P.stats.num_observables = 0;
+        }
         std::vector<uint8_t> used(P.stats.num_qubits + 1, 0);
         for (const Instr *in : flat) {
             if (in->gate == Gate::MPAD || in->gate == Gate::QUBIT_COORDS) continue;
@@ -504,7 +516,13 @@ struct Builder {
         seg_mark.assign(P.num_qubits, 0);
 
         for (uint32_t k = 0; k < (uint32_t)flat.size(); k++) {
+            const Gate g = flat[k]->gate;
+            if (meas_outputs && (g == Gate::DETECTOR || g == Gate::OBSERVABLE_INCLUDE)) continue;
+            const uint64_t before = mcount;
             step(*flat[k], k);
+            if (meas_outputs) {
+                for (uint64_t j = mcount - before; j > 0; j--) det_terms.push_back({lookback_slot((uint32_t)j)});
+            }
         }
         flush_pending();
         if (det_base % STAGE_ROWS) {
@@ -867,8 +885,9 @@ struct Builder {
 
 }  // namespace
 
-Program compile_program(const std::vector<const Instr *> &flat) {
+Program compile_program(const std::vector<const Instr *> &flat, bool measurement_outputs) {
     Builder b;
+    b.meas_outputs = measurement_outputs;
     b.run(flat);
     return std::move(b.P);
 }

@@ -38,7 +38,9 @@ struct Program {
     Stats stats;
 };
 
-Program compile_program(const std::vector<const Instr *> &flat);
+// measurement_outputs: the outputs are the circuit's measurement results in record order (one
+// output row per result) instead of its detectors and observables.
+Program compile_program(const std::vector<const Instr *> &flat, bool measurement_outputs = false);
 
 // Resolves a fault (flat instruction index, group index, code) to (op, site); throws
 // std::out_of_range when the instruction has no such noise site.

@@ -487,6 +487,13 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(const uint32_t *__restric
                         const uint32_t b = w[i], e2 = w[i + 1];
                         Wd<T> v = wzero<T>();
                         for (uint32_t k = b; k < e2; k++) v = wxor(v, row<T>(t.REC, w[k]));
+                        if (P.det_invert) {
+                            const uint32_t d = op.base + i;
+                            if ((P.det_invert[d >> 5] >> (d & 31)) & 1u) {
+#pragma unroll
+                                for (int k = 0; k < T; k++) v.v[k] = ~v.v[k];
+                            }
+                        }
                         row<T>(t.STG, (op.base + i) % STAGE_ROWS) = v;
                     }
                     break;

@@ -124,6 +124,7 @@ struct LaunchParams {
     uint32_t num_obs;
     uint32_t mode;             // 0 = no noise, 1 = sampled, 2 = fixed faults
     uint32_t randomize;        // 1 = frame randomization
+    const uint32_t *det_invert;  // optional bit per output row: the row is inverted (reference sample bits)
 };
 
 // Device-resident program buffers.

@@ -159,6 +159,57 @@ int mfs_compile(const char *stim_text, size_t len, uint64_t seed, int has_seed, 
     });
 }
 
+int mfs_compile_measurements(const char *stim_text, size_t len, uint64_t seed, int has_seed, int skip_reference_sample,
+                             const uint8_t *reference_sample_bits, size_t reference_len, mfs_sampler **out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out is null");
+        *out = nullptr;
+        if (skip_reference_sample && reference_sample_bits)
+            throw std::invalid_argument("skip_reference_sample = True but reference_sample is not None.");
+        auto s = std::make_unique<mfs_sampler>();
+        s->text.assign(stim_text ? stim_text : "", stim_text ? len : 0);
+        s->block = parse_circuit(s->text.data(), s->text.size());
+        s->flat = flatten(s->block);
+        s->prog = compile_program(s->flat, true);
+        const uint64_t M = s->prog.num_measurements;
+        std::vector<uint32_t> words((M + 31) / 32, 0);
+        if (reference_sample_bits) {
+            if (reference_len < (M + 7) / 8) throw std::invalid_argument("reference_sample is shorter than the circuit's measurements");
+            for (uint64_t m = 0; m < M; m++) {
+                if ((reference_sample_bits[m >> 3] >> (m & 7)) & 1) words[m >> 5] |= 1u << (m & 31);
+            }
+        } else if (!skip_reference_sample) {
+            const std::vector<uint8_t> ref = reference_sample(s->flat);
+            for (uint64_t m = 0; m < M; m++) {
+                if (ref[m]) words[m >> 5] |= 1u << (m & 31);
+            }
+        }
+        if (has_seed) {
+            s->key = splitmix64(seed ^ 0xDEADBEEF124Cull);
+        } else {
+            std::random_device rd;
+            s->key = ((uint64_t)rd() << 32) ^ rd();
+        }
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            throw HipError("no HIP device available: the stim_amd sampler runs only on the GPU");
+        }
+        check(hipStreamCreate(&s->stream), "stream");
+        s->meas_mode = true;
+        s->engine_mode = 1;
+        s->pick_tile_words(0);
+        s->upload_program();
+        s->probe_randomization();
+        bool any = false;
+        for (uint32_t w : words) any |= w != 0;
+        if (any) {
+            mfs_sampler::upload(s->d_invert, words, "reference sample");
+            s->has_invert = true;
+        }
+        *out = s.release();
+    });
+}
+
 int mfs_dem_compile(const char *dem_text, size_t len, uint64_t seed, int has_seed, mfs_sampler **out) {
     return guarded([&] {
         if (!out) throw std::invalid_argument("out is null");
@@ -279,7 +330,7 @@ int mfs_sample_write(mfs_sampler *s, uint64_t shots, const char *path, const cha
         const size_t drow = (ncols + 7) / 8, orow = (O + 7) / 8;
         const uint64_t chunk = std::max<uint64_t>(64, ((64ull << 20) / std::max<size_t>(drow + orow, 1)) / 64 * 64);
         std::vector<uint8_t> dbuf, obuf;
-        char p1 = 'D', p2 = 'L';
+        char p1 = s->meas_mode ? 'M' : 'D', p2 = 'L';
         uint32_t transition = D;
         if (prepend_observables) {
             p1 = 'L';
@@ -360,6 +411,10 @@ int mfs_set_engine(mfs_sampler *s, int engine) {
         if (engine < 0 || engine > 2) throw std::invalid_argument("engine must be 0 (auto), 1 (frame) or 2 (symptom)");
         if (s->dem_mode) {
             if (engine == 1) throw std::invalid_argument("a detector error model sampler has no frame engine");
+            return;
+        }
+        if (s->meas_mode) {
+            if (engine == 2) throw std::invalid_argument("the measurement sampler runs on the frame engine");
             return;
         }
         s->engine_mode = engine;

@@ -133,6 +133,11 @@ struct mfs_sampler {
     uint64_t sym_entries = 0;
     double sym_build_ms = 0;
     DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols, d_recs32, d_ovf32, d_groups, d_thr;
+    // Measurement samplers: the outputs are the measurement results (reference sample bits XORed
+    // into the frame's flips through det_invert); they sample on the frame engine.
+    bool meas_mode = false;
+    DevBuf d_invert;
+    bool has_invert = false;
     // Detector-error-model samplers: no circuit; the symptom tables come from the model's mechanisms.
     bool dem_mode = false;
     std::vector<uint32_t> dem_col;   // mechanism index (model order) -> symptom column
@@ -176,6 +181,7 @@ struct mfs_sampler {
         P.num_qubits = prog.num_qubits;
         P.ring = prog.ring;
         P.num_obs = prog.num_observables;
+        P.det_invert = has_invert ? d_invert.as<uint32_t>() : nullptr;
         return P;
     }
 
@@ -255,6 +261,7 @@ struct mfs_sampler {
         P.tile0 = 0;
         P.mode = MODE_NOISELESS;
         P.randomize = 1;
+        P.det_invert = nullptr;   // the probe looks at flips only
         launch(P, tiles);
         std::vector<uint8_t> a(shots * L.det_row), b(shots * L.obs_row);
         if (!a.empty()) check(hipMemcpyAsync(a.data(), d_out_det.p, a.size(), hipMemcpyDeviceToHost, stream), "copy");
@@ -410,6 +417,7 @@ struct mfs_sampler {
             d_out_det.ensure(n * drow);
             d_out_obs.ensure(n * orow);
             LaunchParams P = base_params();
+            P.det_invert = nullptr;   // symptom columns are flips only
             outputs(L, d_out_det.as<uint8_t>(), drow, d_out_obs.as<uint8_t>(), orow, P);
             P.shots = n;
             P.tile0 = 0;
@@ -675,7 +683,7 @@ struct mfs_sampler {
     // engine instead (until a larger call builds the table). ANY_SHOTS skips that test.
     static constexpr uint64_t ANY_SHOTS = ~0ull;
     bool use_symptoms(bool fixed, uint64_t shots = ANY_SHOTS) {
-        if (engine_mode == 1) return false;
+        if (engine_mode == 1 || meas_mode) return false;
         if ((prog.num_detectors + 2ull * prog.num_observables) / 8 > 64 * 1024) {
             if (engine_mode == 2) throw std::invalid_argument("rows too wide for the symptom engine's LDS image");
             return false;

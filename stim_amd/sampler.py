@@ -82,6 +82,12 @@ class Circuit:
     def compile_detector_sampler(self, *, seed=None) -> 'CompiledDetectorSampler':
         return CompiledDetectorSampler(self, seed=seed)
 
+    def compile_sampler(self, *, skip_reference_sample: bool = False, seed=None,
+                        reference_sample=None) -> 'CompiledMeasurementSampler':
+        """Reference: stim.Circuit.compile_sampler (src/stim/circuit/circuit.pybind.cc:588-594)."""
+        return CompiledMeasurementSampler(self, skip_reference_sample=skip_reference_sample, seed=seed,
+                                          reference_sample=reference_sample)
+
     def reference_sample(self, *, bit_packed: bool = False) -> np.ndarray:
         """The noiseless run with every random measurement result taken as 0 (host tableau).
 
@@ -289,3 +295,68 @@ class CompiledDetectorSampler:
         out = (ctypes.c_double * 2)()
         N.check(N.lib().mfs_bench_device(self._h, int(shots), int(warmup), int(iters), out))
         return (out[0], out[1]) if split else out[0]
+
+
+class CompiledMeasurementSampler(CompiledDetectorSampler):
+    """Samples every measurement result of a circuit on the GPU: the circuit's reference sample XOR
+    the frame simulator's flips, one row per shot in record order.
+
+    Reference: stim.CompiledMeasurementSampler (src/stim/py/compiled_measurement_sampler.pybind.cc:27-80,
+    85-150) and sample_batch_measurements (src/stim/simulators/frame_simulator_util.inl:290-372)."""
+
+    def __init__(self, circuit, *, skip_reference_sample: bool = False, seed=None, reference_sample=None):
+        if not isinstance(circuit, Circuit):
+            circuit = Circuit(str(circuit))
+        self._circuit = circuit
+        self._skip = bool(skip_reference_sample)
+        if reference_sample is not None and self._skip:
+            raise ValueError('skip_reference_sample = True but reference_sample is not None.')
+        ref = None
+        if reference_sample is not None:
+            ref = np.asarray(reference_sample)
+            m = circuit.num_measurements
+            if ref.dtype == np.bool_:
+                ref = np.packbits(ref.reshape(-1), bitorder='little')
+            elif ref.dtype != np.uint8:
+                raise ValueError('reference_sample must be a bool or bit-packed uint8 numpy array')
+            ref = np.ascontiguousarray(ref.reshape(-1))
+            if ref.size < (m + 7) // 8:
+                raise ValueError('reference_sample is shorter than the number of measurements')
+        handle = ctypes.c_void_p()
+        t = _text_bytes(str(circuit))
+        has_seed = seed is not None
+        if has_seed:
+            seed = int(seed)
+            if seed < 0 or seed >= 1 << 64:
+                raise ValueError('seed must be a non-negative 64-bit integer')
+        N.check(N.lib().mfs_compile_measurements(t, len(t), seed if has_seed else 0, 1 if has_seed else 0,
+                                                 1 if self._skip else 0,
+                                                 ref.ctypes.data if ref is not None and ref.size else None,
+                                                 0 if ref is None else ref.size, ctypes.byref(handle)))
+        self._h = handle
+        stats = (ctypes.c_uint64 * 5)()
+        N.check(N.lib().mfs_circuit_stats(self._h, stats))
+        # Output rows are the measurement results (no observables).
+        self.num_qubits, self.num_measurements, self.num_detectors, self.num_observables, _ = list(stats)
+
+    def __repr__(self):
+        extra = ', skip_reference_sample=True' if self._skip else ''
+        return f'stim_amd.CompiledMeasurementSampler(stim_amd.Circuit({str(self._circuit)!r}){extra})'
+
+    def sample(self, shots: int, *, bit_packed: bool = False) -> np.ndarray:
+        """(shots, num_measurements) bool, or (shots, ceil(num_measurements / 8)) uint8 bit-packed."""
+        shots = int(shots)
+        if shots < 0:
+            raise ValueError('shots must be non-negative')
+        return self._sample_impl(shots, False, False, False, bit_packed, None, None, None)
+
+    def sample_write(self, shots: int, *, filepath, format: str = '01'):
+        """Writes results in a reference result format ('01', 'b8', 'r8', 'ptb64', 'hits', 'dets')."""
+        N.check(N.lib().mfs_sample_write(self._h, int(shots), os.fsencode(filepath), format.encode(), 0, 0,
+                                         None, None))
+
+    def sample_with_faults(self, faults, *, bit_packed=False):
+        """Measurement results with exactly the given faults per shot and no other noise (the frame
+        simulator's randomization of random results still applies). Same fault addressing as
+        CompiledDetectorSampler.sample_with_faults; no reference counterpart."""
+        return super().sample_with_faults(faults, bit_packed=bit_packed)

@@ -302,7 +302,36 @@ def write_ref_samples():
     return cases
 
 
+def write_meas_counts():
+    """Per-measurement counts of ones from the reference CLI's seeded `stim sample` (reference sample
+    XOR frame flips, random results randomized), for a two-sample test of the measurement sampler."""
+    out = {}
+    shots = 200000
+    texts = {name: open(os.path.join(HERE, 'circuits', name + '.stim')).read()
+             for name in ['surface_d3_r3', 'rep_d7_r5', 'color_d5_r3', 'unrotated_d3_r2', 'surface_x_d5_r3']}
+    for seed in range(3):
+        texts['gate_zoo_%d' % seed] = zoo.gate_zoo(seed)
+        texts['ref_zoo_%d' % seed] = zoo.ref_zoo(seed)
+    import numpy as np
+    for name, text in texts.items():
+        m = st.stats(st.parse_flat(text))['num_measurements']
+        with tempfile.TemporaryDirectory() as td:
+            cpath = os.path.join(td, 'c.stim')
+            with open(cpath, 'w') as f:
+                f.write(text)
+            opath = os.path.join(td, 'o.b8')
+            run(['sample', '--shots', str(shots), '--in', cpath, '--out_format', 'b8', '--out', opath, '--seed', '7'])
+            raw = np.fromfile(opath, dtype=np.uint8).reshape(shots, (m + 7) // 8)
+        bits = np.unpackbits(raw, axis=1, bitorder='little')[:, :m]
+        out[name] = {'shots': shots, 'counts': bits.sum(axis=0).tolist(),
+                     'circuit': text if name.startswith(('gate_zoo', 'ref_zoo')) else None}
+    json.dump(out, open(os.path.join(HERE, 'meas_counts.json'), 'w'))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == '--meas-counts-only':
+        write_meas_counts()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == '--ref-samples-only':
         write_ref_samples()
         return
