@@ -70,6 +70,30 @@ int mfs_parse_stats(const char *stim_text, size_t len, uint64_t out[10]);
 int mfs_compile_measurements(const char *stim_text, size_t len, uint64_t seed, int has_seed, int skip_reference_sample,
                              const uint8_t *reference_sample_bits, size_t reference_len, mfs_sampler **out);
 
+/* Measurement-to-detection-event converter (no sweep data). Compiles the circuit's detectors and
+ * observables and its reference sample (all zeros with skip_reference_sample = 1).
+ * Reference: Circuit.compile_m2d_converter (src/stim/circuit/circuit.pybind.cc:781-785) ->
+ * CompiledMeasurementsToDetectionEventsConverter (src/stim/simulators/measurements_to_detection_events.pybind.cc:159-166)
+ * and the CLI's `stim m2d` (src/stim/cmd/command_m2d.cc). */
+typedef struct mfs_m2d mfs_m2d;
+int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample, mfs_m2d **out);
+void mfs_m2d_destroy(mfs_m2d *c);
+/* out[0..2] = num_measurements, num_detectors, num_observables. */
+int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[3]);
+/* Converts `shots` bit-packed little-endian measurement rows (meas_stride bytes apart) into detection
+ * event rows (with observables appended when MFS_APPEND_OBSERVABLES) and, with
+ * MFS_SEPARATE_OBSERVABLES, observable rows; MFS_BIT_PACKED selects packed outputs (else one byte per
+ * entry); MFS_DEVICE_OUTPUT means every pointer is device memory. Each detector (observable) is the
+ * XOR of its measurements, inverted when the XOR of their reference bits is 1.
+ * Reference: CompiledMeasurementsToDetectionEventsConverter.convert
+ * (measurements_to_detection_events.pybind.cc:78-157) and measurements_to_detection_events_helper
+ * (src/stim/simulators/measurements_to_detection_events.inl:29-131). */
+int mfs_m2d_convert(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_t meas_stride, uint32_t flags, uint8_t *dets,
+                    size_t dets_stride, uint8_t *obs, size_t obs_stride);
+/* Benchmark helper (no reference counterpart): average kernel time in ms of converting `shots`
+ * device-resident rows (observables appended), measured with HIP events on the converter's stream. */
+int mfs_m2d_bench(mfs_m2d *c, uint64_t shots, int warmup, int iters, double out_ms[1]);
+
 /* Reference sample of a circuit (host only, no GPU): the noiseless run with every random measurement
  * result taken as 0, written bit-packed little-endian into out[0 .. ceil(num_measurements / 8)).
  * Reference: Circuit.reference_sample (src/stim/circuit/circuit.pybind.cc:664-671) and

@@ -7,6 +7,7 @@
 
 from .sampler import Circuit, CompiledDetectorSampler, CompiledMeasurementSampler  # noqa: F401
 from .dem import DetectorErrorModel, CompiledDemSampler  # noqa: F401
+from .m2d import CompiledMeasurementsToDetectionEventsConverter  # noqa: F401
 
 __all__ = ['Circuit', 'CompiledDetectorSampler', 'CompiledMeasurementSampler', 'DetectorErrorModel',
-           'CompiledDemSampler']
+           'CompiledDemSampler', 'CompiledMeasurementsToDetectionEventsConverter']

@@ -30,6 +30,7 @@ EXPORTED = [
     'mfs_server_create', 'mfs_server_destroy', 'mfs_server_submit', 'mfs_server_step', 'mfs_server_poll',
     'mfs_server_result', 'mfs_server_copy_result', 'mfs_server_release', 'mfs_server_stats', 'mfs_server_replay',
     'mfs_dem_compile', 'mfs_dem_parse_stats', 'mfs_reference_sample', 'mfs_compile_measurements',
+    'mfs_m2d_compile', 'mfs_m2d_destroy', 'mfs_m2d_stats', 'mfs_m2d_convert', 'mfs_m2d_bench',
 ]
 
 _lib = None
@@ -57,6 +58,12 @@ def lib():
     L.mfs_circuit_stats.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_parse_stats.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(u64)]
     L.mfs_reference_sample.argtypes = [ctypes.c_char_p, sz, p, sz]
+    L.mfs_m2d_compile.argtypes = [ctypes.c_char_p, sz, ctypes.c_int, ctypes.POINTER(p)]
+    L.mfs_m2d_destroy.argtypes = [p]
+    L.mfs_m2d_destroy.restype = None
+    L.mfs_m2d_stats.argtypes = [p, ctypes.POINTER(u64)]
+    L.mfs_m2d_convert.argtypes = [p, u64, p, sz, u32, p, sz, p, sz]
+    L.mfs_m2d_bench.argtypes = [p, u64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.mfs_compile_measurements.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.c_int, p, sz,
                                            ctypes.POINTER(p)]
     L.mfs_sample.argtypes = [p, u64, u32, p, sz, p, sz]

@@ -1,0 +1,234 @@
+// Measurement-to-detection-event converter behind the C ABI (mfs_m2d_*).
+//
+// Mirrors the reference's CompiledMeasurementsToDetectionEventsConverter
+// (src/stim/simulators/measurements_to_detection_events.pybind.cc:78-159) without sweep data: the
+// circuit's detectors and observables become CSR lists of absolute measurement indices, each with
+// the parity of its measurements' reference-sample bits (semantics in m2d.h). Conversion runs on
+// the GPU (m2d_kernel.hip); host rows are staged through device buffers with 16-byte-aligned strides.
+#include "m2d.h"
+
+#include "sampler_impl.h"
+#include "tableau.h"
+
+namespace sa {
+hipError_t launch_m2d(const M2dParams &P, hipStream_t stream);
+size_t m2d_lds_bytes(uint32_t blocks);
+}  // namespace sa
+
+struct mfs_m2d {
+    uint32_t M = 0, D = 0, O = 0;
+    std::vector<uint32_t> off, terms, flip;
+    DevBuf d_off, d_terms, d_flip, d_in, d_out, d_obs, d_tmp;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+
+    ~mfs_m2d() {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    uint32_t blocks() const { return (M + 31) / 32; }
+    size_t in_row() const { return (M + 7) / 8; }
+    size_t in_vec_stride() const { return (size_t)16 * ((blocks() + 3) / 4); }
+
+    M2dParams params(uint64_t shots, const uint8_t *in, size_t in_stride, uint8_t *out, size_t out_stride,
+                     uint32_t n_out, uint8_t *obs, size_t obs_stride, uint32_t n_obs) const {
+        M2dParams P{};
+        P.in = in;
+        P.in_stride = in_stride;
+        P.in_row_bytes = (uint32_t)in_row();
+        P.in_vec = ((uintptr_t)in % 16 == 0 && in_stride % 16 == 0 && in_stride >= in_vec_stride()) ? 1u : 0u;
+        P.blocks = blocks();
+        P.num_det = D;
+        P.out = out;
+        P.out_stride = out_stride;
+        P.n_out = n_out;
+        P.n_obs = n_obs;
+        P.obs = obs;
+        P.obs_stride = obs_stride;
+        P.shots = shots;
+        P.off = const_cast<DevBuf &>(d_off).as<uint32_t>();
+        P.terms = const_cast<DevBuf &>(d_terms).as<uint32_t>();
+        P.flip = const_cast<DevBuf &>(d_flip).as<uint32_t>();
+        return P;
+    }
+
+    void launch(const M2dParams &P) {
+        if ((P.shots + 31) / 32 >= (1ull << 31)) throw std::invalid_argument("too many shots in one conversion");
+        check(launch_m2d(P, stream), "m2d launch");
+    }
+};
+
+extern "C" {
+
+int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample, mfs_m2d **out) {
+    return guarded([&] {
+        if (!out) throw std::invalid_argument("out is null");
+        *out = nullptr;
+        std::string text(stim_text ? stim_text : "", stim_text ? len : 0);
+        Block b = parse_circuit(text.data(), text.size());
+        const std::vector<const Instr *> flat = flatten(b);
+        const Stats st = compute_stats(flat);
+        if (st.num_measurements >= (1ull << 32) || st.num_detectors + st.num_observables >= (1ull << 31))
+            throw std::invalid_argument("circuit too large for the converter");
+        auto c = std::make_unique<mfs_m2d>();
+        c->M = (uint32_t)st.num_measurements;
+        c->D = (uint32_t)st.num_detectors;
+        c->O = (uint32_t)st.num_observables;
+        if (m2d_lds_bytes(c->blocks()) > MAX_LDS)
+            throw std::invalid_argument("the converter keeps a tile's measurements in LDS: at most " +
+                                        std::to_string(MAX_LDS / 4) + " measurements per shot");
+        std::vector<uint8_t> ref(c->M, 0);
+        if (!skip_reference_sample) ref = reference_sample(flat);
+        std::vector<std::vector<uint32_t>> obs_terms(c->O);
+        c->off.push_back(0);
+        std::vector<uint8_t> col_flip;
+        uint64_t m = 0;
+        for (const Instr *in : flat) {
+            if (in->gate == Gate::DETECTOR) {
+                uint8_t f = 0;
+                for (const Target &t : in->targets) {
+                    const uint32_t idx = (uint32_t)(m - t.value);
+                    c->terms.push_back(idx);
+                    f ^= ref[idx];
+                }
+                c->off.push_back((uint32_t)c->terms.size());
+                col_flip.push_back(f);
+            } else if (in->gate == Gate::OBSERVABLE_INCLUDE) {
+                for (const Target &t : in->targets) {
+                    if (t.kind == TK::REC) obs_terms[(size_t)in->args[0]].push_back((uint32_t)(m - t.value));
+                }
+            }
+            m += num_results(*in);
+        }
+        for (uint32_t k = 0; k < c->O; k++) {
+            uint8_t f = 0;
+            for (uint32_t idx : obs_terms[k]) {
+                c->terms.push_back(idx);
+                f ^= ref[idx];
+            }
+            c->off.push_back((uint32_t)c->terms.size());
+            col_flip.push_back(f);
+        }
+        c->flip.assign((col_flip.size() + 31) / 32 + 1, 0);
+        for (size_t i = 0; i < col_flip.size(); i++) {
+            if (col_flip[i]) c->flip[i >> 5] |= 1u << (i & 31);
+        }
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            throw HipError("no HIP device available: the stim_amd converter runs only on the GPU");
+        }
+        check(hipStreamCreate(&c->stream), "stream");
+        mfs_sampler::upload(c->d_off, c->off, "m2d offsets");
+        mfs_sampler::upload(c->d_terms, c->terms, "m2d terms");
+        mfs_sampler::upload(c->d_flip, c->flip, "m2d reference parities");
+        *out = c.release();
+    });
+}
+
+void mfs_m2d_destroy(mfs_m2d *c) {
+    delete c;
+}
+
+int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[3]) {
+    return guarded([&] {
+        out[0] = c->M;
+        out[1] = c->D;
+        out[2] = c->O;
+    });
+}
+
+int mfs_m2d_convert(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_t meas_stride, uint32_t flags,
+                    uint8_t *dets, size_t dets_stride, uint8_t *obs, size_t obs_stride) {
+    return guarded([&] {
+        const bool append = flags & MFS_APPEND_OBSERVABLES, separate = flags & MFS_SEPARATE_OBSERVABLES;
+        const bool packed = flags & MFS_BIT_PACKED, device = flags & MFS_DEVICE_OUTPUT;
+        if (flags & MFS_PREPEND_OBSERVABLES) throw std::invalid_argument("the converter does not prepend observables");
+        if (shots == 0) return;
+        const uint32_t n_out = c->D + (append ? c->O : 0), n_obs = separate ? c->O : 0;
+        const size_t out_row = (n_out + 7) / 8, obs_row = (n_obs + 7) / 8;
+        if (c->M && (!meas || meas_stride < c->in_row())) throw std::invalid_argument("measurement rows are too short");
+        if (n_out && (!dets || dets_stride < (packed ? out_row : n_out))) throw std::invalid_argument("detection event rows are too short");
+        if (n_obs && (!obs || obs_stride < (packed ? obs_row : n_obs))) throw std::invalid_argument("observable rows are too short");
+        if (device && packed) {
+            c->launch(c->params(shots, meas, meas_stride, dets, dets_stride, n_out, obs, obs_stride, n_obs));
+            check(hipStreamSynchronize(c->stream), "m2d");
+            return;
+        }
+        // Staged: batches of shots through device buffers (input rows at the vector stride).
+        const size_t vs = std::max<size_t>(c->in_vec_stride(), 16);
+        const size_t per_shot = vs + out_row + obs_row + (packed ? 0 : n_out + n_obs);
+        const uint64_t batch = std::max<uint64_t>(32, (HOST_BATCH_BYTES / std::max<size_t>(per_shot, 1)) / 32 * 32);
+        for (uint64_t s0 = 0; s0 < shots; s0 += batch) {
+            const uint64_t n = std::min(batch, shots - s0);
+            c->d_in.ensure(n * vs);
+            c->d_out.ensure(n * std::max<size_t>(out_row, 1));
+            c->d_obs.ensure(n * std::max<size_t>(obs_row, 1));
+            if (c->M) {
+                if (device) {
+                    check(hipMemcpy2DAsync(c->d_in.p, vs, meas + s0 * meas_stride, meas_stride, c->in_row(), n,
+                                           hipMemcpyDeviceToDevice, c->stream), "copy measurements");
+                } else {
+                    check(hipMemcpy2DAsync(c->d_in.p, vs, meas + s0 * meas_stride, meas_stride, c->in_row(), n,
+                                           hipMemcpyHostToDevice, c->stream), "copy measurements");
+                }
+            }
+            c->launch(c->params(n, c->d_in.as<uint8_t>(), vs, c->d_out.as<uint8_t>(), std::max<size_t>(out_row, 1),
+                                n_out, c->d_obs.as<uint8_t>(), std::max<size_t>(obs_row, 1), n_obs));
+            const hipMemcpyKind back = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+            auto deliver = [&](DevBuf &src, size_t row, uint32_t ncols, uint8_t *dst, size_t dst_stride) {
+                if (!ncols) return;
+                const uint8_t *from = src.as<uint8_t>();
+                size_t from_stride = row, width = row;
+                if (!packed) {
+                    c->d_tmp.ensure(n * ncols);
+                    check(launch_unpack_rows(from, row, c->d_tmp.as<uint8_t>(), ncols, n, ncols, 0, c->stream), "unpack");
+                    from = c->d_tmp.as<uint8_t>();
+                    from_stride = ncols;
+                    width = ncols;
+                }
+                check(hipMemcpy2DAsync(dst + s0 * dst_stride, dst_stride, from, from_stride, width, n, back, c->stream),
+                      "copy results");
+                check(hipStreamSynchronize(c->stream), "m2d");
+            };
+            deliver(c->d_out, std::max<size_t>(out_row, 1), n_out, dets, dets_stride);
+            deliver(c->d_obs, std::max<size_t>(obs_row, 1), n_obs, obs, obs_stride);
+            check(hipStreamSynchronize(c->stream), "m2d");
+        }
+    });
+}
+
+int mfs_m2d_bench(mfs_m2d *c, uint64_t shots, int warmup, int iters, double out_ms[1]) {
+    return guarded([&] {
+        // Device-resident packed rows: measurements at the vector stride, detection events with
+        // appended observables.
+        const size_t vs = std::max<size_t>(c->in_vec_stride(), 16);
+        const uint32_t n_out = c->D + c->O;
+        const size_t out_row = std::max<size_t>((n_out + 7) / 8, 1);
+        DevBuf in, out;
+        in.ensure(shots * vs);
+        out.ensure(shots * out_row);
+        check(hipMemsetAsync(in.p, 0x5A, shots * vs, c->stream), "fill");
+        const M2dParams P = c->params(shots, in.as<uint8_t>(), vs, out.as<uint8_t>(), out_row, n_out, nullptr, 0, 0);
+        if (!c->ev0) {
+            check(hipEventCreate(&c->ev0), "event");
+            check(hipEventCreate(&c->ev1), "event");
+        }
+        for (int i = 0; i < warmup; i++) c->launch(P);
+        check(hipStreamSynchronize(c->stream), "warmup");
+        double total = 0;
+        for (int i = 0; i < iters; i++) {
+            check(hipEventRecord(c->ev0, c->stream), "event");
+            c->launch(P);
+            check(hipEventRecord(c->ev1, c->stream), "event");
+            check(hipEventSynchronize(c->ev1), "event");
+            float ms = 0;
+            check(hipEventElapsedTime(&ms, c->ev0, c->ev1), "elapsed");
+            total += ms;
+        }
+        out_ms[0] = iters ? total / iters : 0;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,32 @@
+// Measurement-to-detection-event conversion: host structures shared with the kernel.
+//
+// Reference: measurements_to_detection_events_helper (src/stim/simulators/measurements_to_detection_events.inl:29-131).
+// Without sweep data, detector d of a shot is the XOR of the shot's measurement results that d names,
+// inverted when the XOR of those measurements' reference-sample bits is 1; observable k likewise over
+// the measurement records OBSERVABLE_INCLUDE(k) names (its Pauli targets add nothing).
+#pragma once
+
+#include <cstdint>
+
+namespace sa {
+
+struct M2dParams {
+    const uint8_t *in;          // bit-packed measurement rows (shot-major), M bits each
+    uint64_t in_stride;
+    uint32_t in_row_bytes;      // ceil(M / 8)
+    uint32_t in_vec;            // rows are 16-byte aligned with 16-byte-multiple strides
+    uint32_t blocks;            // ceil(M / 32)
+    uint32_t num_det;           // D: columns 0 .. D-1 of the CSR are detectors, D .. D+O-1 observables
+    uint8_t *out;               // bit-packed output rows: n_out columns starting at CSR column 0
+    uint64_t out_stride;
+    uint32_t n_out;
+    uint32_t n_obs;             // separate observable rows (columns D .. D+n_obs-1), or 0
+    uint8_t *obs;
+    uint64_t obs_stride;
+    uint64_t shots;
+    const uint32_t *off;        // CSR offsets over the D + O columns
+    const uint32_t *terms;      // absolute measurement indices
+    const uint32_t *flip;       // bit per column: XOR of its measurements' reference bits
+};
+
+}  // namespace sa

@@ -88,6 +88,11 @@ class Circuit:
         return CompiledMeasurementSampler(self, skip_reference_sample=skip_reference_sample, seed=seed,
                                           reference_sample=reference_sample)
 
+    def compile_m2d_converter(self, *, skip_reference_sample: bool = False):
+        """Reference: stim.Circuit.compile_m2d_converter (src/stim/circuit/circuit.pybind.cc:781-785)."""
+        from .m2d import CompiledMeasurementsToDetectionEventsConverter
+        return CompiledMeasurementsToDetectionEventsConverter(self, skip_reference_sample=skip_reference_sample)
+
     def reference_sample(self, *, bit_packed: bool = False) -> np.ndarray:
         """The noiseless run with every random measurement result taken as 0 (host tableau).
 

@@ -328,7 +328,67 @@ def write_meas_counts():
     json.dump(out, open(os.path.join(HERE, 'meas_counts.json'), 'w'))
 
 
+def _with_random_checks(text, rng, m):
+    """Appends random detectors and observables over a circuit's measurements (the reference converter
+    does not require them to be deterministic), so conversion exercises reference parities."""
+    lines = [text.rstrip('\n')]
+    for _ in range(rng.randint(3, 12)):
+        ks = rng.sample(range(1, m + 1), k=min(m, rng.randint(1, 4)))
+        lines.append('DETECTOR ' + ' '.join('rec[-%d]' % k for k in ks))
+    for o in range(rng.randint(0, 3)):
+        ks = rng.sample(range(1, m + 1), k=min(m, rng.randint(1, 3)))
+        lines.append('OBSERVABLE_INCLUDE(%d) ' % o + ' '.join('rec[-%d]' % k for k in ks))
+    return '\n'.join(lines) + '\n'
+
+
+def write_m2d_fixtures():
+    """Reference `stim m2d` conversions of seeded random measurement data (bit-packed rows as hex),
+    with observables appended, with and without the reference sample."""
+    import numpy as np
+    rng = random.Random(20261204)
+    nrng = np.random.default_rng(20261204)
+    texts = {name: open(os.path.join(HERE, 'circuits', name + '.stim')).read()
+             for name in ['surface_d3_r3', 'surface_d5_r5', 'rep_d7_r5', 'color_d5_r3', 'unrotated_d3_r2',
+                          'surface_x_d5_r3', 'surface_d25_r25']}
+    for seed in range(10):
+        texts['gate_zoo_%d' % seed] = zoo.gate_zoo(seed)
+    for seed in range(30):
+        t = zoo.ref_zoo(1000 + seed, n_qubits=6 + seed % 4)
+        m = st.stats(st.parse_flat(t))['num_measurements']
+        texts['ref_zoo_%d' % (1000 + seed)] = _with_random_checks(t, rng, m)
+    cases = []
+    for name, text in texts.items():
+        s = st.stats(st.parse_flat(text))
+        m, nd, no = s['num_measurements'], s['num_detectors'], s['num_observables']
+        shots = 16 if name == 'surface_d25_r25' else 64
+        bits = nrng.random((shots, m)) < (0.5 if name.startswith(('gate', 'ref')) else 0.05)
+        for skip in (False, True):
+            with tempfile.TemporaryDirectory() as td:
+                cpath = os.path.join(td, 'c.stim')
+                with open(cpath, 'w') as f:
+                    f.write(text)
+                ipath = os.path.join(td, 'm.b8')
+                opath = os.path.join(td, 'd.b8')
+                np.packbits(bits, axis=1, bitorder='little').tofile(ipath)
+                args = ['m2d', '--circuit', cpath, '--in', ipath, '--in_format', 'b8', '--out', opath,
+                        '--out_format', 'b8', '--append_observables']
+                if skip:
+                    args.append('--skip_reference_sample')
+                run(args, stdin='')
+                out = np.fromfile(opath, dtype=np.uint8).reshape(shots, (nd + no + 7) // 8)
+            cases.append({'name': name, 'skip_reference_sample': skip, 'shots': shots,
+                          'reference_sample': ref_sample_cli(text),
+                          'circuit': None if name in ('surface_d3_r3', 'surface_d5_r5', 'rep_d7_r5', 'color_d5_r3',
+                                                      'unrotated_d3_r2', 'surface_x_d5_r3', 'surface_d25_r25') else text,
+                          'measurements': np.packbits(bits, axis=1, bitorder='little').tobytes().hex(),
+                          'expected': out.tobytes().hex()})
+    json.dump(cases, open(os.path.join(HERE, 'm2d_vectors.json'), 'w'))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == '--m2d-only':
+        write_m2d_fixtures()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == '--meas-counts-only':
         write_meas_counts()
         return

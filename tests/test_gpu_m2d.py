@@ -1,0 +1,171 @@
+"""Measurement-to-detection-event conversion on the GPU (Circuit.compile_m2d_converter()).
+
+Pinned by: the reference CLI's `stim m2d` outputs (m2d_vectors.json, bit-exact, with and without the
+reference sample, up to the d=25 circuit); the conversion oracle (oracle/m2d_oracle.py, pinned on the
+same vectors) on larger random data; and, composed with the measurement sampler, the detector
+sampler's fixed-fault vectors (bit-exact) and the exact detector rates of the circuit's error model.
+"""
+
+import os
+
+import numpy as np
+import pytest
+
+from oracle import m2d_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+
+def _rows(hexdata, shots, bits):
+    raw = np.frombuffer(bytes.fromhex(hexdata), dtype=np.uint8).reshape(shots, -1)
+    return np.unpackbits(raw, axis=1, bitorder='little')[:, :bits].astype(bool)
+
+
+def _converter(text, skip=False):
+    import stim_amd
+    return stim_amd.Circuit(text).compile_m2d_converter(skip_reference_sample=skip)
+
+
+def test_matches_reference_conversions(golden):
+    n = 0
+    for c in golden.load('m2d_vectors.json'):
+        text = c['circuit'] or golden.circuit(c['name'])
+        conv = _converter(text, c['skip_reference_sample'])
+        M, D, O = conv.num_measurements, conv.num_detectors, conv.num_observables
+        meas = _rows(c['measurements'], c['shots'], M)
+        want = _rows(c['expected'], c['shots'], D + O)
+        got = conv.convert(measurements=meas, append_observables=True)
+        np.testing.assert_array_equal(got, want, err_msg=c['name'])
+        packed_in = np.packbits(meas, axis=1, bitorder='little')
+        got_p = conv.convert(measurements=packed_in, append_observables=True, bit_packed=True)
+        np.testing.assert_array_equal(got_p, np.packbits(want, axis=1, bitorder='little'), err_msg=c['name'])
+        d, o = conv.convert(measurements=meas, separate_observables=True)
+        np.testing.assert_array_equal(d, want[:, :D])
+        np.testing.assert_array_equal(o, want[:, D:])
+        n += 1
+    assert n >= 90
+
+
+@pytest.mark.parametrize('name', ['color_d5_r3', 'surface_x_d5_r3'])
+def test_random_data_matches_oracle(golden, name):
+    text = golden.circuit(name)
+    conv = _converter(text)
+    rng = np.random.default_rng(5)
+    meas = rng.random((5000, conv.num_measurements)) < 0.3
+    import stim_amd
+    ref = stim_amd.Circuit(text).reference_sample()
+    np.testing.assert_array_equal(conv.convert(measurements=meas, append_observables=True), mo.convert(text, meas, ref))
+
+
+def test_device_rows_aligned_and_unaligned(golden):
+    import torch
+    from stim_amd import _native as N
+    text = golden.circuit('surface_d5_r5')
+    conv = _converter(text)
+    M, D, O = conv.num_measurements, conv.num_detectors, conv.num_observables
+    rng = np.random.default_rng(9)
+    meas = rng.random((777, M)) < 0.2
+    want = conv.convert(measurements=meas, append_observables=True, bit_packed=True)
+    packed = np.packbits(meas, axis=1, bitorder='little')
+    for stride in ((M + 7) // 8, 16 * ((M + 127) // 128), 16 * ((M + 127) // 128) + 3):
+        host = np.zeros((777, stride), np.uint8)
+        host[:, :packed.shape[1]] = packed
+        dev_in = torch.from_numpy(host).to('cuda:0')
+        dev_out = torch.zeros((777, want.shape[1]), dtype=torch.uint8, device='cuda:0')
+        N.check(N.lib().mfs_m2d_convert(conv._h, 777, dev_in.data_ptr(), stride,
+                                        N.MFS_BIT_PACKED | N.MFS_APPEND_OBSERVABLES | N.MFS_DEVICE_OUTPUT,
+                                        dev_out.data_ptr(), want.shape[1], None, 0))
+        np.testing.assert_array_equal(dev_out.cpu().numpy(), want, err_msg=str(stride))
+
+
+def test_fixed_faults_compose_with_measurement_sampler(golden):
+    """m2d(measurement rows under fixed faults) == the detector sampler's fixed-fault detection events."""
+    import stim_amd
+    fixtures = golden.load('fault_vectors.json')
+    n = 0
+    for f in fixtures[:12]:
+        text = f['circuit']
+        faults = [[tuple(x) for x in s['faults']] for s in f['sets']]
+        c = stim_amd.Circuit(text)
+        rows = c.compile_sampler(seed=1).sample_with_faults(faults)
+        got = c.compile_m2d_converter().convert(measurements=rows, append_observables=True)
+        want = np.array([[b == '1' for b in s['expected']] for s in f['sets']], dtype=bool)
+        np.testing.assert_array_equal(got, want, err_msg=f['name'])
+        n += len(faults)
+    assert n >= 100
+
+
+def test_sampled_measurements_convert_to_exact_rates(golden):
+    import stim_amd
+    from scipy.stats import norm
+    rates = np.array(golden.load('dem_rates.json')['surface_d3_r3'])
+    c = stim_amd.Circuit(golden.circuit('surface_d3_r3'))
+    shots = 1 << 20
+    rows = c.compile_sampler(seed=77).sample(shots, bit_packed=True)
+    events = c.compile_m2d_converter().convert(measurements=rows, append_observables=True, bit_packed=True)
+    bits = np.unpackbits(events, axis=1, bitorder='little')[:, :len(rates)]
+    counts = bits.sum(axis=0)
+    z = (counts - shots * rates) / np.sqrt(shots * rates * (1 - rates))
+    assert np.max(np.abs(z)) < abs(norm.ppf(1e-4 / (2 * len(z))))
+
+
+def test_d25_full_size_conversion_rates(golden):
+    """d=25, 25 rounds: 2^16 sampled measurement shots converted on the GPU; per-detector rates against
+    the exact rates of the circuit's error model."""
+    import gzip
+    import stim_amd
+    from scipy.stats import norm
+    from oracle import dem_oracle as do
+    with gzip.open(os.path.join(golden.dir, 'dems', 'surface_d25_r25_folded.dem.gz'), 'rt') as fh:
+        rates = do.exact_rates(do.parse(fh.read()))
+    c = stim_amd.Circuit(golden.circuit('surface_d25_r25'))
+    shots = 1 << 16
+    rows = c.compile_sampler(seed=5).sample(shots, bit_packed=True)
+    events = c.compile_m2d_converter().convert(measurements=rows, append_observables=True, bit_packed=True)
+    assert events.shape == (shots, (15601 + 7) // 8)
+    counts = np.zeros(15608, dtype=np.int64)
+    for b in range(8):
+        counts[b::8] += ((events >> b) & 1).sum(axis=0, dtype=np.int64)
+    counts = counts[:15601]
+    z = (counts - shots * rates) / np.sqrt(shots * rates * (1 - rates))
+    assert np.max(np.abs(z)) < abs(norm.ppf(1e-4 / (2 * len(z)))), np.argsort(-np.abs(z))[:5]
+
+
+def test_convert_file_formats(golden, tmp_path):
+    from stim_amd import formats
+    text = golden.circuit('rep_d7_r5')
+    conv = _converter(text)
+    M, D, O = conv.num_measurements, conv.num_detectors, conv.num_observables
+    rng = np.random.default_rng(3)
+    meas = rng.random((128, M)) < 0.1
+    want = conv.convert(measurements=meas, append_observables=True)
+    for fin in formats.FORMATS:
+        formats.write_rows(str(tmp_path / ('m.' + fin)), fin, meas)
+        for fout in ('01', 'b8', 'dets', 'hits', 'r8', 'ptb64'):
+            out = tmp_path / ('d.' + fout)
+            conv.convert_file(measurements_filepath=str(tmp_path / ('m.' + fin)), measurements_format=fin,
+                              detection_events_filepath=str(out), detection_events_format=fout,
+                              append_observables=True)
+            back = formats.read_rows(str(out), fout, D + O, [('D', D), ('L', O)])
+            np.testing.assert_array_equal(back, want, err_msg=f'{fin}->{fout}')
+    # Separate observables file; dets prefixes.
+    conv.convert_file(measurements_filepath=str(tmp_path / 'm.01'), detection_events_filepath=str(tmp_path / 'd2'),
+                      detection_events_format='dets', obs_out_filepath=str(tmp_path / 'o2'), obs_out_format='dets')
+    first = open(tmp_path / 'd2').read().split('\n')[0]
+    assert first == 'shot' + ''.join(' D%d' % k for k in np.flatnonzero(want[0, :D]))
+    assert open(tmp_path / 'o2').read().split('\n')[0] == 'shot' + ''.join(' L%d' % k for k in np.flatnonzero(want[0, D:]))
+
+
+def test_argument_errors(golden):
+    conv = _converter(golden.circuit('surface_d3_r3'))
+    ms = np.zeros((4, conv.num_measurements), dtype=bool)
+    with pytest.raises(ValueError, match='explicitly specify either separate'):
+        conv.convert(measurements=ms)
+    with pytest.raises(ValueError):
+        conv.convert(measurements=np.zeros((4, conv.num_measurements + 1), dtype=bool), append_observables=False)
+    with pytest.raises(ValueError):
+        conv.convert(measurements=ms, sweep_bits=np.ones((4, 1), dtype=bool), append_observables=False)
+    empty = _converter('')
+    d1 = empty.convert(measurements=np.zeros((50, 0), dtype=bool), append_observables=True)
+    d2, d3 = empty.convert(measurements=np.zeros((50, 0), dtype=bool), separate_observables=True)
+    assert d1.shape == d2.shape == d3.shape == (50, 0)
