@@ -10,6 +10,8 @@ separate observables) into HBM-resident output rows.
     python bench.py --workload serve        # request loop with continuous batching, p50/p99 latency
     python bench.py --workload clifford     # 10,000-qubit, 1,000-layer random-Clifford circuit, 2^16 shots
     python bench.py --workload dem          # detector-error-model sampler, the d=25 r=25 model, 2^20 shots
+    python bench.py --workload measure      # measurement sampler (compile_sampler), d=25 r=25, 2^20 shots
+    python bench.py --workload m2d          # measurement -> detection event conversion, d=25 r=25, 2^20 shots
 """
 
 import argparse
@@ -409,13 +411,161 @@ def dem(args):
     }
 
 
+def _cores():
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    return max(1, min(cores, 16))
+
+
+def _run_parallel(cmds):
+    procs = [subprocess.Popen(c, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, stdin=subprocess.DEVNULL)
+             for c in cmds]
+    t0 = time.perf_counter()
+    ok = all(p.wait() == 0 for p in procs)
+    return ok, time.perf_counter() - t0
+
+
+def measure_cpu_baseline():
+    """The reference CLI `stim sample` (measurement results), one process per core, bounded sample."""
+    if not (os.path.exists(REF_STIM) and os.access(REF_STIM, os.X_OK)):
+        return None
+    cores, shots = _cores(), 1 << 18
+    ok, wall = _run_parallel([[REF_STIM, 'sample', '--shots', str(shots), '--in', CIRCUIT, '--out_format', 'b8',
+                               '--out', '/dev/null', '--seed', str(k)] for k in range(cores)])
+    if not ok:
+        return None
+    return {'value': cores * shots / wall, 'unit': 'shots/s', 'cores': cores, 'kind': 'reference',
+            'sample': f'stim sample b8 to /dev/null, {cores} processes x {shots} shots (d=25 r=25 p=1e-3), '
+                      f'wall {wall:.2f}s'}
+
+
+def m2d_cpu_baseline():
+    """The reference CLI `stim m2d` on a file of 2^16 sampled d=25 measurement shots, one process per core."""
+    import tempfile
+    if not (os.path.exists(REF_STIM) and os.access(REF_STIM, os.X_OK)):
+        return None
+    cores, shots = _cores(), 1 << 16
+    with tempfile.TemporaryDirectory() as td:
+        meas = os.path.join(td, 'm.b8')
+        ok, _ = _run_parallel([[REF_STIM, 'sample', '--shots', str(shots), '--in', CIRCUIT, '--out_format', 'b8',
+                                '--out', meas, '--seed', '1']])
+        if not ok:
+            return None
+        ok, wall = _run_parallel([[REF_STIM, 'm2d', '--circuit', CIRCUIT, '--in', meas, '--in_format', 'b8',
+                                   '--out', '/dev/null', '--out_format', 'b8', '--append_observables']
+                                  for _ in range(cores)])
+    if not ok:
+        return None
+    return {'value': cores * shots / wall, 'unit': 'shots/s', 'cores': cores, 'kind': 'reference',
+            'sample': f'stim m2d b8 -> b8 (observables appended) to /dev/null, {cores} processes x {shots} shots '
+                      f'(d=25 r=25 measurement file), wall {wall:.2f}s'}
+
+
+def _timed(step, args):
+    import torch
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    return time.perf_counter() - t0
+
+
+def measure(args):
+    """Measurement sampler (every measurement result) on the d=25, 25-round circuit."""
+    import torch
+    import stim_amd
+    from stim_amd import _native as N
+    circuit = stim_amd.Circuit(open(CIRCUIT).read())
+    t0 = time.perf_counter()
+    sampler = circuit.compile_sampler(seed=2026)
+    compile_s = time.perf_counter() - t0
+    M, S = sampler.num_measurements, args.shots
+    row = (M + 7) // 8
+    out = torch.empty((S, row), dtype=torch.uint8, device='cuda:0')
+    flags = N.MFS_BIT_PACKED | N.MFS_DEVICE_OUTPUT
+    L = N.lib()
+
+    def step():
+        N.check(L.mfs_sample(sampler._h, S, flags, out.data_ptr(), row, None, 0))
+
+    wall = _timed(step, args)
+    kernel_ms = sampler.bench_device(S, warmup=3, iters=max(5, min(args.steps, 20)))
+    info = sampler.info()
+    b_alg = S * row
+    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    return {
+        'metric': 'measurement shots/s @ d=25 r=25 p=1e-3 (all 16,225 results per shot)',
+        'value': args.steps * S / wall, 'unit': 'shots/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': wall / args.steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'u32',
+        'data': 'synthetic: rotated surface code memory-Z d=25 r=25 p=1e-3 (stim gen text), seeded Philox noise and '
+                'frame randomization',
+        'config': {'workload': 'measure_d25_r25', 'measurements': M, 'shots_per_step': S, 'engine': 'frame',
+                   'tile_words': info['tile_words'], 'lds_bytes_per_tile': info['lds_bytes'],
+                   'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'kernel': 'frame_sim_kernel',
+                     'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
+        'reference_sample_and_compile_s': compile_s,
+        'cpu_baseline': None if args.no_cpu else measure_cpu_baseline(),
+    }
+
+
+def m2d(args):
+    """Measurement -> detection event conversion of HBM-resident d=25 measurement rows."""
+    import torch
+    import stim_amd
+    from stim_amd import _native as N
+    circuit = stim_amd.Circuit(open(CIRCUIT).read())
+    conv = circuit.compile_m2d_converter()
+    M, D, O, S = conv.num_measurements, conv.num_detectors, conv.num_observables, args.shots
+    in_stride = 16 * ((M + 127) // 128)
+    out_row = (D + O + 7) // 8
+    meas = torch.zeros((S, in_stride), dtype=torch.uint8, device='cuda:0')
+    # Realistic input: sampled measurement rows written straight into the padded device rows.
+    sampler = circuit.compile_sampler(seed=7)
+    N.check(N.lib().mfs_sample(sampler._h, S, N.MFS_BIT_PACKED | N.MFS_DEVICE_OUTPUT, meas.data_ptr(), in_stride,
+                               None, 0))
+    torch.cuda.synchronize()
+    out = torch.empty((S, out_row), dtype=torch.uint8, device='cuda:0')
+    flags = N.MFS_BIT_PACKED | N.MFS_DEVICE_OUTPUT | N.MFS_APPEND_OBSERVABLES
+    L = N.lib()
+
+    def step():
+        N.check(L.mfs_m2d_convert(conv._h, S, meas.data_ptr(), in_stride, flags, out.data_ptr(), out_row, None, 0))
+
+    wall = _timed(step, args)
+    kernel_ms = conv.bench_device(S, warmup=3, iters=max(10, min(args.steps, 50)))
+    b_alg = S * ((M + 7) // 8 + out_row)
+    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    return {
+        'metric': 'converted shots/s @ d=25 r=25 (measurements -> detection events + observables)',
+        'value': args.steps * S / wall, 'unit': 'shots/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
+        'ms_per_step': wall / args.steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'u32',
+        'data': 'synthetic: measurement rows sampled on the GPU from the d=25 r=25 p=1e-3 circuit, resident in HBM',
+        'config': {'workload': 'm2d_d25_r25', 'measurements': M, 'detectors': D, 'observables': O,
+                   'shots_per_step': S, 'tile_shots': 32, 'lds_bytes_per_tile': 128 * ((M + 31) // 32),
+                   'parallelism': 'single GPU'},
+        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'kernel': 'm2d_kernel',
+                     'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
+        'cpu_baseline': None if args.no_cpu else m2d_cpu_baseline(),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=50)
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--shots', type=int, default=1 << 20)
-    ap.add_argument('--workload', choices=['surface_d25', 'batch64', 'serve', 'clifford', 'dem'], default='surface_d25')
+    ap.add_argument('--workload', choices=['surface_d25', 'batch64', 'serve', 'clifford', 'dem', 'measure', 'm2d'], default='surface_d25')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--tile-words', type=int, default=0)
     ap.add_argument('--calibrate-copy', action='store_true',
@@ -436,6 +586,12 @@ def main():
         line = clifford(args)
     elif args.workload == 'dem':
         line = dem(args)
+    elif args.workload == 'measure':
+        args.steps = min(args.steps, 10)
+        args.warmup = min(args.warmup, 2)
+        line = measure(args)
+    elif args.workload == 'm2d':
+        line = m2d(args)
     else:
         from stim_amd import serve
         line = serve.bench(qps=args.qps, requests=args.requests)

@@ -2,8 +2,9 @@
 //
 // Mirrors the reference's CompiledMeasurementsToDetectionEventsConverter
 // (src/stim/simulators/measurements_to_detection_events.pybind.cc:78-159) without sweep data: the
-// circuit's detectors and observables become CSR lists of absolute measurement indices, each with
-// the parity of its measurements' reference-sample bits (semantics in m2d.h). Conversion runs on
+// circuit's detectors and observables become column records (up to two absolute measurement indices
+// inline, longer lists in an overflow table) with the parity of their measurements' reference-sample
+// bits (semantics and layout in m2d.h). Conversion runs on
 // the GPU (m2d_kernel.hip); host rows are staged through device buffers with 16-byte-aligned strides.
 #include "m2d.h"
 
@@ -13,12 +14,14 @@
 namespace sa {
 hipError_t launch_m2d(const M2dParams &P, hipStream_t stream);
 size_t m2d_lds_bytes(uint32_t blocks);
+size_t m2d_staged_lds_bytes(const M2dParams &P);
 }  // namespace sa
 
 struct mfs_m2d {
     uint32_t M = 0, D = 0, O = 0;
-    std::vector<uint32_t> off, terms, flip;
-    DevBuf d_off, d_terms, d_flip, d_in, d_out, d_obs, d_tmp;
+    std::vector<uint2> cols;
+    std::vector<uint32_t> ovf;
+    DevBuf d_cols, d_ovf, d_in, d_out, d_obs, d_tmp;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -48,9 +51,9 @@ struct mfs_m2d {
         P.obs = obs;
         P.obs_stride = obs_stride;
         P.shots = shots;
-        P.off = const_cast<DevBuf &>(d_off).as<uint32_t>();
-        P.terms = const_cast<DevBuf &>(d_terms).as<uint32_t>();
-        P.flip = const_cast<DevBuf &>(d_flip).as<uint32_t>();
+        P.cols = const_cast<DevBuf &>(d_cols).as<uint2>();
+        P.ovf = const_cast<DevBuf &>(d_ovf).as<uint32_t>();
+        P.stage = m2d_staged_lds_bytes(P) <= MAX_LDS ? 1u : 0u;
         return P;
     }
 
@@ -81,20 +84,15 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
                                         std::to_string(MAX_LDS / 4) + " measurements per shot");
         std::vector<uint8_t> ref(c->M, 0);
         if (!skip_reference_sample) ref = reference_sample(flat);
+        // Columns: detectors in order, then observables; terms are absolute measurement indices.
+        std::vector<std::vector<uint32_t>> col_terms;
         std::vector<std::vector<uint32_t>> obs_terms(c->O);
-        c->off.push_back(0);
-        std::vector<uint8_t> col_flip;
         uint64_t m = 0;
         for (const Instr *in : flat) {
             if (in->gate == Gate::DETECTOR) {
-                uint8_t f = 0;
-                for (const Target &t : in->targets) {
-                    const uint32_t idx = (uint32_t)(m - t.value);
-                    c->terms.push_back(idx);
-                    f ^= ref[idx];
-                }
-                c->off.push_back((uint32_t)c->terms.size());
-                col_flip.push_back(f);
+                std::vector<uint32_t> ts;
+                for (const Target &t : in->targets) ts.push_back((uint32_t)(m - t.value));
+                col_terms.push_back(std::move(ts));
             } else if (in->gate == Gate::OBSERVABLE_INCLUDE) {
                 for (const Target &t : in->targets) {
                     if (t.kind == TK::REC) obs_terms[(size_t)in->args[0]].push_back((uint32_t)(m - t.value));
@@ -102,27 +100,28 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
             }
             m += num_results(*in);
         }
-        for (uint32_t k = 0; k < c->O; k++) {
+        for (auto &ts : obs_terms) col_terms.push_back(std::move(ts));
+        for (const auto &ts : col_terms) {
             uint8_t f = 0;
-            for (uint32_t idx : obs_terms[k]) {
-                c->terms.push_back(idx);
-                f ^= ref[idx];
+            for (uint32_t idx : ts) f ^= ref[idx];
+            uint2 r;
+            if (ts.size() <= 2) {
+                r = make_uint2(((uint32_t)ts.size() << 28) | (ts.size() >= 1 ? ts[0] : 0u), ts.size() == 2 ? ts[1] : 0u);
+            } else {
+                r = make_uint2(3u << 28, (uint32_t)c->ovf.size());
+                c->ovf.push_back((uint32_t)ts.size());
+                c->ovf.insert(c->ovf.end(), ts.begin(), ts.end());
             }
-            c->off.push_back((uint32_t)c->terms.size());
-            col_flip.push_back(f);
-        }
-        c->flip.assign((col_flip.size() + 31) / 32 + 1, 0);
-        for (size_t i = 0; i < col_flip.size(); i++) {
-            if (col_flip[i]) c->flip[i >> 5] |= 1u << (i & 31);
+            if (f) r.x |= M2D_INVERT;
+            c->cols.push_back(r);
         }
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
             throw HipError("no HIP device available: the stim_amd converter runs only on the GPU");
         }
         check(hipStreamCreate(&c->stream), "stream");
-        mfs_sampler::upload(c->d_off, c->off, "m2d offsets");
-        mfs_sampler::upload(c->d_terms, c->terms, "m2d terms");
-        mfs_sampler::upload(c->d_flip, c->flip, "m2d reference parities");
+        mfs_sampler::upload(c->d_cols, c->cols, "m2d columns");
+        mfs_sampler::upload(c->d_ovf, c->ovf, "m2d overflow terms");
         *out = c.release();
     });
 }

@@ -6,6 +6,8 @@
 // the measurement records OBSERVABLE_INCLUDE(k) names (its Pauli targets add nothing).
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
 namespace sa {
@@ -16,17 +18,23 @@ struct M2dParams {
     uint32_t in_row_bytes;      // ceil(M / 8)
     uint32_t in_vec;            // rows are 16-byte aligned with 16-byte-multiple strides
     uint32_t blocks;            // ceil(M / 32)
-    uint32_t num_det;           // D: columns 0 .. D-1 of the CSR are detectors, D .. D+O-1 observables
-    uint8_t *out;               // bit-packed output rows: n_out columns starting at CSR column 0
+    uint32_t num_det;           // D: columns 0 .. D-1 are detectors, D .. D+O-1 observables
+    uint8_t *out;               // bit-packed output rows: n_out columns starting at column 0
     uint64_t out_stride;
     uint32_t n_out;
     uint32_t n_obs;             // separate observable rows (columns D .. D+n_obs-1), or 0
     uint8_t *obs;
     uint64_t obs_stride;
     uint64_t shots;
-    const uint32_t *off;        // CSR offsets over the D + O columns
-    const uint32_t *terms;      // absolute measurement indices
-    const uint32_t *flip;       // bit per column: XOR of its measurements' reference bits
+    uint32_t stage;             // 1: output rows are staged shot-major in LDS and written per tile
+    const uint2 *cols;          // column records over the D + O columns (below)
+    const uint32_t *ovf;        // overflow term lists: count, then absolute measurement indices
 };
+
+// Column record: x bits 0..27 = first measurement index, bits 28..29 = term count c (0..2; 3 = the
+// terms are in the overflow list at offset y), bit 31 = the column's reference parity (invert);
+// y = second measurement index (c = 2) or the overflow offset (c = 3).
+constexpr uint32_t M2D_INDEX_MASK = 0x0FFFFFFFu;
+constexpr uint32_t M2D_INVERT = 1u << 31;
 
 }  // namespace sa
