@@ -496,6 +496,7 @@ def measure(args):
     wall = _timed(step, args)
     kernel_ms = sampler.bench_device(S, warmup=3, iters=max(5, min(args.steps, 20)))
     info = sampler.info()
+    engine = sampler.engine_info()
     b_alg = S * row
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
     return {
@@ -505,11 +506,14 @@ def measure(args):
         'dtype': 'u32',
         'data': 'synthetic: rotated surface code memory-Z d=25 r=25 p=1e-3 (stim gen text), seeded Philox noise and '
                 'frame randomization',
-        'config': {'workload': 'measure_d25_r25', 'measurements': M, 'shots_per_step': S, 'engine': 'frame',
-                   'tile_words': info['tile_words'], 'lds_bytes_per_tile': info['lds_bytes'],
-                   'parallelism': 'single GPU'},
+        'config': {'workload': 'measure_d25_r25', 'measurements': M, 'shots_per_step': S,
+                   'engine': engine['engine'], 'symptom_columns': engine['columns'],
+                   'tile_shots': engine['tile_shots'] if engine['engine'] == 'symptom' else 32 * info['tile_words'],
+                   'lds_bytes_per_tile': engine['lds_bytes_per_tile'] if engine['engine'] == 'symptom'
+                   else info['lds_bytes'], 'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'kernel': 'frame_sim_kernel',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
+                     'kernel': 'sym_scan_kernel' if engine['engine'] == 'symptom' else 'frame_sim_kernel',
                      'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
         'reference_sample_and_compile_s': compile_s,
         'cpu_baseline': None if args.no_cpu else measure_cpu_baseline(),
