@@ -53,21 +53,6 @@ __device__ __forceinline__ Wd<T> &row(uint32_t *base, uint32_t r) {
     return *reinterpret_cast<Wd<T> *>(base + (size_t)r * T);
 }
 
-// Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel Random Numbers: As Easy as 1, 2, 3", SC'11).
-__device__ __forceinline__ uint4 philox(uint4 c, uint32_t k0, uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
-        uint32_t lo0 = 0xD2511F53u * c.x;
-        uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
-        uint32_t lo1 = 0xCD9E8D57u * c.z;
-        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    return c;
-}
-
 // Stream tags keep the counter spaces of different uses apart.
 constexpr uint32_t TAG_NOISE = 0u << 28;
 constexpr uint32_t TAG_RAND = 1u << 28;
@@ -78,7 +63,7 @@ struct Rng {
     uint32_t k0, k1;
     uint32_t tile;
     __device__ uint4 draw(uint32_t a, uint32_t b, uint32_t tag_and_id) const {
-        return philox(make_uint4(a, b, tile, tag_and_id), k0, k1);
+        return philox10(make_uint4(a, b, tile, tag_and_id), k0, k1);
     }
 };
 

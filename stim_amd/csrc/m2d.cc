@@ -14,7 +14,6 @@
 namespace sa {
 hipError_t launch_m2d(const M2dParams &P, hipStream_t stream);
 size_t m2d_lds_bytes(uint32_t blocks);
-size_t m2d_staged_lds_bytes(const M2dParams &P);
 }  // namespace sa
 
 struct mfs_m2d {
@@ -53,7 +52,6 @@ struct mfs_m2d {
         P.shots = shots;
         P.cols = const_cast<DevBuf &>(d_cols).as<uint2>();
         P.ovf = const_cast<DevBuf &>(d_ovf).as<uint32_t>();
-        P.stage = m2d_staged_lds_bytes(P) <= MAX_LDS ? 1u : 0u;
         return P;
     }
 
@@ -108,9 +106,11 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
             if (ts.size() <= 2) {
                 r = make_uint2(((uint32_t)ts.size() << 28) | (ts.size() >= 1 ? ts[0] : 0u), ts.size() == 2 ? ts[1] : 0u);
             } else {
-                r = make_uint2(3u << 28, (uint32_t)c->ovf.size());
-                c->ovf.push_back((uint32_t)ts.size());
+                if (ts.size() > M2D_INDEX_MASK || c->ovf.size() + ts.size() + 3 >= (1ull << 32))
+                    throw std::invalid_argument("too many measurement terms for the converter");
+                r = make_uint2((3u << 28) | (uint32_t)ts.size(), (uint32_t)c->ovf.size());
                 c->ovf.insert(c->ovf.end(), ts.begin(), ts.end());
+                while (c->ovf.size() % 4) c->ovf.push_back(m2d_zero_word(c->blocks()));
             }
             if (f) r.x |= M2D_INVERT;
             c->cols.push_back(r);

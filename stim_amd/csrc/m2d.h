@@ -26,15 +26,25 @@ struct M2dParams {
     uint8_t *obs;
     uint64_t obs_stride;
     uint64_t shots;
-    uint32_t stage;             // 1: output rows are staged shot-major in LDS and written per tile
     const uint2 *cols;          // column records over the D + O columns (below)
     const uint32_t *ovf;        // overflow term lists: count, then absolute measurement indices
 };
 
-// Column record: x bits 0..27 = first measurement index, bits 28..29 = term count c (0..2; 3 = the
-// terms are in the overflow list at offset y), bit 31 = the column's reference parity (invert);
-// y = second measurement index (c = 2) or the overflow offset (c = 3).
+// Column record: bits 28..29 of x = term count class c, bit 31 = the column's reference parity
+// (invert). c = 0..2: the column has c terms; x bits 0..27 = first measurement index, y = second
+// (unused fields hold 0). c = 3: x bits 0..27 = term count n, y = offset of the term list in the
+// overflow table (a multiple of 4); the list holds the n indices padded to a multiple of 4 with
+// the index of a word that is always zero (m2d_zero_word), so it is read 16 bytes at a time.
 constexpr uint32_t M2D_INDEX_MASK = 0x0FFFFFFFu;
 constexpr uint32_t M2D_INVERT = 1u << 31;
+
+// LDS words of a tile: measurement blocks rounded up to whole 16-byte load groups (4 blocks of 32
+// words), then 4 words of zeros.
+__host__ __device__ inline uint32_t m2d_word_blocks(uint32_t blocks) {
+    return (blocks + 3) / 4 * 4;
+}
+__host__ __device__ inline uint32_t m2d_zero_word(uint32_t blocks) {
+    return m2d_word_blocks(blocks) * 32;
+}
 
 }  // namespace sa

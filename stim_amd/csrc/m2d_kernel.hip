@@ -4,9 +4,12 @@
 // in flight per lane) and transposed 32x32 bits at a time across half-waves into measurement-major
 // words in LDS (word m = measurement m of the tile's 32 shots). Each output column is then the XOR
 // of its measurements' words, inverted when its reference parity is 1 (semantics in m2d.h), and the
-// column words are transposed back into shot-major rows, staged in LDS when they fit and written as
-// one contiguous block per tile. HBM-bound: each shot's measurement row is read once and its output
-// rows written once.
+// column words are transposed back into shot-major rows, staged in LDS a chunk of columns at a time
+// and copied to the rows with coalesced stores. HBM-bound: each shot's measurement row is read once
+// and its output rows written once.
+//
+// Work is laid out so that lane-dependent branches are rare: loads and LDS stores whose lanes fall
+// past the tile's shots or the row's end go to clamped addresses or padding that no output reads.
 #include <hip/hip_runtime.h>
 
 #include "bitops.h"
@@ -16,9 +19,13 @@ using namespace sa;
 
 namespace {
 
-constexpr int NT = 512;
+constexpr int NT = 1024;
 constexpr uint32_t HALVES = NT / 32;
-constexpr uint32_t UNROLL = 4;   // 16-byte loads in flight per lane
+constexpr uint32_t UNROLL = 4;                       // 16-byte loads in flight per lane
+constexpr uint32_t CHUNK_BLOCKS = 64;                // output columns staged per pass: 64 blocks of 32
+constexpr uint32_t CHUNK_BYTES = CHUNK_BLOCKS * 4;   // staged bytes per shot row
+constexpr uint32_t PITCH = CHUNK_BYTES + 4;          // LDS bytes per staged row: the pad spreads the
+                                                     // 32 rows of one column block over 32 banks
 
 // Bytes [4b, 4b + 4) of a row, zero past the row's end.
 __device__ __forceinline__ uint32_t load_word_bytes(const uint8_t *row, uint32_t row_bytes, uint32_t b) {
@@ -36,72 +43,53 @@ __device__ __forceinline__ uint32_t column_word(const M2dParams &P, const uint32
     uint32_t v;
     if (c == 3) {
         v = 0;
-        const uint32_t n = P.ovf[r.y];
-        for (uint32_t t = 0; t < n; t++) v ^= words[P.ovf[r.y + 1 + t]];
+        const uint32_t n = r.x & M2D_INDEX_MASK;
+        const uint4 *q = reinterpret_cast<const uint4 *>(P.ovf + r.y);
+        for (uint32_t t = 0; t < n; t += 4) {
+            const uint4 a = q[t / 4];
+            v ^= words[a.x] ^ words[a.y] ^ words[a.z] ^ words[a.w];
+        }
     } else {
-        v = c >= 1 ? words[r.x & M2D_INDEX_MASK] : 0u;
-        if (c == 2) v ^= words[r.y];
+        // Unused index fields hold 0, so both reads stay inside the tile's words.
+        const uint32_t w0 = words[r.x & M2D_INDEX_MASK], w1 = words[r.y];
+        v = (c >= 1 ? w0 : 0u) ^ (c == 2 ? w1 : 0u);
     }
     return (r.x & M2D_INVERT) ? ~v : v;
 }
 
-// Produces columns [c0, c0 + ncols) of the tile, transposed: lane l gets shot l's 4 bytes of each
-// 32-column block, which go either to the LDS row image img (pitch bytes per shot) or straight to
-// the shot's row in dst. CB column blocks per iteration keep that many record loads in flight.
-__device__ __forceinline__ void emit_columns(const M2dParams &P, const uint32_t *words, uint8_t *img, uint32_t pitch,
-                                             uint8_t *dst, uint64_t stride, uint32_t ncols, uint32_t c0,
-                                             uint32_t half, uint32_t lane, bool have, uint64_t shot) {
-    constexpr uint32_t CB = 4;
-    const uint32_t nb = (ncols + 31) / 32;
-    // Every lane of a half-wave runs the same iterations (the transpose shuffles within the half).
-    for (uint32_t b0 = half * CB; b0 < nb; b0 += HALVES * CB) {
+// Produces columns [c0, c0 + ncols) of the tile (ncols >= 1), transposed, into shot rows dst (bit 0 =
+// column c0), CHUNK_BLOCKS column blocks at a time: lane l of a half-wave gets shot l's 4 bytes of a
+// 32-column block and puts them in the chunk's LDS image (PITCH bytes per shot); the chunk is then
+// copied to the rows with coalesced byte stores (rows need no alignment). Image slots of lanes past
+// the tile's shots or blocks past the last are written but never copied; columns past the last
+// read as zero, so the padding bits of each row's last byte are 0.
+__device__ __forceinline__ void emit_columns(const M2dParams &P, const uint32_t *words, uint8_t *img, uint8_t *dst,
+                                             uint64_t stride, uint32_t ncols, uint32_t c0, uint32_t half,
+                                             uint32_t lane, uint64_t s0, uint32_t valid) {
+    constexpr uint32_t CB = CHUNK_BLOCKS / HALVES;
+    const uint32_t nb = (ncols + 31) / 32, pitch = (ncols + 7) / 8;
+    uint32_t *const slot = reinterpret_cast<uint32_t *>(img + lane * PITCH);
+    for (uint32_t cb0 = 0; cb0 < nb; cb0 += CHUNK_BLOCKS) {
+        // Every lane of a half-wave runs the same iterations (the transpose moves data within the half).
         uint2 r[CB];
 #pragma unroll
         for (uint32_t u = 0; u < CB; u++) {
-            const uint32_t c = (b0 + u) * 32 + lane;
-            r[u] = c < ncols ? P.cols[c0 + c] : make_uint2(0, 0);
+            const uint32_t c = (cb0 + half * CB + u) * 32 + lane;
+            const uint2 rec = P.cols[c0 + (c < ncols ? c : ncols - 1)];
+            r[u] = c < ncols ? rec : make_uint2(0, 0);
         }
 #pragma unroll
         for (uint32_t u = 0; u < CB; u++) {
-            const uint32_t b = b0 + u;
-            const uint32_t v = transpose32(column_word(P, words, r[u]), lane);
-            if (have && b < nb) {
-                const uint32_t rem = ncols - b * 32;
-                const uint32_t nbytes = rem >= 32 ? 4u : (rem + 7) / 8;
-                if (img) {
-                    uint8_t *q = img + lane * pitch + 4 * b;
-                    for (uint32_t k = 0; k < nbytes; k++) q[k] = (uint8_t)(v >> (8 * k));
-                } else {
-                    store_bytes(dst + shot * stride + 4 * (uint64_t)b, v, nbytes);
-                }
-            }
+            slot[half * CB + u] = transpose32(column_word(P, words, r[u]), lane);
         }
-    }
-}
-
-// Copies the tile's staged rows (valid rows of pitch bytes) to dst rows [s0, s0 + valid): one
-// contiguous block with 16-byte stores when the rows are packed back to back and aligned, else
-// coalesced bytes.
-__device__ __forceinline__ void flush_image(const uint8_t *img, uint32_t pitch, uint32_t valid, uint8_t *dst,
-                                            uint64_t stride, uint64_t s0) {
-    const uint32_t tid = threadIdx.x;
-    uint8_t *base = dst + s0 * stride;
-    const uint32_t total = valid * pitch;
-    if (stride == pitch && ((uintptr_t)base & 15) == 0) {
-        const uint32_t n16 = total / 16;
-        for (uint32_t i = tid; i < n16; i += NT) {
-            const uint4 v = *reinterpret_cast<const uint4 *>(img + 16 * i);
-            __builtin_nontemporal_store(v.x, reinterpret_cast<uint32_t *>(base + 16 * i));
-            __builtin_nontemporal_store(v.y, reinterpret_cast<uint32_t *>(base + 16 * i + 4));
-            __builtin_nontemporal_store(v.z, reinterpret_cast<uint32_t *>(base + 16 * i + 8));
-            __builtin_nontemporal_store(v.w, reinterpret_cast<uint32_t *>(base + 16 * i + 12));
+        __syncthreads();
+        const uint32_t off = 4 * cb0;
+        const uint32_t nbytes = pitch - off < CHUNK_BYTES ? pitch - off : CHUNK_BYTES;
+        for (uint32_t i = threadIdx.x; i < valid * CHUNK_BYTES; i += NT) {
+            const uint32_t row = i / CHUNK_BYTES, c = i % CHUNK_BYTES;
+            if (c < nbytes) __builtin_nontemporal_store(img[row * PITCH + c], dst + (s0 + row) * stride + off + c);
         }
-        for (uint32_t i = 16 * n16 + tid; i < total; i += NT) base[i] = img[i];
-    } else {
-        for (uint32_t i = tid; i < total; i += NT) {
-            const uint32_t r = i / pitch, c = i % pitch;
-            base[r * stride + c] = img[i];
-        }
+        __syncthreads();
     }
 }
 
@@ -112,28 +100,29 @@ __global__ __launch_bounds__(NT) void m2d_kernel(M2dParams P) {
     const uint64_t left = P.shots - s0;
     const uint32_t valid = left < 32 ? (uint32_t)left : 32u;
     const bool have = lane < valid;
-    const uint64_t shot = s0 + lane;
-    const uint8_t *row = P.in + (have ? shot : s0) * P.in_stride;
+    // Lanes past the tile's shots read the tile's first row; their words are never used.
+    const uint8_t *row = P.in + (s0 + (have ? lane : 0)) * P.in_stride;
     const uint32_t nblk = P.blocks;
+    const uint32_t zero = m2d_zero_word(nblk);
+    if (tid < 4) words[zero + tid] = 0;
 
     if (P.in_vec) {
-        // Groups of 4 blocks (16 bytes of each row); UNROLL groups per iteration.
+        // Groups of 4 blocks (16 bytes of each row); UNROLL groups per iteration. Loads past the last
+        // group re-read it; the word region holds whole groups.
         const uint32_t ngrp = (nblk + 3) / 4;
         for (uint32_t g0 = half * UNROLL; g0 < ngrp; g0 += HALVES * UNROLL) {
             uint4 v[UNROLL];
 #pragma unroll
             for (uint32_t u = 0; u < UNROLL; u++) {
-                const uint32_t g = g0 + u;
-                v[u] = (have && g < ngrp) ? *reinterpret_cast<const uint4 *>(row + 16 * (uint64_t)g) : make_uint4(0, 0, 0, 0);
+                const uint32_t g = g0 + u < ngrp ? g0 + u : ngrp - 1;
+                v[u] = *reinterpret_cast<const uint4 *>(row + 16 * (uint64_t)g);
             }
 #pragma unroll
             for (uint32_t u = 0; u < UNROLL; u++) {
-                const uint32_t xs[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+                if (g0 + u < ngrp) {
+                    const uint32_t xs[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
-                for (uint32_t k = 0; k < 4; k++) {
-                    const uint32_t x = transpose32(xs[k], lane);
-                    const uint32_t blk = 4 * (g0 + u) + k;
-                    if (blk < nblk) words[blk * 32 + lane] = x;
+                    for (uint32_t k = 0; k < 4; k++) words[(4 * (g0 + u) + k) * 32 + lane] = transpose32(xs[k], lane);
                 }
             }
         }
@@ -145,40 +134,24 @@ __global__ __launch_bounds__(NT) void m2d_kernel(M2dParams P) {
     }
     __syncthreads();
 
-    const uint32_t out_pitch = (P.n_out + 7) / 8, obs_pitch = (P.n_obs + 7) / 8;
-    uint8_t *img_out = nullptr, *img_obs = nullptr;
-    if (P.stage) {
-        img_out = reinterpret_cast<uint8_t *>(words + (size_t)nblk * 32);
-        img_obs = img_out + ((32 * out_pitch + 15) / 16) * 16;
-    }
-    emit_columns(P, words, img_out, out_pitch, P.out, P.out_stride, P.n_out, 0, half, lane, have, shot);
-    if (P.n_obs)
-        emit_columns(P, words, img_obs, obs_pitch, P.obs, P.obs_stride, P.n_obs, P.num_det, half, lane, have, shot);
-    if (P.stage) {
-        __syncthreads();
-        if (P.n_out) flush_image(img_out, out_pitch, valid, P.out, P.out_stride, s0);
-        if (P.n_obs) flush_image(img_obs, obs_pitch, valid, P.obs, P.obs_stride, s0);
-    }
+    uint8_t *const img = reinterpret_cast<uint8_t *>(words + zero + 4);
+    if (P.n_out) emit_columns(P, words, img, P.out, P.out_stride, P.n_out, 0, half, lane, s0, valid);
+    if (P.n_obs) emit_columns(P, words, img, P.obs, P.obs_stride, P.n_obs, P.num_det, half, lane, s0, valid);
 }
 
 }  // namespace
 
 namespace sa {
 
+// LDS of a launch: the tile's measurement words and zero words plus the staged output chunk.
 size_t m2d_lds_bytes(uint32_t blocks) {
-    return (size_t)blocks * 32 * 4;
-}
-
-// LDS of a staged launch: the measurement words plus both 32-shot output images (16-byte aligned).
-size_t m2d_staged_lds_bytes(const M2dParams &P) {
-    return m2d_lds_bytes(P.blocks) + ((32 * (size_t)((P.n_out + 7) / 8) + 15) / 16) * 16 +
-           ((32 * (size_t)((P.n_obs + 7) / 8) + 15) / 16) * 16;
+    return ((size_t)m2d_zero_word(blocks) + 4) * 4 + 32 * PITCH;
 }
 
 hipError_t launch_m2d(const M2dParams &P, hipStream_t stream) {
     const uint64_t tiles = (P.shots + 31) / 32;
     if (tiles == 0) return hipSuccess;
-    const size_t lds = P.stage ? m2d_staged_lds_bytes(P) : m2d_lds_bytes(P.blocks);
+    const size_t lds = m2d_lds_bytes(P.blocks);
     static size_t configured = 0;
     if (lds > configured) {
         const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&m2d_kernel),

@@ -12,6 +12,7 @@
 #include <algorithm>
 
 #include "program.h"
+#include "bitops.h"
 #include "symptom.h"
 
 using namespace sa;
@@ -21,20 +22,6 @@ namespace {
 constexpr int NT = 256;
 
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-
-__device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
-#pragma unroll
-    for (int r = 0; r < 10; r++) {
-        const uint32_t hi0 = __umulhi(0xD2511F53u, c.x);
-        const uint32_t lo0 = 0xD2511F53u * c.x;
-        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c.z);
-        const uint32_t lo1 = 0xCD9E8D57u * c.z;
-        c = make_uint4(hi1 ^ c.y ^ k0, lo1, hi0 ^ c.w ^ k1, lo0);
-        k0 += 0x9E3779B9u;
-        k1 += 0xBB67AE85u;
-    }
-    return c;
-}
 
 constexpr uint32_t TAG_SYM = 5u << 28;
 constexpr uint32_t TAG_SCAN = 6u << 28;
