@@ -44,11 +44,19 @@ __device__ __forceinline__ uint32_t xor_lanes(uint32_t x) {
 // halves shifted up by J; lanes with bit J keep their high halves and take the partner's high
 // halves shifted down. Both moves are one rotate of the partner's word (right by J, or by 32 - J),
 // merged with one bit-field select: keep = m or ~m, rot = 32 - J or J, per lane.
+// Bit-field select (keep & a) | (~keep & b) as one v_bfi_b32: written as the plain expression, the
+// compiler hoists keep and ~keep as separate loop invariants and emits an AND plus an AND-OR.
+__device__ __forceinline__ uint32_t bit_select(uint32_t keep, uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(keep), "v"(a), "v"(b));
+    return r;
+}
+
 template <int J>
 __device__ __forceinline__ uint32_t transpose_level(uint32_t x, uint32_t keep, uint32_t rot) {
     const uint32_t y = xor_lanes<J>(x);
     const uint32_t ys = __builtin_amdgcn_alignbit(y, y, rot);
-    return (x & keep) | (ys & ~keep);
+    return bit_select(keep, x, ys);
 }
 
 template <int J>

@@ -57,6 +57,9 @@ struct mfs_m2d {
 
     void launch(const M2dParams &P) {
         if ((P.shots + 31) / 32 >= (1ull << 31)) throw std::invalid_argument("too many shots in one conversion");
+        // The kernel addresses a tile's output rows with 32-bit offsets.
+        if (P.out_stride >= (1ull << 30) || P.obs_stride >= (1ull << 30))
+            throw std::invalid_argument("output row strides must be below 2^30 bytes");
         check(launch_m2d(P, stream), "m2d launch");
     }
 };

@@ -57,6 +57,33 @@ __device__ __forceinline__ uint32_t column_word(const M2dParams &P, const uint32
     return (r.x & M2D_INVERT) ? ~v : v;
 }
 
+// Copies the staged chunk to the rows: staged row r's bytes [0, nbytes) go to bytes [off, off + nbytes)
+// of shot row s0 + r, with coalesced byte stores (rows need no alignment). Thread t copies byte
+// t % CHUNK_BYTES of rows t / CHUNK_BYTES + k * ROWS_PER_PASS: per pass the row base is the same for
+// the whole workgroup and each thread's offset from it is fixed, so a store costs no address
+// arithmetic. The offset (3 rows + 255 bytes) fits 32 bits because strides are below 2^30.
+constexpr uint32_t ROWS_PER_PASS = NT / CHUNK_BYTES;
+static_assert(NT % CHUNK_BYTES == 0 && 32 % ROWS_PER_PASS == 0, "flush passes cover whole tiles");
+
+__device__ __forceinline__ void flush_chunk(const uint8_t *img, uint8_t *dst, uint64_t stride, uint32_t off,
+                                            uint32_t nbytes, uint64_t s0, uint32_t valid) {
+    const uint32_t c = threadIdx.x % CHUNK_BYTES, r0 = threadIdx.x / CHUNK_BYTES;
+    if (c >= nbytes) return;
+    const uint32_t voff = r0 * (uint32_t)stride + c;
+    const uint8_t *const src = img + r0 * PITCH + c;
+    uint8_t *const base = dst + s0 * stride + off;
+    if (valid == 32) {
+#pragma unroll
+        for (uint32_t k = 0; k < 32 / ROWS_PER_PASS; k++) {
+            __builtin_nontemporal_store(src[k * ROWS_PER_PASS * PITCH], base + k * ROWS_PER_PASS * stride + voff);
+        }
+    } else {
+        for (uint32_t k = 0; r0 + k * ROWS_PER_PASS < valid; k++) {
+            __builtin_nontemporal_store(src[k * ROWS_PER_PASS * PITCH], base + k * ROWS_PER_PASS * stride + voff);
+        }
+    }
+}
+
 // Produces columns [c0, c0 + ncols) of the tile (ncols >= 1), transposed, into shot rows dst (bit 0 =
 // column c0), CHUNK_BLOCKS column blocks at a time: lane l of a half-wave gets shot l's 4 bytes of a
 // 32-column block and puts them in the chunk's LDS image (PITCH bytes per shot); the chunk is then
@@ -85,10 +112,7 @@ __device__ __forceinline__ void emit_columns(const M2dParams &P, const uint32_t 
         __syncthreads();
         const uint32_t off = 4 * cb0;
         const uint32_t nbytes = pitch - off < CHUNK_BYTES ? pitch - off : CHUNK_BYTES;
-        for (uint32_t i = threadIdx.x; i < valid * CHUNK_BYTES; i += NT) {
-            const uint32_t row = i / CHUNK_BYTES, c = i % CHUNK_BYTES;
-            if (c < nbytes) __builtin_nontemporal_store(img[row * PITCH + c], dst + (s0 + row) * stride + off + c);
-        }
+        flush_chunk(img, dst, stride, off, nbytes, s0, valid);
         __syncthreads();
     }
 }
