@@ -442,11 +442,11 @@ def measure_cpu_baseline():
 
 
 def m2d_cpu_baseline():
-    """The reference CLI `stim m2d` on a file of 2^16 sampled d=25 measurement shots, one process per core."""
+    """The reference CLI `stim m2d` on a file of 2^19 sampled d=25 measurement shots, one process per core."""
     import tempfile
     if not (os.path.exists(REF_STIM) and os.access(REF_STIM, os.X_OK)):
         return None
-    cores, shots = _cores(), 1 << 16
+    cores, shots = _cores(), 1 << 19
     with tempfile.TemporaryDirectory() as td:
         meas = os.path.join(td, 'm.b8')
         ok, _ = _run_parallel([[REF_STIM, 'sample', '--shots', str(shots), '--in', CIRCUIT, '--out_format', 'b8',

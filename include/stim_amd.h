@@ -84,7 +84,10 @@ int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[3]);
  * event rows (with observables appended when MFS_APPEND_OBSERVABLES) and, with
  * MFS_SEPARATE_OBSERVABLES, observable rows; MFS_BIT_PACKED selects packed outputs (else one byte per
  * entry); MFS_DEVICE_OUTPUT means every pointer is device memory. Each detector (observable) is the
- * XOR of its measurements, inverted when the XOR of their reference bits is 1.
+ * XOR of its measurements, inverted when the XOR of their reference bits is 1. Packed device
+ * measurement rows with 16-byte-aligned address and stride >= 16 * ceil(M / 128) are read in place,
+ * 16 bytes at a time, so that buffer must span shots * meas_stride bytes. Output strides must be
+ * below 2^30 bytes.
  * Reference: CompiledMeasurementsToDetectionEventsConverter.convert
  * (measurements_to_detection_events.pybind.cc:78-157) and measurements_to_detection_events_helper
  * (src/stim/simulators/measurements_to_detection_events.inl:29-131). */

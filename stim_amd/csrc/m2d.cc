@@ -80,9 +80,12 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
         c->M = (uint32_t)st.num_measurements;
         c->D = (uint32_t)st.num_detectors;
         c->O = (uint32_t)st.num_observables;
-        if (m2d_lds_bytes(c->blocks()) > MAX_LDS)
+        if (m2d_lds_bytes(c->blocks()) > MAX_LDS) {
+            uint32_t max_blocks = 0;
+            while (m2d_lds_bytes(max_blocks + 1) <= MAX_LDS) max_blocks++;
             throw std::invalid_argument("the converter keeps a tile's measurements in LDS: at most " +
-                                        std::to_string(MAX_LDS / 4) + " measurements per shot");
+                                        std::to_string((uint64_t)max_blocks * 32) + " measurements per shot");
+        }
         std::vector<uint8_t> ref(c->M, 0);
         if (!skip_reference_sample) ref = reference_sample(flat);
         // Columns: detectors in order, then observables; terms are absolute measurement indices.

@@ -169,3 +169,16 @@ def test_argument_errors(golden):
     d1 = empty.convert(measurements=np.zeros((50, 0), dtype=bool), append_observables=True)
     d2, d3 = empty.convert(measurements=np.zeros((50, 0), dtype=bool), separate_observables=True)
     assert d1.shape == d2.shape == d3.shape == (50, 0)
+
+
+def test_columns_without_measurements():
+    """Detectors and observables over no measurements (a circuit with no measurements at all) are
+    constant zero columns, in every output layout and with a partial last tile."""
+    conv = _converter('DETECTOR\nDETECTOR(1, 2)\nOBSERVABLE_INCLUDE(0)\nOBSERVABLE_INCLUDE(2)')
+    ms = np.zeros((45, 0), dtype=bool)
+    d = conv.convert(measurements=ms, append_observables=True)
+    assert d.shape == (45, 5) and not d.any()
+    p = conv.convert(measurements=np.zeros((45, 0), dtype=np.uint8), bit_packed=True, append_observables=True)
+    assert p.shape == (45, 1) and not p.any()
+    d, o = conv.convert(measurements=ms, separate_observables=True)
+    assert d.shape == (45, 2) and o.shape == (45, 3) and not d.any() and not o.any()
