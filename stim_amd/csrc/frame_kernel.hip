@@ -22,6 +22,7 @@
 #include <algorithm>
 
 #include "bitops.h"
+#include "launch_util.h"
 #include "program.h"
 
 using namespace sa;
@@ -591,8 +592,11 @@ hipError_t launch_frame_kernel(int T, const DevProgram &prog, const LaunchParams
     if (num_tiles == 0) return hipSuccess;
     dim3 grid(num_tiles), block(NT);
 #define STIM_AMD_LAUNCH(TW)                                                                                     \
-    (void)hipFuncSetAttribute((const void *)frame_sim_kernel<TW>, hipFuncAttributeMaxDynamicSharedMemorySize,  \
-                              (int)lds);                                                                         \
+    {                                                                                                           \
+        static size_t reserved = 0;                                                                             \
+        const hipError_t e = reserve_lds((const void *)frame_sim_kernel<TW>, lds, reserved);                    \
+        if (e != hipSuccess) return e;                                                                          \
+    }                                                                                                           \
     hipLaunchKernelGGL(frame_sim_kernel<TW>, grid, block, lds, stream, prog.blob, prog.ntgt, prog.otab, P);
     switch (T) {
         case 1: STIM_AMD_LAUNCH(1) break;

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include "bitops.h"
+#include "launch_util.h"
 #include "m2d.h"
 
 using namespace sa;
@@ -176,13 +177,9 @@ hipError_t launch_m2d(const M2dParams &P, hipStream_t stream) {
     const uint64_t tiles = (P.shots + 31) / 32;
     if (tiles == 0) return hipSuccess;
     const size_t lds = m2d_lds_bytes(P.blocks);
-    static size_t configured = 0;
-    if (lds > configured) {
-        const hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void *>(&m2d_kernel),
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-        configured = lds;
-    }
+    static size_t reserved = 0;
+    const hipError_t e = reserve_lds(reinterpret_cast<const void *>(&m2d_kernel), lds, reserved);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(m2d_kernel, dim3((uint32_t)tiles), dim3(NT), lds, stream, P);
     return hipGetLastError();
 }

@@ -86,6 +86,9 @@ struct DevEpoch {
 //                           to the blob start.
 // Each blob is copied into one of two LDS staging buffers while the previous epoch runs, so the
 // op loop reads headers and targets from LDS only.
+// LDS one workgroup may use on gfx950.
+constexpr uint32_t LDS_PER_WORKGROUP = 160 * 1024;
+
 constexpr uint32_t BLOB_HEADER = 8;
 constexpr uint32_t STAGE_WORDS = 1024;  // staged words per epoch blob (4 KB; double-buffered)
 constexpr uint32_t OP_WORDS_MAX = STAGE_WORDS - 2 * BLOB_HEADER;  // words of one frame op

@@ -87,7 +87,7 @@ inline uint64_t splitmix64(uint64_t x) {
     return x ^ (x >> 31);
 }
 
-constexpr size_t MAX_LDS = 160 * 1024;
+constexpr size_t MAX_LDS = LDS_PER_WORKGROUP;
 constexpr size_t HOST_BATCH_BYTES = (size_t)1 << 30;
 
 // Records the message of a failed C-ABI call (read back by mfs_last_error) and returns its kind.

@@ -13,6 +13,7 @@
 
 #include "program.h"
 #include "bitops.h"
+#include "launch_util.h"
 #include "symptom.h"
 
 using namespace sa;
@@ -717,14 +718,18 @@ namespace sa {
 
 hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream) {
     if (!tiles) return hipSuccess;
-    (void)hipFuncSetAttribute((const void *)sym_sample_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static size_t reserved = 0;
+    const hipError_t e = reserve_lds((const void *)sym_sample_kernel, lds, reserved);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sym_sample_kernel, dim3(tiles), dim3(NT), lds, stream, P);
     return hipGetLastError();
 }
 
 hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream) {
     if (!tiles) return hipSuccess;
-    (void)hipFuncSetAttribute((const void *)sym_scan_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static size_t reserved = 0;
+    const hipError_t e = reserve_lds((const void *)sym_scan_kernel, lds, reserved);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sym_scan_kernel, dim3(tiles), dim3(NT), lds, stream, P);
     return hipGetLastError();
 }
@@ -745,10 +750,7 @@ hipError_t launch_sym_batch(SymDevParams *jobs, const uint64_t *prefix, const ui
 // workgroups with that much LDS fit on one CU. Every launch size is announced through here first.
 int sym_batch_blocks_per_cu(size_t lds) {
     static size_t reserved = 0;
-    if (lds > reserved) {
-        (void)hipFuncSetAttribute((const void *)sym_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        reserved = lds;
-    }
+    (void)reserve_lds((const void *)sym_batch_kernel, lds, reserved);
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sym_batch_kernel, NT, lds) != hipSuccess) return 1;
     return n > 0 ? n : 1;
@@ -757,7 +759,9 @@ int sym_batch_blocks_per_cu(size_t lds) {
 hipError_t launch_sym_fixed(const SymDevParams &P, const uint64_t *off, const uint32_t *cols, uint32_t tiles,
                             size_t lds, hipStream_t stream) {
     if (!tiles) return hipSuccess;
-    (void)hipFuncSetAttribute((const void *)sym_fixed_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    static size_t reserved = 0;
+    const hipError_t e = reserve_lds((const void *)sym_fixed_kernel, lds, reserved);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sym_fixed_kernel, dim3(tiles), dim3(NT), lds, stream, P, off, cols);
     return hipGetLastError();
 }
