@@ -16,6 +16,10 @@ namespace sa {
 // there each output word's two XORs (with the other word and the round key) are one three-input
 // v_bitop3_b32 (truth table 0x96 = XOR3).
 __device__ __forceinline__ uint4 philox10(uint4 c, uint32_t k0, uint32_t k1) {
+    // The keys are uniform across the wave. Hiding their values here makes the compiler form the
+    // round keys per call with scalar adds, instead of hoisting all 20 out of the caller's loops,
+    // where they outlive the scalar registers and spill into vector-register lanes.
+    asm volatile("" : "+s"(k0), "+s"(k1));
 #pragma unroll
     for (int r = 0; r < 4; r++) {
         const uint64_t p0 = (uint64_t)0xD2511F53u * c.x;
