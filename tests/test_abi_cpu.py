@@ -62,6 +62,8 @@ def test_d25_program_fits_lds(golden):
     ('DEPOLARIZE2(0.1) 0 0\n', ValueError),
     ('PAULI_CHANNEL_1(0.5, 0.5, 0.5) 0\n', ValueError),
     ('DETECTOR 0\n', ValueError),
+    # frame_simulator_util.test.cc:46-48 (bad_detector): a bare record target is not an instruction.
+    ('rec[-1]\n', ValueError),
 ])
 def test_parse_errors_map_to_reference_exceptions(text, exc):
     with pytest.raises(exc):
