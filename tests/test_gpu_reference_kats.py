@@ -76,3 +76,73 @@ OBSERVABLE_INCLUDE(2) rec[-1]
     want = (np.arange(125 * 8) < 399).astype(np.uint8)
     assert (bits == want[None, :]).all()
     assert (obs == 0x3).all()
+
+
+def _with_detectors(text, n):
+    """The circuit plus one detector per measurement result, in result order."""
+    return text + ''.join(f'DETECTOR rec[-{n - i}]\n' for i in range(n))
+
+
+CHAINS = [
+    ((0, 0, 0), '0000'), ((1, 0, 0), '1100'), ((0, 1, 0), '0110'), ((0, 0, 1), '0011'),
+    ((1, 1, 0), '1100'), ((1, 1, 1), '1100'),
+]
+
+
+@pytest.mark.parametrize('probs,want', CHAINS)
+def test_correlated_error_chains(probs, want):
+    """frame_simulator.test.cc:425-545: an ELSE_CORRELATED_ERROR chain applies its first link that
+    fires and skips the rest. Checked through the measurement sampler and, with one detector per
+    result, through the detector sampler on both engines."""
+    import stim_amd
+    a, b, c = probs
+    text = (f'CORRELATED_ERROR({a}) X0 X1\nELSE_CORRELATED_ERROR({b}) X1 X2\n'
+            f'ELSE_CORRELATED_ERROR({c}) X2 X3\nM 0 1 2 3\n')
+    expect = np.array([ch == '1' for ch in want])
+    m = stim_amd.Circuit(text).compile_sampler(seed=1).sample(8)
+    assert (m == expect[None, :]).all()
+    for engine in ENGINES:
+        d = _sampler(_with_detectors(text, 4), engine).sample(8)
+        assert (d == expect[None, :]).all(), engine
+
+
+def test_correlated_error_chain_then_new_chain():
+    """frame_simulator.test.cc:535-545: a CORRELATED_ERROR after a chain starts a new chain."""
+    import stim_amd
+    text = ('CORRELATED_ERROR(1) X0 X1\nELSE_CORRELATED_ERROR(1) X1 X2\nELSE_CORRELATED_ERROR(1) X2 X3\n'
+            'CORRELATED_ERROR(1) X3 X4\nM 0 1 2 3 4\n')
+    expect = np.array([True, True, False, True, True])
+    assert (stim_amd.Circuit(text).compile_sampler(seed=1).sample(8) == expect[None, :]).all()
+    for engine in ENGINES:
+        assert (_sampler(_with_detectors(text, 5), engine).sample(8) == expect[None, :]).all(), engine
+
+
+def test_mpad_results():
+    """frame_simulator.test.cc:1489-1500: MPAD 0 1 records 0 then 1 on every shot."""
+    import stim_amd
+    m = stim_amd.Circuit('MPAD 0 1\n').compile_sampler(seed=1).sample(100)
+    assert m.shape == (100, 2) and not m[:, 0].any() and m[:, 1].all()
+
+
+def test_pair_measurements_after_matching_resets():
+    """frame_simulator.test.cc:1502-1518: MXX after RX, MYY after RY, MZZ after RZ all give 0."""
+    import stim_amd
+    text = 'RX 0 1\nMXX 0 1\nRY 0 1\nMYY 0 1\nRZ 0 1\nMZZ 0 1\n'
+    m = stim_amd.Circuit(text).compile_sampler(seed=1).sample(100)
+    assert m.shape == (100, 3) and not m.any()
+
+
+def test_pair_measurement_inversions():
+    """frame_simulator.test.cc:1520-1542: inverted targets of MXX/MYY/MZZ invert the result, and the
+    three products of one pair multiply to -1 (result 8 = 1 ^ result 0 ^ result 4)."""
+    import stim_amd
+    text = 'MXX 0 1 0 !1 !0 1 !0 !1\nMYY 0 1 0 !1 !0 1 !0 !1\nMZZ 0 1 0 !1 !0 1 !0 !1\n'
+    m = stim_amd.Circuit(text).compile_sampler(seed=1).sample(100)
+    assert m.shape == (100, 12)
+    for base in (0, 4, 8):
+        assert (m[:, base + 1] == ~m[:, base]).all()
+        assert (m[:, base + 2] == ~m[:, base]).all()
+        assert (m[:, base + 3] == m[:, base]).all()
+    assert (m[:, 8] == (True ^ m[:, 0] ^ m[:, 4])).all()
+    # The X and Y products are random: both values occur.
+    assert 0 < m[:, 0].sum() < 100 and 0 < m[:, 4].sum() < 100
