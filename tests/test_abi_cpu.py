@@ -64,6 +64,13 @@ def test_d25_program_fits_lds(golden):
     ('DETECTOR 0\n', ValueError),
     # frame_simulator_util.test.cc:46-48 (bad_detector): a bare record target is not an instruction.
     ('rec[-1]\n', ValueError),
+    # frame_simulator.test.cc:573-643 (quantum_cannot_control_classical): a record target may only be
+    # the control of a classically controlled Pauli.
+    ('M 0\nCNOT 1 rec[-1]\n', ValueError),
+    ('M 0\nCY 1 rec[-1]\n', ValueError),
+    ('M 0\nYCZ rec[-1] 1\n', ValueError),
+    ('M 0\nXCZ rec[-1] 1\n', ValueError),
+    ('M 0\nSWAP 1 rec[-1]\n', ValueError),
 ])
 def test_parse_errors_map_to_reference_exceptions(text, exc):
     with pytest.raises(exc):

@@ -137,3 +137,44 @@ def test_fixture_faults_cover_every_channel_kind(golden):
     for name in ['X_ERROR', 'Y_ERROR', 'Z_ERROR', 'DEPOLARIZE1', 'DEPOLARIZE2', 'PAULI_CHANNEL_1', 'PAULI_CHANNEL_2',
                  'E', 'ELSE_CORRELATED_ERROR', 'HERALDED_ERASE', 'HERALDED_PAULI_CHANNEL_1', 'MPP', 'MR', 'M']:
         assert name in kinds, name
+
+
+# frame_simulator.test.cc:645-830 (classical_can_control_quantum, classical_controls): flips of the two
+# results relative to the noiseless circuit, written as one detector per result.
+CLASSICAL_CONTROL = [
+    ('X_ERROR(1) 0\nM !0\nCX rec[-1] 1\nM 1\n', (1, 1)),
+    ('X_ERROR(1) 0\nM !0\nCY rec[-1] 1\nM 1\n', (1, 1)),
+    ('X_ERROR(1) 0\nM !0\nXCZ 1 rec[-1]\nM 1\n', (1, 1)),
+    ('X_ERROR(1) 0\nM !0\nYCZ 1 rec[-1]\nM 1\n', (1, 1)),
+    ('M 0\nCX rec[-1] 1\nM 1\n', (0, 0)),
+    ('M !0\nCX rec[-1] 1\nM 1\n', (0, 0)),
+    ('X_ERROR(1) 0\nM 0\nCX rec[-1] 1\nM 1\n', (1, 1)),
+    ('X_ERROR(1) 0\nM 0\nH 1\nCZ rec[-1] 1\nH 1\nM 1\n', (1, 1)),
+    ('X_ERROR(1) 0\nM 0\nCY rec[-1] 1\nM 1\n', (1, 1)),
+]
+
+
+@pytest.mark.parametrize('text,want', CLASSICAL_CONTROL)
+def test_oracle_classical_control_known_answers(text, want):
+    orc = fo.FrameOracle(text + 'DETECTOR rec[-2]\nDETECTOR rec[-1]\n')
+    dets, _ = orc.run(16, rng=np.random.default_rng(0))
+    assert (dets == np.array(want, dtype=bool)[None, :]).all()
+
+
+def test_oracle_classical_control_copies_a_random_result():
+    """frame_simulator.test.cc:726-741: X_ERROR(0.5) then CX rec[-1] 1: both results agree per shot."""
+    orc = fo.FrameOracle('X_ERROR(0.5) 0\nM 0\nCX rec[-1] 1\nM 1\nDETECTOR rec[-2]\nDETECTOR rec[-1]\n')
+    dets, _ = orc.run(1000, rng=np.random.default_rng(1))
+    assert (dets[:, 0] == dets[:, 1]).all() and 400 < dets[:, 0].sum() < 600
+
+
+@pytest.mark.parametrize('reset,zero,pair', [('RZ', 2, (0, 1)), ('RY', 1, (0, 2)), ('RX', 0, (1, 2))])
+def test_randomized_oracle_observable_pauli_targets(reset, zero, pair):
+    """frame_simulator.test.cc:1685-1743: after a reset, the observable on the reset basis's Pauli
+    never flips and the other two are random and equal (shot by shot, which implies the reference's
+    equal counts)."""
+    text = f'{reset} 0\nOBSERVABLE_INCLUDE(0) X0\nOBSERVABLE_INCLUDE(1) Y0\nOBSERVABLE_INCLUDE(2) Z0\n'
+    _, obs = fo.FrameOracle(text, randomize=True).run(1024, rng=np.random.default_rng(2))
+    assert not obs[:, zero].any()
+    a, b = pair
+    assert (obs[:, a] == obs[:, b]).all() and 300 < obs[:, a].sum() < 700
