@@ -133,3 +133,19 @@ def test_library_parse_counts_match_oracle_d25(golden):
     assert (nd, no, len(errors)) == (15600, 1, 356321)
     rc, out = _stats(text)
     assert rc == 0 and out == [nd, no, len(errors)]
+
+
+def test_reference_basic_probability_model_rates():
+    """dem_sampler.test.cc:50-75 (resample_basic_probabilities): the model's detectors fire at exactly
+    0, 1/4, 1/2, 3/4, 1, 1 (the '^' separator still flips both D4 and D5) and L0 with D1."""
+    from oracle import dem_oracle as do
+    model = do.parse('error(0) D0\nerror(0.25) D1 L0\nerror(0.5) D2\nerror(0.75) D3\nerror(1) D4 ^ D5\n')
+    nd, no, _ = model
+    assert (nd, no) == (6, 1)
+    np.testing.assert_allclose(do.exact_rates(model), [0, 0.25, 0.5, 0.75, 1, 1, 0.25], atol=1e-12)
+    # Replaying mechanism 1 flips D1 and L0; mechanism 4 flips D4 and D5.
+    dets, obs = do.replay(model, [[1], [4], [0, 1, 2, 3, 4]])
+    assert dets.tolist() == [[False, True, False, False, False, False],
+                             [False, False, False, False, True, True],
+                             [True] * 6]
+    assert obs.tolist() == [[True], [False], [True]]

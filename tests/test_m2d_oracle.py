@@ -5,6 +5,7 @@ sample). A planted mistake in the oracle (dropping the reference parity) must be
 """
 
 import numpy as np
+import pytest
 
 from oracle import m2d_oracle as mo
 
@@ -41,3 +42,43 @@ def test_planted_mistake_is_caught(golden):
         want = _rows(c['expected'], c['shots'], ncols)
         caught += not np.array_equal(mo.convert(text, meas, np.zeros(m, bool)), want)
     assert caught >= 10
+
+
+# measurements_to_detection_events.test.cc:25-143 (single_detector_no_sweep_data): shot-0 measurement
+# bits, circuit, expected detector 0. Conversion against the circuit's own reference sample (the host
+# tableau, no GPU needed) checks the oracle and the reference sample together.
+SINGLE_DETECTOR = [
+    ((0,), 'M 0\nDETECTOR rec[-1]\n', 0),
+    ((0,), 'M !0\nDETECTOR rec[-1]\n', 1),
+    ((0,), 'X 0\nM 0\nDETECTOR rec[-1]\n', 1),
+    ((1,), 'M 0\nDETECTOR rec[-1]\n', 1),
+    ((1,), 'M !0\nDETECTOR rec[-1]\n', 0),
+    ((1,), 'X 0\nM 0\nDETECTOR rec[-1]\n', 0),
+    ((1, 0), 'M 0 1\nDETECTOR rec[-1]\n', 0),
+    ((1, 0), 'M 0 1\nDETECTOR rec[-2]\n', 1),
+    ((1, 0), 'M 0 1\nDETECTOR rec[-1] rec[-2]\n', 1),
+    ((1, 0), 'M 0 1\nDETECTOR\n', 0),
+]
+
+
+@pytest.mark.parametrize('meas,text,want', SINGLE_DETECTOR)
+def test_reference_single_detector_answers(meas, text, want):
+    import stim_amd
+    ref = np.asarray(stim_amd.Circuit(text).reference_sample(), dtype=bool)
+    out = mo.convert(text, np.array([meas], dtype=bool), ref)
+    assert out.shape == (1, 1) and bool(out[0, 0]) == bool(want)
+
+
+def test_reference_appended_observable_answers():
+    """measurements_to_detection_events.test.cc:299-371 (append_observables): all-zero measurements;
+    observable 9 lands at column D + 9 when appended."""
+    import stim_amd
+    text = 'X 0\nM 0\nOBSERVABLE_INCLUDE(9) rec[-1]\n'
+    ref = np.asarray(stim_amd.Circuit(text).reference_sample(), dtype=bool)
+    out = mo.convert(text, np.zeros((1, 1), dtype=bool), ref)
+    assert out.shape == (1, 10) and out[0, 9] and not out[0, :9].any()
+    text = 'X 0\nM 0\nDETECTOR rec[-1]\nOBSERVABLE_INCLUDE(9) rec[-1]\nDETECTOR rec[-1]\n'
+    ref = np.asarray(stim_amd.Circuit(text).reference_sample(), dtype=bool)
+    out = mo.convert(text, np.zeros((1, 1), dtype=bool), ref)
+    assert out.shape == (1, 12)
+    assert out[0, 0] and out[0, 1] and not out[0, 2:11].any() and out[0, 11]
