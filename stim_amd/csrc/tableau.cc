@@ -570,7 +570,19 @@ struct Runner {
 
 std::vector<uint8_t> reference_sample(const std::vector<const Instr *> &flat) {
     const Stats st = compute_stats(flat);
-    Runner run((uint32_t)std::max<uint64_t>(st.num_qubits, 1));
+    if (st.num_measurements == 0) return {};
+    // The tableau holds 2n + 1 rows of n X bits and n Z bits (about n^2 / 2 bytes), as the reference's
+    // TableauSimulator does; circuits whose tableau would exceed the limit are refused up front rather
+    // than failing inside the allocator.
+    constexpr double MAX_TABLEAU_BYTES = 16.0 * (1ull << 30);
+    const uint64_t n = std::max<uint64_t>(st.num_qubits, 1);
+    const double bytes = 2.0 * (2.0 * (double)n + 1.0) * (double)((n + 63) / 64) * 8.0;
+    if (n >= (1ull << 31) || bytes > MAX_TABLEAU_BYTES) {
+        throw std::runtime_error("the reference sample of a circuit on " + std::to_string(n) +
+                                 " qubits needs a stabilizer tableau of " + std::to_string((uint64_t)(bytes / (1 << 20))) +
+                                 " MiB (at most 16 GiB); skip the reference sample or pass one");
+    }
+    Runner run((uint32_t)n);
     run.rec.reserve(st.num_measurements);
     for (const Instr *in : flat) run.step(*in);
     return run.rec;

@@ -16,8 +16,10 @@
 static void run_circuit(const std::string &t, bool full) {
     uint64_t st[10] = {0};
     int rc = mfs_parse_stats(t.data(), t.size(), st);
-    if (rc == 0) {
-        std::vector<uint8_t> ref(st[0] + 8);
+    // st[0] = qubits, st[1] = measurements. The reference sample's tableau grows with the square of
+    // the qubit count; mutants with huge qubit indices only go through parsing and compiling.
+    if (rc == 0 && st[0] <= 4096) {
+        std::vector<uint8_t> ref((st[1] + 7) / 8 + 1);
         (void)mfs_reference_sample(t.data(), t.size(), ref.data(), ref.size());
         if (full) {
             mfs_m2d *c = nullptr;
@@ -28,8 +30,30 @@ static void run_circuit(const std::string &t, bool full) {
     }
 }
 
+// Seeded mutations of a circuit text: replaced characters drawn from the syntax's alphabet, inserted
+// digits (huge counts, indices, probabilities) and deleted spans.
+static std::string mutate(const std::string &t, uint64_t &state) {
+    static const char alphabet[] = "0123456789 \n\t()[]{}!*^-.,:#rec_XYZMRDETCOBSAPIHNLQ";
+    auto next = [&state]() {
+        state = state * 6364136223846793005ull + 1442695040888963407ull;
+        return (uint32_t)(state >> 33);
+    };
+    std::string s = t;
+    const int edits = 1 + (int)(next() % 4);
+    for (int e = 0; e < edits && !s.empty(); e++) {
+        const size_t at = next() % s.size();
+        switch (next() % 3) {
+            case 0: s[at] = alphabet[next() % (sizeof(alphabet) - 1)]; break;
+            case 1: s.insert(at, std::string(1 + next() % 20, (char)('0' + next() % 10))); break;
+            default: s.erase(at, 1 + next() % 8); break;
+        }
+    }
+    return s;
+}
+
 int main(int argc, char **argv) {
-    int files = 0, prefixes = 0;
+    int files = 0, prefixes = 0, mutants = 0;
+    uint64_t state = 12345;
     for (int i = 1; i < argc; i++) {
         std::ifstream f(argv[i]);
         std::stringstream ss;
@@ -45,9 +69,12 @@ int main(int argc, char **argv) {
             if (i <= 40) {
                 for (size_t k = 0; k < t.size() && k < 3000; k += 5) { run_circuit(t.substr(0, k), false); prefixes++; }
             }
+            if (t.size() < 20000) {
+                for (int k = 0; k < 20; k++) { run_circuit(mutate(t, state), false); mutants++; }
+            }
         }
         files++;
     }
-    printf("files %d prefixes %d\n", files, prefixes);
+    printf("files %d prefixes %d mutants %d\n", files, prefixes, mutants);
     return 0;
 }

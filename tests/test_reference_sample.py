@@ -57,3 +57,13 @@ def test_reference_sample_rejects_invalid_circuits(bad):
     import stim_amd
     with pytest.raises(ValueError):
         stim_amd.Circuit(bad).reference_sample()
+
+
+def test_reference_sample_tableau_limit():
+    """The reference sample's stabilizer tableau grows as the square of the qubit count: a circuit on
+    200,001 qubits (about 20 GB of tableau) is refused with RuntimeError before allocating, and a
+    circuit without measurements needs no tableau at all."""
+    import stim_amd
+    with pytest.raises(RuntimeError, match='tableau'):
+        stim_amd.Circuit('H 200000\nM 0\n').reference_sample()
+    assert len(stim_amd.Circuit('H 5000000\nX 3\n').reference_sample()) == 0
