@@ -428,15 +428,21 @@ Block parse_circuit(const char *text, size_t len) {
 }
 
 namespace {
+constexpr size_t MAX_UNROLLED_INSTRUCTIONS = (size_t)1 << 27;
+
 void flatten_into(const Block &b, std::vector<const Instr *> &out) {
     for (const Item &it : b.items) {
         if (it.is_repeat) {
-            for (uint64_t r = 0; r < it.reps; r++) {
-                flatten_into(*it.body, out);
-                if (out.size() > (size_t)1 << 27) {
-                    throw std::invalid_argument("circuit too long to unroll (more than 2^27 instructions)");
-                }
+            // The body is flattened once and copied: an empty body costs nothing whatever the count,
+            // and the size is checked before copying.
+            std::vector<const Instr *> body;
+            flatten_into(*it.body, body);
+            if (body.empty()) continue;
+            if (out.size() > MAX_UNROLLED_INSTRUCTIONS ||
+                it.reps > (MAX_UNROLLED_INSTRUCTIONS - out.size()) / body.size()) {
+                throw std::invalid_argument("circuit too long to unroll (more than 2^27 instructions)");
             }
+            for (uint64_t r = 0; r < it.reps; r++) out.insert(out.end(), body.begin(), body.end());
         } else {
             out.push_back(&it.instr);
         }

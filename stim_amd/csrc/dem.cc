@@ -127,6 +127,40 @@ void cancel_pairs(std::vector<uint32_t> &v) {
     v.swap(out);
 }
 
+uint64_t sat_add(uint64_t a, uint64_t b) {
+    return a + b < a ? UINT64_MAX : a + b;
+}
+
+uint64_t sat_mul(uint64_t a, uint64_t b) {
+    return (a && b > UINT64_MAX / a) ? UINT64_MAX : a * b;
+}
+
+// Unrolled size of a block: error mechanisms and the detector shift of one pass (both saturating).
+struct BlockSize {
+    uint64_t errors = 0, shift = 0;
+};
+
+BlockSize block_size(const std::vector<Block> &blocks, size_t b) {
+    BlockSize s;
+    const Block &blk = blocks[b];
+    for (size_t k = 0; k < blk.lines.size(); k++) {
+        const Line &ln = blk.lines[k];
+        if (blk.child[k] != SIZE_MAX) {
+            const BlockSize c = block_size(blocks, blk.child[k]);
+            s.errors = sat_add(s.errors, sat_mul(c.errors, ln.targets[0].value));
+            s.shift = sat_add(s.shift, sat_mul(c.shift, ln.targets[0].value));
+        } else if (ln.name == "error") {
+            s.errors = sat_add(s.errors, 1);
+        } else if (ln.name == "shift_detectors") {
+            for (const auto &tg : ln.targets) s.shift = sat_add(s.shift, tg.value);
+        }
+    }
+    return s;
+}
+
+// Error mechanisms the sampler unrolls at most (the model is materialized for the GPU).
+constexpr uint64_t MAX_DEM_MECHANISMS = 1ull << 28;
+
 struct Flattener {
     const std::vector<Block> &blocks;
     DemModel &model;
@@ -138,6 +172,20 @@ struct Flattener {
             const Line &ln = blk.lines[k];
             if (blk.child[k] != SIZE_MAX) {
                 const uint64_t reps = ln.targets[0].value;
+                if (reps == 0) continue;
+                const BlockSize c = block_size(blocks, blk.child[k]);
+                if (c.errors == 0) {
+                    // No mechanisms: only detector declarations and shifts, which never shrink, so
+                    // the last pass (at its final offset) declares the largest detectors.
+                    const uint64_t before = sat_mul(c.shift, reps - 1);
+                    if (sat_add(det_offset, before) >= (1ull << 62)) fail(ln.line_no, "detector shift too large");
+                    det_offset += before;
+                    run(blk.child[k]);
+                    continue;
+                }
+                if (sat_add(model.errors.size(), sat_mul(c.errors, reps)) > MAX_DEM_MECHANISMS) {
+                    fail(ln.line_no, "detector error model too large to unroll (more than 2^28 error mechanisms)");
+                }
                 for (uint64_t r = 0; r < reps; r++) run(blk.child[k]);
                 continue;
             }

@@ -21,9 +21,9 @@ static void run_circuit(const std::string &t, bool full) {
     if (rc == 0 && st[0] <= 4096) {
         std::vector<uint8_t> ref((st[1] + 7) / 8 + 1);
         (void)mfs_reference_sample(t.data(), t.size(), ref.data(), ref.size());
+        mfs_m2d *m2d = nullptr;
+        if (mfs_m2d_compile(t.data(), t.size(), 0, &m2d) == 0) mfs_m2d_destroy(m2d);
         if (full) {
-            mfs_m2d *c = nullptr;
-            if (mfs_m2d_compile(t.data(), t.size(), 0, &c) == 0) mfs_m2d_destroy(c);
             mfs_sampler *s = nullptr;
             if (mfs_compile(t.data(), t.size(), 1, 1, &s) == 0) mfs_destroy(s);
         }
@@ -64,6 +64,13 @@ int main(int argc, char **argv) {
             uint64_t st[3];
             (void)mfs_dem_parse_stats(t.data(), t.size(), st);
             for (size_t k = 0; k < t.size() && k < 4000; k += 7) (void)mfs_dem_parse_stats(t.data(), k, st);
+            if (t.size() < 200000) {
+                for (int k = 0; k < 40; k++) {
+                    const std::string u = mutate(t, state);
+                    (void)mfs_dem_parse_stats(u.data(), u.size(), st);
+                    mutants++;
+                }
+            }
         } else {
             run_circuit(t, true);
             if (i <= 40) {

@@ -111,3 +111,36 @@ def test_parser_accepts_reference_syntax():
 def test_barriers_fewer_than_ops(golden):
     s = _stats(golden.circuit('surface_d5_r5'))
     assert 0 < s[6] < s[5]
+
+
+def test_huge_repeat_counts_are_bounded():
+    """REPEAT blocks are unrolled for the device program, so their size is checked before unrolling:
+    empty bodies cost nothing whatever the count, oversized circuits and detector error models fail at
+    once with a clear error (instead of growing until memory runs out), and blocks without error
+    mechanisms still declare the detectors of their last pass."""
+    import ctypes
+    import time
+    from stim_amd import _native as N
+    L = N.lib()
+
+    def run(fn, text, n):
+        out = (ctypes.c_uint64 * n)()
+        b = text.encode()
+        t0 = time.time()
+        rc = fn(b, len(b), out)
+        assert time.time() - t0 < 5
+        return rc, list(out), (L.mfs_last_error().decode() if rc else '')
+
+    rc, out, _ = run(L.mfs_parse_stats, 'REPEAT 1000000000000000000 {\n}\nH 0\n', 10)
+    assert rc == 0 and out[0] == 1
+    rc, out, err = run(L.mfs_parse_stats, 'REPEAT 1000000000000 {\nM 0\n}\n', 10)
+    assert rc == N.MFS_ERR_INVALID_ARGUMENT and 'unroll' in err
+    rc, out, err = run(L.mfs_dem_parse_stats, 'repeat 366666666666666666 {\nerror(0.1) D0\nshift_detectors 1\n}\n', 3)
+    assert rc == N.MFS_ERR_INVALID_ARGUMENT and 'too large' in err
+    rc, out, _ = run(L.mfs_dem_parse_stats, 'repeat 1000000000000000000 {\n}\nerror(0.1) D0\n', 3)
+    assert rc == 0 and out == [1, 0, 1]
+    # The last pass declares D2 at offset 12; the error after the block is D0 at offset 15.
+    rc, out, _ = run(L.mfs_dem_parse_stats, 'repeat 5 {\ndetector D2\nshift_detectors 3\n}\nerror(0.1) D0\n', 3)
+    assert rc == 0 and out == [16, 0, 1]
+    rc, out, _ = run(L.mfs_dem_parse_stats, 'repeat 5 {\ndetector D2\nshift_detectors 3\n}\n', 3)
+    assert rc == 0 and out == [15, 0, 0]
