@@ -5,8 +5,8 @@
 // words in LDS (word m = measurement m of the tile's 32 shots). Each output column is then the XOR
 // of its measurements' words, inverted when its reference parity is 1 (semantics in m2d.h), and the
 // column words are transposed back into shot-major rows, staged in LDS a chunk of columns at a time
-// and copied to the rows with coalesced stores. HBM-bound: each shot's measurement row is read once
-// and its output rows written once.
+// and copied to the rows with coalesced stores. HBM traffic is the minimum: each shot's measurement
+// row is read once and its output rows written once.
 //
 // Work is laid out so that lane-dependent branches are rare: loads and LDS stores whose lanes fall
 // past the tile's shots or the row's end go to clamped addresses or padding that no output reads.
