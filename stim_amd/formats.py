@@ -87,6 +87,10 @@ def read_rows(path, fmt, bits_per_shot, prefixes=None):
             if not toks or toks[0] != 'shot':
                 raise ValueError(f'dets line {i} does not start with "shot"')
             for tok in toks[1:]:
+                if tok[0] not in base:
+                    # measure_record_reader.inl:573-576 rejects an unknown prefix the same way.
+                    raise ValueError(f'Unrecognized DETS prefix {tok[0]!r} in {tok!r}; expected one of '
+                                     f'{", ".join(base)}')
                 start, count = base[tok[0]]
                 k = int(tok[1:])
                 if not 0 <= k < count:

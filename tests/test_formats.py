@@ -60,3 +60,38 @@ def test_readers_reject_malformed_data(tmp_path):
         formats.read_rows(str(p), 'r8', 4)
     with pytest.raises(ValueError):
         formats.read_rows(str(p), 'xyz', 4)
+
+
+def test_malformed_files_raise_value_error(tmp_path):
+    """Seeded byte mutations of valid files of every format either read or raise ValueError (the
+    reference's readers throw std::invalid_argument), and never take long."""
+    import random
+    import time
+    from stim_amd import formats as F
+    rng = random.Random(7)
+    n = 13
+    rows = np.random.default_rng(0).integers(0, 2, size=(64, n)).astype(bool)
+    for fmt in ['01', 'b8', 'r8', 'hits', 'dets', 'ptb64']:
+        p = tmp_path / ('x.' + fmt)
+        F.write_rows(str(p), fmt, rows)
+        good = p.read_bytes()
+        for _ in range(400):
+            d = bytearray(good)
+            for _ in range(rng.randint(1, 4)):
+                if not d:
+                    break
+                i = rng.randrange(len(d))
+                op = rng.randrange(3)
+                if op == 0:
+                    d[i] = rng.randrange(256)
+                elif op == 1:
+                    d[i:i] = bytes(rng.choice(b'0123456789 ,\nDLMshot\xff') for _ in range(rng.randint(1, 12)))
+                else:
+                    del d[i:i + rng.randint(1, 8)]
+            p.write_bytes(bytes(d))
+            t0 = time.time()
+            try:
+                F.read_rows(str(p), fmt, n)
+            except ValueError:
+                pass
+            assert time.time() - t0 < 2, fmt
