@@ -34,14 +34,44 @@ TRAFFIC = os.path.join(ROOT, 'profiles', 'sym_scan_kernel_d25_r25_2p20_traffic.m
 DEM_D25 = os.path.join(ROOT, 'tests', 'golden', 'dems', 'surface_d25_r25_folded.dem.gz')
 
 
+# One reference process per host core, at most 16: a one-GPU lease's share of the host is 16 CPUs,
+# even where the affinity mask shows more. Every baseline records both counts.
+CPU_SHARE = 16
+
+
+def _host_cpus():
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _cores():
+    return max(1, min(_host_cpus(), CPU_SHARE))
+
+
+def _cpu_model():
+    try:
+        with open('/proc/cpuinfo') as f:
+            for line in f:
+                if line.startswith('model name'):
+                    return line.split(':', 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def _baseline(value, cores, sample):
+    """A cpu_baseline record: the reference CLI, `cores` processes at once."""
+    return {'value': value, 'unit': 'shots/s', 'cores': cores, 'kind': 'reference', 'sample': sample,
+            'affinity_cpus': _host_cpus(), 'os_cpu_count': os.cpu_count(), 'cpu_model': _cpu_model(),
+            'processes_cap': CPU_SHARE}
+
+
 def cpu_baseline(circuit_path):
     """The reference CLI `stim detect` (built from the reference's sources into oracle/_ref), one process
     per core, on a bounded sample; the numpy oracle port when the CLI is absent."""
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores = _cores()
     if os.path.exists(REF_STIM) and os.access(REF_STIM, os.X_OK):
         shots = 1 << 19
         procs = []
@@ -54,9 +84,8 @@ def cpu_baseline(circuit_path):
         ok = all(p.wait() == 0 for p in procs)
         wall = time.perf_counter() - t0
         if ok:
-            return {'value': cores * shots / wall, 'unit': 'shots/s', 'cores': cores, 'kind': 'reference',
-                    'sample': f'stim detect b8 to /dev/null, {cores} processes x {shots} shots '
-                              f'(d=25 r=25 p=1e-3), wall {wall:.2f}s'}
+            return _baseline(cores * shots / wall, cores, f'stim detect b8 to /dev/null, {cores} processes x {shots} shots '
+                              f'(d=25 r=25 p=1e-3), wall {wall:.2f}s')
     from oracle import frame_oracle as fo
     import numpy as np
     text = open(circuit_path).read()
@@ -322,11 +351,7 @@ def clifford(args):
 def dem_cpu_baseline(text):
     """The reference CLI `stim sample_dem`, one process per core, on a bounded sample of the same model."""
     import tempfile
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+    cores = _cores()
     if not (os.path.exists(REF_STIM) and os.access(REF_STIM, os.X_OK)):
         return None
     shots = 1 << 18
@@ -347,9 +372,8 @@ def dem_cpu_baseline(text):
         os.unlink(path)
     if not ok:
         return None
-    return {'value': cores * shots / wall, 'unit': 'shots/s', 'cores': cores, 'kind': 'reference',
-            'sample': f'stim sample_dem b8 to /dev/null, {cores} processes x {shots} shots '
-                      f'(d=25 r=25 p=1e-3 model), wall {wall:.2f}s'}
+    return _baseline(cores * shots / wall, cores, f'stim sample_dem b8 to /dev/null, {cores} processes x {shots} shots '
+                      f'(d=25 r=25 p=1e-3 model), wall {wall:.2f}s')
 
 
 def dem(args):
@@ -411,12 +435,6 @@ def dem(args):
     }
 
 
-def _cores():
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except AttributeError:
-        cores = os.cpu_count() or 1
-    return max(1, min(cores, 16))
 
 
 def _run_parallel(cmds):
@@ -436,9 +454,8 @@ def measure_cpu_baseline():
                                '--out', '/dev/null', '--seed', str(k)] for k in range(cores)])
     if not ok:
         return None
-    return {'value': cores * shots / wall, 'unit': 'shots/s', 'cores': cores, 'kind': 'reference',
-            'sample': f'stim sample b8 to /dev/null, {cores} processes x {shots} shots (d=25 r=25 p=1e-3), '
-                      f'wall {wall:.2f}s'}
+    return _baseline(cores * shots / wall, cores, f'stim sample b8 to /dev/null, {cores} processes x {shots} shots (d=25 r=25 p=1e-3), '
+                      f'wall {wall:.2f}s')
 
 
 def m2d_cpu_baseline():
@@ -458,9 +475,8 @@ def m2d_cpu_baseline():
                                   for _ in range(cores)])
     if not ok:
         return None
-    return {'value': cores * shots / wall, 'unit': 'shots/s', 'cores': cores, 'kind': 'reference',
-            'sample': f'stim m2d b8 -> b8 (observables appended) to /dev/null, {cores} processes x {shots} shots '
-                      f'(d=25 r=25 measurement file), wall {wall:.2f}s'}
+    return _baseline(cores * shots / wall, cores, f'stim m2d b8 -> b8 (observables appended) to /dev/null, {cores} processes x {shots} shots '
+                      f'(d=25 r=25 measurement file), wall {wall:.2f}s')
 
 
 def _timed(step, args):

@@ -233,6 +233,25 @@ int mfs_server_replay(mfs_server *v, uint64_t n, mfs_sampler *const *samplers, c
 int mfs_set_tile_words(mfs_sampler *s, int words);
 int mfs_get_info(const mfs_sampler *s, uint64_t out[8]); /* tile_words, lds_bytes, ops, syncs, noise_ops, randomize, ring, dense_qubits */
 
+/* Position of a sampler's random stream: the global shot-tile index its next call starts at. Every
+ * call advances it by the tiles it used, so consecutive calls draw fresh randomness; setting it back
+ * reproduces a call. Tile indices are 64-bit and the stream does not repeat. No reference counterpart
+ * (the reference's samplers hold one std::mt19937_64, src/stim/util_bot/probability_util.h:40-76). */
+int mfs_set_stream_position(mfs_sampler *s, uint64_t tiles);
+int mfs_get_stream_position(const mfs_sampler *s, uint64_t *tiles);
+
+/* Checks of the random streams (no reference counterpart; the reference's generator is libstdc++'s
+ * std::mt19937_64). mfs_rng_counter reports the Philox counter out[0..3] and key out[4..5] that the
+ * symptom kernels (kernel 0) or the frame kernel (kernel 1) use for draw `draw` of lane `lane` in
+ * shot tile `tile`, stream tag `tag`, under kernel key `key` (host only). mfs_philox runs the
+ * kernels' Philox4x32-10 on the GPU over n counters (4 words each) under one key. */
+int mfs_rng_counter(int kernel, uint64_t key, uint64_t tile, uint32_t lane, uint32_t draw, uint32_t tag,
+                    uint32_t out[6]);
+int mfs_philox(const uint32_t *counters, uint64_t n, uint64_t key, uint32_t *out);
+
+/* Hash of the sources the library was built from (16 hex digits; stim_amd/_buildid.py). */
+const char *mfs_build_id(void);
+
 const char *mfs_last_error(void);
 
 #ifdef __cplusplus

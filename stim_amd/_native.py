@@ -31,6 +31,7 @@ EXPORTED = [
     'mfs_server_result', 'mfs_server_copy_result', 'mfs_server_release', 'mfs_server_stats', 'mfs_server_replay',
     'mfs_dem_compile', 'mfs_dem_parse_stats', 'mfs_reference_sample', 'mfs_compile_measurements',
     'mfs_m2d_compile', 'mfs_m2d_destroy', 'mfs_m2d_stats', 'mfs_m2d_convert', 'mfs_m2d_bench',
+    'mfs_set_stream_position', 'mfs_get_stream_position', 'mfs_rng_counter', 'mfs_philox', 'mfs_build_id',
 ]
 
 _lib = None
@@ -103,13 +104,34 @@ def lib():
                                     ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
     L.mfs_dem_compile.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.POINTER(p)]
     L.mfs_dem_parse_stats.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(u64)]
+    L.mfs_set_stream_position.argtypes = [p, u64]
+    L.mfs_get_stream_position.argtypes = [p, ctypes.POINTER(u64)]
+    L.mfs_rng_counter.argtypes = [ctypes.c_int, u64, u64, u32, u32, u32, ctypes.POINTER(u32)]
+    L.mfs_philox.argtypes = [p, u64, u64, p]
+    L.mfs_build_id.argtypes = []
+    L.mfs_build_id.restype = ctypes.c_char_p
     L.mfs_last_error.argtypes = []
     L.mfs_last_error.restype = ctypes.c_char_p
     for name in EXPORTED:
-        if name not in ('mfs_destroy', 'mfs_last_error', 'mfs_compile', 'mfs_batch_destroy', 'mfs_server_destroy'):
+        if name not in ('mfs_destroy', 'mfs_last_error', 'mfs_compile', 'mfs_batch_destroy', 'mfs_server_destroy',
+                        'mfs_build_id'):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
+
+
+def build_id():
+    """Source hash the loaded library was built from."""
+    return lib().mfs_build_id().decode()
+
+
+def check_build_id():
+    """Raises RuntimeError unless the loaded library was built from the sources in this tree."""
+    from stim_amd._buildid import source_hash
+    want, have = source_hash(), build_id()
+    if want != have:
+        raise RuntimeError(f'{LIB_PATH} was built from other sources (library {have}, tree {want}); rebuild it')
+    return have
 
 
 def check(rc):

@@ -24,6 +24,7 @@
 #include "bitops.h"
 #include "launch_util.h"
 #include "program.h"
+#include "rng.h"
 
 using namespace sa;
 
@@ -225,7 +226,8 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(const uint32_t *__restric
     const uint32_t tid = threadIdx.x;
     const uint32_t W = 32 * T;
     const uint64_t shot0 = (uint64_t)blockIdx.x * W;
-    Rng rng{(uint32_t)P.seed, (uint32_t)(P.seed >> 32), (uint32_t)(P.tile0 + blockIdx.x)};
+    const uint64_t tile64 = P.tile0 + blockIdx.x;
+    Rng rng{(uint32_t)P.seed, frame_key_hi((uint32_t)(P.seed >> 32), tile64), (uint32_t)tile64};
     const bool randomize = P.randomize != 0;
 
     for (uint32_t i = tid; i < P.num_qubits; i += NT) {

@@ -3,6 +3,10 @@
 #include "sampler_impl.h"
 #include "tableau.h"
 
+#ifndef STIM_AMD_BUILD_ID
+#define STIM_AMD_BUILD_ID "unknown"
+#endif
+
 namespace {
 
 enum class Fmt { F01, B8, R8, HITS, DETS, PTB64 };
@@ -452,6 +456,46 @@ int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb) {
         s->sym_scan = kb > 0;
         s->sym_tile_kb = kb > 0 ? kb : -kb;
     });
+}
+
+int mfs_set_stream_position(mfs_sampler *s, uint64_t tiles) {
+    return guarded([&] { s->tiles_used = tiles; });
+}
+
+int mfs_get_stream_position(const mfs_sampler *s, uint64_t *tiles) {
+    return guarded([&] { *tiles = s->tiles_used; });
+}
+
+int mfs_rng_counter(int kernel, uint64_t key, uint64_t tile, uint32_t lane, uint32_t draw, uint32_t tag,
+                    uint32_t out[6]) {
+    return guarded([&] {
+        if (kernel != 0 && kernel != 1) throw std::invalid_argument("kernel must be 0 (symptom) or 1 (frame)");
+        if (kernel == 0 && lane > 255) throw std::invalid_argument("symptom kernels have 256 lanes");
+        out[0] = draw;
+        out[1] = kernel == 0 ? sym_lane_word(lane, tile) : lane;
+        out[2] = (uint32_t)tile;
+        out[3] = tag;
+        out[4] = (uint32_t)key;
+        out[5] = kernel == 0 ? (uint32_t)(key >> 32) : frame_key_hi((uint32_t)(key >> 32), tile);
+    });
+}
+
+int mfs_philox(const uint32_t *counters, uint64_t n, uint64_t key, uint32_t *out) {
+    return guarded([&] {
+        if (!n) return;
+        if (n > (1ull << 24)) throw std::invalid_argument("at most 2^24 counters per call");
+        DevBuf d_in, d_out;
+        d_in.ensure(16 * n);
+        d_out.ensure(16 * n);
+        check(hipMemcpy(d_in.p, counters, 16 * n, hipMemcpyHostToDevice), "hipMemcpy");
+        check(launch_philox(d_in.as<uint32_t>(), n, key, d_out.as<uint32_t>(), nullptr), "philox kernel");
+        check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        check(hipMemcpy(out, d_out.p, 16 * n, hipMemcpyDeviceToHost), "hipMemcpy");
+    });
+}
+
+const char *mfs_build_id(void) {
+    return STIM_AMD_BUILD_ID;
 }
 
 int mfs_get_info(const mfs_sampler *s, uint64_t out[8]) {

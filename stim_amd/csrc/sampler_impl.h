@@ -25,6 +25,7 @@
 #include "program.h"
 #include "dem.h"
 #include "symptom.h"
+#include "rng.h"
 
 namespace sa {
 hipError_t launch_frame_kernel(int T, const DevProgram &prog, const LaunchParams &P, uint32_t num_tiles, size_t lds,
@@ -44,6 +45,7 @@ hipError_t launch_sym_fill(const uint8_t *det, uint32_t det_row, const uint8_t *
                            uint32_t D, const uint32_t *offs, uint32_t *ent, hipStream_t stream);
 hipError_t launch_unpack_rows(const uint8_t *src, uint64_t src_stride, uint8_t *dst, uint64_t dst_stride, uint64_t rows,
                               uint32_t ncols, uint32_t base_char, hipStream_t stream);
+hipError_t launch_philox(const uint32_t *ctr, uint64_t n, uint64_t key, uint32_t *out, hipStream_t stream);
 }  // namespace sa
 
 using namespace sa;

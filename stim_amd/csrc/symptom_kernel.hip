@@ -13,6 +13,7 @@
 
 #include "program.h"
 #include "bitops.h"
+#include "rng.h"
 #include "launch_util.h"
 #include "symptom.h"
 
@@ -160,7 +161,8 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
     if (tid == 0) qcount[0] = 0;
 
     const uint64_t s0 = (uint64_t)blockIdx.x * S;
-    const uint32_t tile = (uint32_t)(P.tile0 + blockIdx.x);
+    const uint64_t tile64 = P.tile0 + blockIdx.x;
+    const uint32_t tile = (uint32_t)tile64, lane_word = sym_lane_word(tid, tile64);
     const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
     const uint64_t total = P.total_sites << P.tile_log2;
     const uint64_t R = (total + NT - 1) / NT;
@@ -206,7 +208,7 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
                 cend = ce < hi ? ce : hi;
                 continue;
             }
-            const uint4 r = philox10(make_uint4(ctr++, tid, tile, TAG_SYM), k0, k1);
+            const uint4 r = philox10(make_uint4(ctr++, lane_word, tile, TAG_SYM), k0, k1);
             const uint64_t p1 = pos + gap(r.x, cl.inv_log);
             if (p1 >= cend) {
                 pos = cend;
@@ -276,10 +278,12 @@ __global__ __launch_bounds__(NT) void sym_sample_kernel(SymDevParams P) {
 // lane).
 constexpr uint32_t SCAN_M_MAX = 8;
 
+// Gaps are clamped at 2^31, above any group's position count N (at most 2^31 - 1), so a clamped gap
+// always passes the end of the group and g + 1 cannot overflow.
 __device__ __forceinline__ uint32_t gap32(uint32_t r, float inv_log2_1mp) {
     const float u = ((float)(r >> 8) + 1.0f) * (1.0f / 16777216.0f);
     const float g = __log2f(u) * inv_log2_1mp;
-    return g < 1.0e9f ? (uint32_t)g : 1000000000u;
+    return g < 2147483648.0f ? (uint32_t)g : 0x80000000u;
 }
 
 __device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
@@ -453,7 +457,8 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
     __syncthreads();
 
     const uint64_t s0 = (uint64_t)btile * S;
-    const uint32_t tile = (uint32_t)(P.tile0 + btile);
+    const uint64_t tile64 = P.tile0 + btile;
+    const uint32_t tile = (uint32_t)tile64, lane_word = sym_lane_word(tid, tile64);
     const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
     for (uint32_t gi = 0; gi < P.ngroups; gi++) {
         const SymGroup G = P.groups[gi];
@@ -471,7 +476,7 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
 #pragma unroll
             for (uint32_t k = 0; k < SCAN_M_MAX / 2; k++) {
                 if (2 * k < m) {
-                    const uint4 r = philox10(make_uint4(round * (SCAN_M_MAX / 2) + k, tid, tile, TAG_SCAN | gi), k0, k1);
+                    const uint4 r = philox10(make_uint4(round * (SCAN_M_MAX / 2) + k, lane_word, tile, TAG_SCAN | gi), k0, k1);
                     g[2 * k] = gap32(r.x, G.inv_log);
                     oc[2 * k] = r.y;
                     g[2 * k + 1] = gap32(r.z, G.inv_log);
@@ -712,9 +717,24 @@ __global__ void unpack_rows_kernel(const uint8_t *__restrict__ src, uint64_t src
     }
 }
 
+// Philox4x32-10 of the samplers on n counters under one key (checks of the generator itself).
+__global__ void philox_kernel(const uint4 *__restrict__ ctr, uint64_t n, uint32_t k0, uint32_t k1,
+                              uint4 *__restrict__ out) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = philox10(ctr[i], k0, k1);
+}
+
 }  // namespace
 
 namespace sa {
+
+hipError_t launch_philox(const uint32_t *ctr, uint64_t n, uint64_t key, uint32_t *out, hipStream_t stream) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(philox_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, stream,
+                       reinterpret_cast<const uint4 *>(ctr), n, (uint32_t)key, (uint32_t)(key >> 32),
+                       reinterpret_cast<uint4 *>(out));
+    return hipGetLastError();
+}
 
 hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream) {
     if (!tiles) return hipSuccess;

@@ -286,6 +286,17 @@ class CompiledDetectorSampler:
     def set_symptom_tile_kb(self, kb: int):
         N.check(N.lib().mfs_set_symptom_tile_kb(self._h, int(kb)))
 
+    @property
+    def stream_position(self) -> int:
+        """Global shot-tile index of the random stream where the next call starts."""
+        v = ctypes.c_uint64()
+        N.check(N.lib().mfs_get_stream_position(self._h, ctypes.byref(v)))
+        return v.value
+
+    @stream_position.setter
+    def stream_position(self, tiles: int):
+        N.check(N.lib().mfs_set_stream_position(self._h, int(tiles)))
+
     def engine_info(self):
         out = (ctypes.c_uint64 * 8)()
         N.check(N.lib().mfs_engine_info(self._h, out))
