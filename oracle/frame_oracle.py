@@ -298,6 +298,9 @@ class FrameOracle:
                 raise NotImplementedError(n)
 
         D = np.array(dets, dtype=bool).reshape(len(dets), S)
+        # The flips of every measurement result, in record order (rows = results), for checks of
+        # measurement outputs.
+        self.last_record = rec
         return D.T.copy(), obs.T.copy()
 
     # ------------------------------------------------------------------ pieces
@@ -499,14 +502,43 @@ def noise_sites(flat, allow_chains=False):
     return chans
 
 
-def exact_detector_rates(text_or_flat, batch=4096):
-    """Exact P(detector fires) and P(observable flips) for independent disjoint channels.
+def _channels(flat):
+    """Every channel as a list of disjoint outcomes (instruction_index, group_index, code, probability).
 
-    Each channel's outcome symptoms come from the frame oracle in fixed-fault mode (linearity over
-    GF(2)); then P(odd) = (1 - prod_c (1 - 2 q_c)) / 2 per output bit."""
+    A correlated chain is one channel: CORRELATED_ERROR starts it and each later
+    ELSE_CORRELATED_ERROR fires only when no earlier element of the chain fired, so element i fires with
+    p_i * prod_{j<i} (1 - p_j) (frame_simulator.inl:746-776: the `last_correlated_error_occurred` mask,
+    cleared only by CORRELATED_ERROR)."""
+    out = []
+    chain = None   # (outcome list, probability that no element fired yet)
+    for (k, j, outs) in noise_sites(flat, allow_chains=True):
+        name = flat[k].name
+        if name in ('E', 'CORRELATED_ERROR'):
+            p = outs[0][1]
+            chain = [[(k, 0, 1, p)], 1.0 - p]
+            out.append(chain[0])
+        elif name == 'ELSE_CORRELATED_ERROR':
+            if chain is None:
+                # No chain yet: the mask of fired elements starts clear, so the element fires freely.
+                chain = [[], 1.0]
+                out.append(chain[0])
+            p = outs[0][1]
+            chain[0].append((k, 0, 1, p * chain[1]))
+            chain[1] *= 1.0 - p
+        else:
+            out.append([(k, j, code, p) for (code, p) in outs])
+    return out
+
+
+def exact_detector_rates(text_or_flat, batch=4096):
+    """Exact P(detector fires) and P(observable flips) for independent channels of disjoint outcomes.
+
+    Each outcome's symptom comes from the frame oracle in fixed-fault mode (linearity over GF(2)); a
+    channel flips an output bit with q_c = sum of its outcomes' probabilities whose symptom holds the
+    bit, and P(odd) = (1 - prod_c (1 - 2 q_c)) / 2 per output bit."""
     orc = FrameOracle(text_or_flat)
-    chans = noise_sites(orc.flat)
-    items = [(k, j, code, p) for (k, j, outs) in chans for (code, p) in outs]
+    chans = _channels(orc.flat)
+    items = [it for ch in chans for it in ch]
     D = orc.stats['num_detectors'] + orc.stats['num_observables']
     syms = np.zeros((len(items), D), dtype=bool)
     for b in range(0, len(items), batch):
@@ -514,13 +546,15 @@ def exact_detector_rates(text_or_flat, batch=4096):
         dets, obs = orc.run(len(chunk), faults=[[(k, j, c)] for (k, j, c, _) in chunk])
         syms[b:b + len(chunk)] = np.concatenate([dets, obs], axis=1)
     log_prod = np.zeros(D)
+    sign = np.ones(D)
     pos = 0
-    for (k, j, outs) in chans:
+    for ch in chans:
         q = np.zeros(D)
-        for (code, p) in outs:
+        for (_, _, _, p) in ch:
             q += p * syms[pos]
             pos += 1
+        f = 1 - 2 * q
+        sign = np.where(f < 0, -sign, sign)
         with np.errstate(divide='ignore'):
-            log_prod += np.log(np.abs(1 - 2 * q))
-        # sign: (1-2q) can be negative only if q > 1/2, which disjoint small channels never reach
-    return (1 - np.exp(log_prod)) / 2
+            log_prod += np.log(np.abs(f))
+    return (1 - sign * np.exp(log_prod)) / 2

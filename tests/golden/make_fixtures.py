@@ -385,7 +385,68 @@ def write_m2d_fixtures():
     json.dump(cases, open(os.path.join(HERE, 'm2d_vectors.json'), 'w'))
 
 
+def write_zoo_counts():
+    """Seeded detector/observable firing counts of the reference CLI (`stim detect`, 2^20 shots) on
+    gate-zoo circuits whose noise includes PAULI_CHANNEL_1/2, heralded channels and correlated-error
+    chains: the statistical pin of the oracle's exact rates for those channels."""
+    cases = {}
+    seed = 0
+    while len(cases) < 6:
+        text = zoo.gate_zoo(500 + seed, n_qubits=12, layers=40)
+        seed += 1
+        kinds = [k for k in ('PAULI_CHANNEL_1', 'PAULI_CHANNEL_2', 'HERALDED_ERASE', 'HERALDED_PAULI_CHANNEL_1',
+                             'ELSE_CORRELATED_ERROR') if k in text]
+        if len(kinds) < 3 or 'ELSE_CORRELATED_ERROR' not in text:
+            continue
+        with tempfile.NamedTemporaryFile('w', suffix='.stim', delete=False) as f:
+            f.write(text)
+            path = f.name
+        try:
+            shots = 1 << 20
+            cases['zoo_counts_%d' % (500 + seed - 1)] = {'circuit': text, 'shots': shots, 'seed': 11,
+                                                         'counts': seeded_counts(path, shots, 11), 'kinds': kinds}
+        finally:
+            os.unlink(path)
+    # Large probabilities, where a chain's conditions, a heralded channel's outcome order or a
+    # PAULI_CHANNEL_2 argument order change the rates by far more than the test's resolution.
+    stress = '\n'.join([
+        'R 0 1 2 3 4 5',
+        'H 1 4',
+        'E(0.25) X0 X1',
+        'ELSE_CORRELATED_ERROR(0.35) X1 X2',
+        'CX 4 5',
+        'CX 4 5',
+        'ELSE_CORRELATED_ERROR(0.45) X0 X2 Y3',
+        'PAULI_CHANNEL_2(0.01, 0.02, 0.03, 0.04, 0.05, 0.06, 0.07, 0.015, 0.025, 0.035, 0.045, 0.055, 0.065, '
+        '0.075, 0.085) 4 5',
+        'PAULI_CHANNEL_1(0.1, 0.2, 0.05) 3',
+        'HERALDED_ERASE(0.2) 2',
+        'HERALDED_PAULI_CHANNEL_1(0.05, 0.1, 0.15, 0.2) 1',
+        'M 0 2 3',
+        'MX 1 4',
+        'M 5',
+        'DETECTOR rec[-8]',
+        'DETECTOR rec[-7]',
+    ] + ['DETECTOR rec[-%d]' % k for k in range(1, 7)]
+      + ['DETECTOR rec[-%d] rec[-%d]' % (a, b) for a in range(1, 7) for b in range(a + 1, 9)]
+      + ['OBSERVABLE_INCLUDE(0) rec[-6] rec[-3]']) + '\n'
+    with tempfile.NamedTemporaryFile('w', suffix='.stim', delete=False) as f:
+        f.write(stress)
+        path = f.name
+    try:
+        cases['chain_stress'] = {'circuit': stress, 'shots': 1 << 20, 'seed': 11,
+                                 'counts': seeded_counts(path, 1 << 20, 11),
+                                 'kinds': ['PAULI_CHANNEL_1', 'PAULI_CHANNEL_2', 'HERALDED_ERASE',
+                                           'HERALDED_PAULI_CHANNEL_1', 'ELSE_CORRELATED_ERROR']}
+    finally:
+        os.unlink(path)
+    json.dump(cases, open(os.path.join(HERE, 'zoo_counts.json'), 'w'))
+
+
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == '--zoo-counts-only':
+        write_zoo_counts()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == '--m2d-only':
         write_m2d_fixtures()
         return

@@ -178,3 +178,70 @@ def test_randomized_oracle_observable_pauli_targets(reset, zero, pair):
     assert not obs[:, zero].any()
     a, b = pair
     assert (obs[:, a] == obs[:, b]).all() and 300 < obs[:, a].sum() < 700
+
+
+def _zoo_counts_z(case, rates):
+    return _binomial_z(case['counts'], case['shots'], rates)
+
+
+def test_exact_rates_with_chains_heralds_and_pauli_channels_match_reference_counts(golden):
+    """Gate-zoo circuits with PAULI_CHANNEL_1/2, heralded channels and correlated-error chains: the
+    oracle's exact rates against the reference CLI's seeded counts (2^20 shots each)."""
+    cases = golden.load('zoo_counts.json')
+    assert len(cases) >= 6
+    zmax = []
+    for name, c in cases.items():
+        exact = fo.exact_detector_rates(c['circuit'])
+        z = _zoo_counts_z(c, exact)
+        zcrit = abs(_norm_ppf(1e-4 / (2 * len(z))))
+        assert np.max(np.abs(z)) < zcrit, (name, np.max(np.abs(z)))
+        zmax.append(np.max(np.abs(z)))
+
+
+def test_exact_rates_planted_chain_mistake_fails(golden):
+    """Treating each ELSE_CORRELATED_ERROR as an independent channel (a dropped chain condition) must
+    fail the same test on some circuit."""
+    cases = golden.load('zoo_counts.json')
+    orig = fo._channels
+
+    def independent(flat):
+        out = []
+        for ch in orig(flat):
+            if len(ch) > 1 and flat[ch[0][0]].name in ('E', 'CORRELATED_ERROR'):
+                # every element at its own unconditional probability
+                out.extend([[(k, j, c, flat[k].args[0])] for (k, j, c, _) in ch])
+            else:
+                out.append(ch)
+        return out
+
+    worst = 0.0
+    try:
+        fo._channels = independent
+        for name, c in cases.items():
+            z = _zoo_counts_z(c, fo.exact_detector_rates(c['circuit']))
+            worst = max(worst, float(np.max(np.abs(z)) / abs(_norm_ppf(1e-4 / (2 * len(z))))))
+    finally:
+        fo._channels = orig
+    assert worst > 1.0
+
+
+@pytest.mark.parametrize('gate', ['PAULI_CHANNEL_2', 'HERALDED_PAULI_CHANNEL_1'])
+def test_exact_rates_planted_outcome_order_mistake_fails(golden, gate):
+    """Reversing the outcome order of one channel kind must fail the reference-count test."""
+    case = golden.load('zoo_counts.json')['chain_stress']
+    orig = fo.noise_sites
+
+    def wrong(flat, allow_chains=False):
+        out = []
+        for (k, j, outs) in orig(flat, allow_chains):
+            if flat[k].name == gate:
+                outs = list(zip([c for c, _ in outs], [p for _, p in outs][::-1]))
+            out.append((k, j, outs))
+        return out
+
+    try:
+        fo.noise_sites = wrong
+        z = _zoo_counts_z(case, fo.exact_detector_rates(case['circuit']))
+    finally:
+        fo.noise_sites = orig
+    assert np.max(np.abs(z)) > abs(_norm_ppf(1e-4 / (2 * len(z))))
