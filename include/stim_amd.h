@@ -110,6 +110,19 @@ int mfs_reference_sample(const char *stim_text, size_t len, uint8_t *out, size_t
 int mfs_sample(mfs_sampler *s, uint64_t shots, uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs,
                size_t obs_stride);
 
+/* Allocation-free form of mfs_sample for device-resident rows (MFS_DEVICE_OUTPUT required). No
+ * counterpart in the reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). It calls no hipMalloc, hipFree
+ * or allocating library routine: device staging (packed rows before they are expanded to one byte per
+ * entry, unpacked rows before packing across a non-byte-aligned observable boundary) comes from a fixed
+ * 64 MiB region that each GPU holds for the life of the process, and calls are cut into batches of
+ * whole RNG tiles that fit it, so the rows equal mfs_sample's bit for bit. A call whose rows do not fit
+ * the region one tile at a time fails with MFS_ERR_INVALID_ARGUMENT, and so does a call that would
+ * first have to build the sampler's symptom table (prepare the sampler once with mfs_engine_info or an
+ * allocating call). Calls that overlap on one GPU share the region and race. */
+int mfs_sample_noalloc(mfs_sampler *s, uint64_t shots, uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs,
+                       size_t obs_stride);
+
 /* Samples and writes to a file in format "01", "b8", "r8", "ptb64", "hits" or "dets".
  * obs_path may be NULL. prepend + append + (obs_path != NULL) > 1 -> MFS_ERR_OUT_OF_RANGE.
  * Reference: sample_batch_detection_events_writing_results_to_disk (frame_simulator_util.inl:114-187, 206-288). */

@@ -32,6 +32,7 @@ EXPORTED = [
     'mfs_dem_compile', 'mfs_dem_parse_stats', 'mfs_reference_sample', 'mfs_compile_measurements',
     'mfs_m2d_compile', 'mfs_m2d_destroy', 'mfs_m2d_stats', 'mfs_m2d_convert', 'mfs_m2d_bench',
     'mfs_set_stream_position', 'mfs_get_stream_position', 'mfs_rng_counter', 'mfs_philox', 'mfs_build_id',
+    'mfs_sample_noalloc',
 ]
 
 _lib = None
@@ -68,6 +69,7 @@ def lib():
     L.mfs_compile_measurements.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.c_int, p, sz,
                                            ctypes.POINTER(p)]
     L.mfs_sample.argtypes = [p, u64, u32, p, sz, p, sz]
+    L.mfs_sample_noalloc.argtypes = [p, u64, u32, p, sz, p, sz]
     L.mfs_sample_write.argtypes = [p, u64, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_char_p, ctypes.c_char_p]
     L.mfs_sample_with_faults.argtypes = [p, u64, p, p, u32, p, sz, p, sz]

@@ -150,6 +150,40 @@ struct mfs_sampler {
     int sym_tile_kb = 32;        // target LDS image size per symptom tile
     int sym_scan = 1;            // scan kernel (0: per-lane-walk queue kernel)
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // Device staging of a call comes from the sampler's own buffers (grown with hipMalloc as needed),
+    // or, while `arena` is set (the allocation-free entry point), from consecutive 256-byte-aligned
+    // pieces of a fixed device region, reused from its start by every batch of shots.
+    struct Arena {
+        uint8_t *base = nullptr;
+        size_t cap = 0, used = 0;
+    };
+    Arena *arena = nullptr;
+
+    uint8_t *staging(DevBuf &b, size_t n) {
+        if (!arena) {
+            b.ensure(n);
+            return b.as<uint8_t>();
+        }
+        const size_t at = (arena->used + 255) & ~(size_t)255;
+        if (at + n > arena->cap) throw std::invalid_argument("staging exceeds the fixed scratch region");
+        arena->used = at + n;
+        return arena->base + at;
+    }
+    void staging_reset() {
+        if (arena) arena->used = 0;
+    }
+    // Shots per batch when every batch stages `row_bytes` per shot in at most three pieces: the
+    // arena's capacity, in whole units of `unit` shots (the RNG tile), so splitting a call into batches
+    // never changes its tile numbering. Without an arena, `limit`.
+    uint64_t arena_batch(uint64_t limit, size_t row_bytes, uint64_t unit) const {
+        if (!arena || row_bytes == 0) return limit;
+        const size_t usable = arena->cap > 3 * 256 ? arena->cap - 3 * 256 : 0;
+        const uint64_t fit = usable / row_bytes / unit * unit;
+        if (fit == 0)
+            throw std::invalid_argument("rows too wide for the fixed scratch region (" + std::to_string(row_bytes) +
+                                        " bytes of staging per shot, " + std::to_string(unit) + " shots per tile)");
+        return std::min(limit, fit);
+    }
 
     ~mfs_sampler() {
         if (ev0) (void)hipEventDestroy(ev0);
@@ -294,6 +328,10 @@ struct mfs_sampler {
         const size_t row_bytes = L.det_row + (L.separate ? L.obs_row : 0) + (L.via_unpacked ? L.ncols : 0);
         uint64_t batch = device_out && !L.via_unpacked ? shots
                                                         : std::max<uint64_t>(W64, (HOST_BATCH_BYTES / std::max<size_t>(row_bytes, 1)) / W64 * W64);
+        if (!(device_out && !L.via_unpacked)) {
+            const size_t staged = (L.via_unpacked ? L.ncols : 0) + (device_out ? 0 : L.det_row + (L.separate ? L.obs_row : 0));
+            batch = arena_batch(batch, staged, W64);
+        }
         batch = std::min(batch, shots);
         const bool fixed = faults != nullptr;
         const uint64_t call_tile0 = tiles_used;
@@ -306,25 +344,24 @@ struct mfs_sampler {
             P.mode = fixed ? MODE_FIXED : MODE_SAMPLED;
             P.randomize = (!fixed && randomize) ? 1 : 0;
             if (fixed) stage_faults(s0, n, tiles, fault_offsets, faults, P);
+            staging_reset();
             uint8_t *det_dst, *obs_dst;
             size_t det_stride_dev, obs_stride_dev;
             if (L.via_unpacked) {
-                d_tmp.ensure(n * L.ncols);
+                uint8_t *tmp = staging(d_tmp, n * L.ncols);
                 Layout U = L;
                 U.packed = false;
                 U.via_unpacked = false;
-                outputs(U, d_tmp.as<uint8_t>(), L.ncols, nullptr, 0, P);
+                outputs(U, tmp, L.ncols, nullptr, 0, P);
                 if (device_out) {
                     det_dst = dets + s0 * dets_stride;
                     det_stride_dev = dets_stride;
                 } else {
-                    d_out_det.ensure(n * L.det_row);
-                    det_dst = d_out_det.as<uint8_t>();
+                    det_dst = staging(d_out_det, n * L.det_row);
                     det_stride_dev = L.det_row;
                 }
                 launch(P, tiles);
-                check(launch_pack_rows(d_tmp.as<uint8_t>(), L.ncols, det_dst, det_stride_dev, n, L.ncols, stream),
-                      "pack rows");
+                check(launch_pack_rows(tmp, L.ncols, det_dst, det_stride_dev, n, L.ncols, stream), "pack rows");
                 obs_dst = nullptr;
                 obs_stride_dev = 0;
             } else {
@@ -334,11 +371,9 @@ struct mfs_sampler {
                     obs_dst = (L.separate && obs) ? obs + s0 * obs_stride : nullptr;
                     obs_stride_dev = obs_stride;
                 } else {
-                    d_out_det.ensure(n * std::max<size_t>(L.det_row, 1));
-                    det_dst = d_out_det.as<uint8_t>();
+                    det_dst = staging(d_out_det, n * std::max<size_t>(L.det_row, 1));
                     det_stride_dev = L.det_row;
-                    d_out_obs.ensure(n * std::max<size_t>(L.obs_row, 1));
-                    obs_dst = L.separate ? d_out_obs.as<uint8_t>() : nullptr;
+                    obs_dst = L.separate ? staging(d_out_obs, n * std::max<size_t>(L.obs_row, 1)) : nullptr;
                     obs_stride_dev = L.obs_row;
                 }
                 if (L.packed && (L.det_row * 8 != L.ncols)) {
@@ -703,6 +738,10 @@ struct mfs_sampler {
             }
             if (column_count > 2 * shots) return false;
         }
+        if (arena && sym_state == 0)
+            throw std::invalid_argument(
+                "the sampler's symptom table is not built yet (building it allocates device memory); prepare the "
+                "sampler first with mfs_engine_info or one allocating call");
         const bool ok = ensure_symptoms();
         if (!ok && engine_mode == 2) throw std::invalid_argument("symptom engine unavailable: " + sym_why);
         return ok;
@@ -818,9 +857,16 @@ struct mfs_sampler {
         const uint32_t O = prog.num_observables;
         const size_t pdet = (L.ncols + 7) / 8, pobs = (O + 7) / 8;
         const bool want_obs = L.separate && obs && O;
-        const uint64_t batch = device_out && L.packed
-                                   ? shots
-                                   : std::max<uint64_t>(1024, HOST_BATCH_BYTES / std::max<size_t>(pdet + pobs + L.ncols + O, 1));
+        uint64_t batch = shots;
+        if (!(device_out && L.packed)) {
+            // Batches are whole tiles of the launch's tile size, so the split never changes the tiles'
+            // RNG streams.
+            static uint8_t dummy;
+            const uint64_t S = sym_prepare(L, 1, &dummy, pdet, want_obs ? &dummy : nullptr, pobs, 0).P.shots_per_tile;
+            batch = std::max<uint64_t>(1024, HOST_BATCH_BYTES / std::max<size_t>(pdet + pobs + L.ncols + O, 1));
+            batch = std::max<uint64_t>(S, batch / S * S);
+            batch = arena_batch(batch, pdet + (want_obs ? pobs : 0) + (device_out ? 0 : L.ncols + O), S);
+        }
         const uint64_t call_tile0 = tiles_used;
         uint64_t tiles_total = 0;
         for (uint64_t s0 = 0; s0 < shots; s0 += batch) {
@@ -836,12 +882,11 @@ struct mfs_sampler {
                     po_stride = obs_stride;
                 }
             } else {
-                d_out_det.ensure(n * std::max<size_t>(pdet, 1));
-                pd = d_out_det.as<uint8_t>();
+                staging_reset();
+                pd = staging(d_out_det, n * std::max<size_t>(pdet, 1));
                 pd_stride = pdet;
                 if (want_obs) {
-                    d_out_obs.ensure(n * std::max<size_t>(pobs, 1));
-                    po = d_out_obs.as<uint8_t>();
+                    po = staging(d_out_obs, n * std::max<size_t>(pobs, 1));
                     po_stride = pobs;
                 }
             }
@@ -855,10 +900,9 @@ struct mfs_sampler {
                 uint8_t *uo = device_out && want_obs ? obs + s0 * obs_stride : nullptr;
                 size_t ud_stride = dets_stride, uo_stride = obs_stride;
                 if (!device_out) {
-                    d_tmp.ensure(n * std::max<size_t>(L.ncols + O, 1));
-                    ud = d_tmp.as<uint8_t>();
+                    ud = staging(d_tmp, n * std::max<size_t>(L.ncols + O, 1));
                     ud_stride = L.ncols;
-                    uo = d_tmp.as<uint8_t>() + n * L.ncols;
+                    uo = ud + n * L.ncols;
                     uo_stride = O;
                 }
                 check(launch_unpack_rows(pd, pd_stride, ud, ud_stride, n, L.ncols, (uint8_t)base_char, stream), "unpack");
