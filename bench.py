@@ -5,7 +5,10 @@ Default workload (the headline): detector shots/s of the d=25, 25-round rotated 
 circuit at p=1e-3 on one MI355X. One step = one sample of 2^20 shots (bit-packed detection events +
 separate observables) into HBM-resident output rows.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots S] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--shots S] [--no-cpu] [--no-serve]
+
+The default line also carries the metric's second half: a bounded replay of the serving workload (the
+200-circuit pool, Poisson arrivals, --serve-requests requests at --qps) with served p50/p99 latency.
     python bench.py --workload batch64      # 64 heterogeneous surface codes: one persistent kernel vs 64 calls
     python bench.py --workload serve        # request loop with continuous batching, p50/p99 latency
     python bench.py --workload clifford     # 10,000-qubit, 1,000-layer random-Clifford circuit, 2^16 shots
@@ -164,6 +167,13 @@ def headline(args):
         engines[other] = {'shots_per_s': S / (alt_ms * 1e-3), 'kernel_ms': alt_ms}
     except ValueError:
         pass
+    if engine['engine'] == 'symptom':
+        # The generic scan kernel (basis-column records) on the same circuit, for comparison.
+        gen_s = circuit.compile_detector_sampler(seed=2028)
+        gen_s.set_engine('symptom')
+        gen_s.set_scan_variant('generic')
+        gen_ms = gen_s.bench_device(S, warmup=2, iters=10)
+        engines['symptom_generic_scan'] = {'shots_per_s': S / (gen_ms * 1e-3), 'kernel_ms': gen_ms}
     b_alg = S * (drow + orow)
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
     traffic = measured_traffic() if S == 1 << 20 and engine['engine'] == 'symptom' else None
@@ -197,6 +207,21 @@ def headline(args):
         'engines': engines,
         'cpu_baseline': cpu,
     }
+
+
+def add_serving(line, args):
+    """Bounded serving replay (the 200-circuit pool) attached to the headline line."""
+    from stim_amd import serve
+    sv = serve.bench(qps=args.qps, requests=args.serve_requests)
+    line['served_p50_ms'] = sv['latency_ms']['p50']
+    line['served_p99_ms'] = sv['latency_ms']['p99']
+    line['served_shots_per_s'] = sv['value']
+    line['served'] = {k: sv[k] for k in ('latency_ms', 'qps_achieved', 'output_GBps', 'roofline', 'gpu_busy_frac',
+                                          'steps', 'wall_s', 'compile_and_build_s', 'peak_bytes_in_use')}
+    line['served']['qps_offered'] = args.qps
+    line['served']['requests'] = args.serve_requests
+    line['served']['data'] = sv['data']
+    return line
 
 
 def batch64_configs(seed=2026, n=64):
@@ -294,10 +319,51 @@ def batch64(args):
         'plain_64_calls_wall_ms': plain_ms,
         'speedup_vs_plain': plain_ms / wall_ms,
         'achieved_GBps': total_bytes / (dev_ms * 1e-3) / 1e9,
+        'roofline': {'bound': 'hbm', 'achieved': total_bytes / (dev_ms * 1e-3) / 1e9, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': total_bytes / (dev_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 'traffic': None,
+                     'kernel': 'sym_batch_kernel', 'kernel_ms': dev_ms, 'algorithmic_bytes_per_launch': total_bytes},
         'bit_exact_vs_plain': mismatches == 0,
         'mismatched_jobs': mismatches,
         'compile_and_build_s': compile_s,
+        'cpu_baseline': None if args.no_cpu else jobs_cpu_baseline(texts, shots, 256, 'batch64: 64 surface codes'),
     }
+
+
+def jobs_cpu_baseline(texts, shots, scale, what):
+    """The reference CLI `stim detect` over a job mix (circuit text, shots), each job's shots divided by
+    `scale` (at least 1,024), run by one worker per core pulling jobs in turn; shots/s = sampled shots /
+    wall time."""
+    import tempfile
+    import concurrent.futures as cf
+    if not (os.path.exists(REF_STIM) and os.access(REF_STIM, os.X_OK)):
+        return None
+    cores = _cores()
+    with tempfile.TemporaryDirectory() as td:
+        paths = {}
+        jobs = []
+        for t, n in zip(texts, shots):
+            if t not in paths:
+                paths[t] = os.path.join(td, f'c{len(paths)}.stim')
+                with open(paths[t], 'w') as f:
+                    f.write(t)
+            jobs.append((paths[t], max(1024, int(n) // scale)))
+
+        def one(job):
+            path, n = job
+            r = subprocess.run([REF_STIM, 'detect', '--shots', str(n), '--in', path, '--out_format', 'b8',
+                                '--out', '/dev/null', '--obs_out', '/dev/null', '--obs_out_format', 'b8'],
+                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, stdin=subprocess.DEVNULL)
+            return r.returncode == 0
+        t0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(cores) as ex:
+            ok = all(ex.map(one, jobs))
+        wall = time.perf_counter() - t0
+    if not ok:
+        return None
+    total = sum(n for _, n in jobs)
+    return _baseline(total / wall, cores, f'stim detect b8 to /dev/null over the {what} job mix, each job at '
+                     f'1/{scale} of its shots (at least 1,024): {len(jobs)} jobs, {total} shots, {cores} workers, '
+                     f'wall {wall:.2f}s')
 
 
 def clifford(args):
@@ -323,6 +389,7 @@ def clifford(args):
         N.check(L.mfs_sample(sampler._h, S, flags, dets.data_ptr(), dets.stride(0), obs.data_ptr(), obs.stride(0)))
     torch.cuda.synchronize()
     wall = (time.perf_counter() - t0) / steps
+    kernel_ms = sampler.bench_device(S, warmup=1, iters=steps)
     info = sampler.info()
     return {
         'metric': 'detector shots/s, random-Clifford stress circuit (10,000 qubits x 1,000 layers)',
@@ -344,8 +411,34 @@ def clifford(args):
                    'tile_words': info['tile_words'], 'lds_bytes_per_tile': info['lds_bytes'],
                    'device_ops': info['ops'], 'parallelism': 'single GPU'},
         'output_GBps': S * (drow + orow) / wall / 1e9,
+        'roofline': {'bound': 'hbm', 'achieved': S * (drow + orow) / (kernel_ms * 1e-3) / 1e9, 'peak': HBM_PEAK_GBS,
+                     'unit': 'GB/s', 'frac': S * (drow + orow) / (kernel_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     'traffic': None, 'kernel': 'frame_sim_kernel<1>', 'kernel_ms': kernel_ms,
+                     'algorithmic_bytes_per_launch': S * (drow + orow)},
         'generate_and_compile_s': prep_s,
+        'cpu_baseline': None if args.no_cpu else clifford_cpu_baseline(text),
     }
+
+
+def clifford_cpu_baseline(text):
+    """The reference CLI `stim detect` on the same 10,000-qubit circuit, one process per core, 1,024
+    shots each (one batch of the reference's 256-bit SIMD words)."""
+    import tempfile
+    if not (os.path.exists(REF_STIM) and os.access(REF_STIM, os.X_OK)):
+        return None
+    cores, shots = _cores(), 1024
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, 'clifford.stim')
+        with open(path, 'w') as f:
+            f.write(text)
+        ok, wall = _run_parallel([[REF_STIM, 'detect', '--shots', str(shots), '--in', path, '--out_format', 'b8',
+                                   '--out', '/dev/null', '--obs_out', '/dev/null', '--obs_out_format', 'b8',
+                                   '--seed', str(k)] for k in range(cores)])
+    if not ok:
+        return None
+    return _baseline(cores * shots / wall, cores, f'stim detect b8 to /dev/null, {cores} processes x {shots} shots '
+                     f'(random Clifford, 10,000 qubits x 1,000 layers, includes parsing the circuit), '
+                     f'wall {wall:.2f}s')
 
 
 def dem_cpu_baseline(text):
@@ -500,6 +593,8 @@ def measure(args):
     t0 = time.perf_counter()
     sampler = circuit.compile_sampler(seed=2026)
     compile_s = time.perf_counter() - t0
+    if args.tile_words:
+        sampler.set_tile_words(args.tile_words)
     M, S = sampler.num_measurements, args.shots
     row = (M + 7) // 8
     out = torch.empty((S, row), dtype=torch.uint8, device='cuda:0')
@@ -592,6 +687,9 @@ def main():
                     help='run a 1 GiB device-to-device copy first (reference for FETCH_SIZE/WRITE_SIZE counters)')
     ap.add_argument('--qps', type=float, default=300.0)
     ap.add_argument('--requests', type=int, default=3000)
+    ap.add_argument('--serve-requests', type=int, default=1000,
+                    help='requests of the serving replay attached to the default line')
+    ap.add_argument('--no-serve', action='store_true', help='leave the serving replay out of the default line')
     args = ap.parse_args()
     if args.gpus != 1:
         print('bench.py measures one GPU only: this build has no multi-GPU path (shots would shard across GPUs '
@@ -599,6 +697,8 @@ def main():
         sys.exit(2)
     if args.workload == 'surface_d25':
         line = headline(args)
+        if not args.no_serve:
+            line = add_serving(line, args)
     elif args.workload == 'batch64':
         args.steps = min(args.steps, 5)
         line = batch64(args)
@@ -614,7 +714,12 @@ def main():
         line = m2d(args)
     else:
         from stim_amd import serve
-        line = serve.bench(qps=args.qps, requests=args.requests)
+        trace = {}
+        line = serve.bench(qps=args.qps, requests=args.requests, trace=trace)
+        if not args.no_cpu:
+            texts = [trace['pool'][c][1] for c in trace['chosen'][:64]]
+            line['cpu_baseline'] = jobs_cpu_baseline(texts, trace['shots'][:64], 64,
+                                                     'serving trace (first 64 requests)')
     print(json.dumps(line), flush=True)
 
 

@@ -70,16 +70,18 @@ int mfs_parse_stats(const char *stim_text, size_t len, uint64_t out[10]);
 int mfs_compile_measurements(const char *stim_text, size_t len, uint64_t seed, int has_seed, int skip_reference_sample,
                              const uint8_t *reference_sample_bits, size_t reference_len, mfs_sampler **out);
 
-/* Measurement-to-detection-event converter (no sweep data). Compiles the circuit's detectors and
- * observables and its reference sample (all zeros with skip_reference_sample = 1).
+/* Measurement-to-detection-event converter. Compiles the circuit's detectors and observables, its
+ * reference sample (all zeros with skip_reference_sample = 1) and, for circuits with sweep-controlled
+ * gates, each sweep bit's effect on them.
  * Reference: Circuit.compile_m2d_converter (src/stim/circuit/circuit.pybind.cc:781-785) ->
  * CompiledMeasurementsToDetectionEventsConverter (src/stim/simulators/measurements_to_detection_events.pybind.cc:159-166)
  * and the CLI's `stim m2d` (src/stim/cmd/command_m2d.cc). */
 typedef struct mfs_m2d mfs_m2d;
 int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample, mfs_m2d **out);
 void mfs_m2d_destroy(mfs_m2d *c);
-/* out[0..2] = num_measurements, num_detectors, num_observables. */
-int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[3]);
+/* out[0..3] = num_measurements, num_detectors, num_observables, num_sweep_bits (largest sweep[k]
+ * index plus one). */
+int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[4]);
 /* Converts `shots` bit-packed little-endian measurement rows (meas_stride bytes apart) into detection
  * event rows (with observables appended when MFS_APPEND_OBSERVABLES) and, with
  * MFS_SEPARATE_OBSERVABLES, observable rows; MFS_BIT_PACKED selects packed outputs (else one byte per
@@ -93,6 +95,16 @@ int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[3]);
  * (src/stim/simulators/measurements_to_detection_events.inl:29-131). */
 int mfs_m2d_convert(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_t meas_stride, uint32_t flags, uint8_t *dets,
                     size_t dets_stride, uint8_t *obs, size_t obs_stride);
+/* mfs_m2d_convert with sweep data: sweep holds `shots` bit-packed little-endian rows of
+ * ceil(num_sweep_bits / 8) bytes, sweep_stride bytes apart (device memory with MFS_DEVICE_OUTPUT), or
+ * is NULL for all-zero sweep bits. Each sweep bit flips the detectors and observables that the
+ * noiseless circuit's gates controlled by it reach (tabulated when the converter is compiled).
+ * Reference: convert(..., sweep_bits=...) and convert_file(..., sweep_bits_filepath, sweep_bits_format)
+ * (measurements_to_detection_events.pybind.cc:44-157), `stim m2d --sweep` (command_m2d.cc:21-75), and the
+ * sweep-driven frame simulator of measurements_to_detection_events.inl:57-75, 108-114. */
+int mfs_m2d_convert_sweep(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_t meas_stride, const uint8_t *sweep,
+                          size_t sweep_stride, uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs,
+                          size_t obs_stride);
 /* Benchmark helper (no reference counterpart): average kernel time in ms of converting `shots`
  * device-resident rows (observables appended), measured with HIP events on the converter's stream. */
 int mfs_m2d_bench(mfs_m2d *c, uint64_t shots, int warmup, int iters, double out_ms[1]);
@@ -172,6 +184,14 @@ int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb);
  * errors (recorded_errors_to_replay): a fault (e, 0, 1) fires mechanism e, counted over error
  * instructions in REPEAT-unrolled order. Every mechanism fires independently with its own probability. */
 int mfs_dem_compile(const char *dem_text, size_t len, uint64_t seed, int has_seed, mfs_sampler **out);
+/* Samples like mfs_sample (observables separate) and also returns which mechanisms fired: errs holds
+ * `shots` rows of num_errors entries (bit-packed little-endian with MFS_BIT_PACKED, else one byte per
+ * mechanism), errs_stride bytes apart, mechanism e counted over error instructions in REPEAT-unrolled
+ * order. Reference: CompiledDemSampler.sample(shots, bit_packed, return_errors=True) and the err_out
+ * file of sample_write (dem_sampler.pybind.cc:45-79, dem_sampler.inl:75-99, where err_buffer holds each
+ * mechanism's fired bits). */
+int mfs_dem_sample_errors(mfs_sampler *s, uint64_t shots, uint32_t flags, uint8_t *dets, size_t dets_stride,
+                          uint8_t *obs, size_t obs_stride, uint8_t *errs, size_t errs_stride);
 /* Parses on the host only: out[0..2] = detectors, observables, error mechanisms. */
 int mfs_dem_parse_stats(const char *dem_text, size_t len, uint64_t out[3]);
 
@@ -234,8 +254,9 @@ int mfs_server_copy_result(mfs_server *v, uint64_t id, uint8_t *dets, size_t det
                            size_t obs_stride);
 /* Frees a finished (or still queued) request and its rows. */
 int mfs_server_release(mfs_server *v, uint64_t id);
-/* out[0..5] = steps, queued, running, bytes in use, peak bytes in use, budget. */
-int mfs_server_stats(mfs_server *v, uint64_t out[6]);
+/* out[0..7] = steps, queued, running, bytes in use, peak bytes in use, budget, device time of the
+ * serving launches in microseconds (HIP events), row bytes those launches wrote. */
+int mfs_server_stats(mfs_server *v, uint64_t out[8]);
 /* Replays a request trace in real time: request i (samplers[i], shots[i], seeds[i]) arrives
  * arrival_s[i] seconds after the call starts (sorted); each result is released as soon as it is
  * complete. latency_ms[i] = finish - arrival; *wall_s = duration of the whole replay. */
@@ -245,6 +266,13 @@ int mfs_server_replay(mfs_server *v, uint64_t n, mfs_sampler *const *samplers, c
 /* Configuration knobs: words per tile (1, 2 or 4; 0 = automatic). Returns the value in use. */
 int mfs_set_tile_words(mfs_sampler *s, int words);
 int mfs_get_info(const mfs_sampler *s, uint64_t out[8]); /* tile_words, lds_bytes, ops, syncs, noise_ops, randomize, ring, dense_qubits */
+
+/* Symptom-engine scan kernel: 0 (default) = the specialised kernel with one precomputed outcome record
+ * per (noise site, outcome) where the model allows it (plain groups with uniform outcomes, as in the
+ * QEC memory circuits), else the generic kernel; 1 = always the generic kernel, which applies a hit's
+ * basis-column records one by one. Both draw the same hits, so rows are bit-identical. No reference
+ * counterpart. */
+int mfs_set_scan_variant(mfs_sampler *s, int variant);
 
 /* Position of a sampler's random stream: the global shot-tile index its next call starts at. Every
  * call advances it by the tiles it used, so consecutive calls draw fresh randomness; setting it back

@@ -148,41 +148,41 @@ class FrameOracle:
             elif n in st.RESET_1Q:
                 for t in T:
                     q = t.value
+                    # The commuting component is kept, and randomized only when randomizing (XOR with
+                    # uniform bits has the distribution of the reference's fresh random bits).
                     r = self._rand_bits(rng, S) if self.randomize else np.zeros(S, dtype=bool)
-                    if n == 'R':        # do_RZ :281-289
+                    if n == 'R':        # do_RZ :266-274
                         x[q] = False
-                        z[q] = r
+                        z[q] ^= r
                     elif n == 'RX':     # do_RX :210-219
                         z[q] = False
-                        x[q] = r
-                    else:               # do_RY :270-279
-                        z[q] = r
-                        x[q] = r.copy()
+                        x[q] ^= r
+                    else:               # do_RY :255-263
+                        z[q] ^= r
+                        x[q] = z[q].copy()
             elif n in st.MEASURE_1Q:
                 p = a[0] if a else 0.0
                 fl = flips_for(k, len(T)) if fixed else self._bern(rng, p, (len(T), S))
                 for i, t in enumerate(T):
                     q = t.value
                     r = self._rand_bits(rng, S) if self.randomize else np.zeros(S, dtype=bool)
+                    # do_MZ/MX/MY :173-208 and do_MRX/MRY/MRZ :276-317: without randomization the
+                    # commuting component of a measured or reset qubit is kept.
                     if n in ('M', 'MR'):
                         rec.append(x[q] ^ fl[i])
                         if n == 'MR':
                             x[q] = False
-                            z[q] = r
-                        else:
-                            z[q] ^= r
+                        z[q] ^= r
                     elif n in ('MX', 'MRX'):
                         rec.append(z[q] ^ fl[i])
                         if n == 'MRX':
                             z[q] = False
-                            x[q] = r
-                        else:
-                            x[q] ^= r
+                        x[q] ^= r
                     else:  # MY, MRY
                         rec.append(x[q] ^ z[q] ^ fl[i])
                         if n == 'MRY':
-                            z[q] = r
-                            x[q] = r.copy()
+                            z[q] ^= r
+                            x[q] = z[q].copy()
                         else:
                             x[q] ^= r
                             z[q] ^= r

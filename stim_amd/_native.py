@@ -32,7 +32,7 @@ EXPORTED = [
     'mfs_dem_compile', 'mfs_dem_parse_stats', 'mfs_reference_sample', 'mfs_compile_measurements',
     'mfs_m2d_compile', 'mfs_m2d_destroy', 'mfs_m2d_stats', 'mfs_m2d_convert', 'mfs_m2d_bench',
     'mfs_set_stream_position', 'mfs_get_stream_position', 'mfs_rng_counter', 'mfs_philox', 'mfs_build_id',
-    'mfs_sample_noalloc',
+    'mfs_sample_noalloc', 'mfs_m2d_convert_sweep', 'mfs_dem_sample_errors', 'mfs_set_scan_variant',
 ]
 
 _lib = None
@@ -65,6 +65,7 @@ def lib():
     L.mfs_m2d_destroy.restype = None
     L.mfs_m2d_stats.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_m2d_convert.argtypes = [p, u64, p, sz, u32, p, sz, p, sz]
+    L.mfs_m2d_convert_sweep.argtypes = [p, u64, p, sz, p, sz, u32, p, sz, p, sz]
     L.mfs_m2d_bench.argtypes = [p, u64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     L.mfs_compile_measurements.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.c_int, p, sz,
                                            ctypes.POINTER(p)]
@@ -106,7 +107,9 @@ def lib():
                                     ctypes.POINTER(dbl), ctypes.POINTER(dbl), ctypes.POINTER(dbl)]
     L.mfs_dem_compile.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.POINTER(p)]
     L.mfs_dem_parse_stats.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(u64)]
+    L.mfs_dem_sample_errors.argtypes = [p, u64, u32, p, sz, p, sz, p, sz]
     L.mfs_set_stream_position.argtypes = [p, u64]
+    L.mfs_set_scan_variant.argtypes = [p, ctypes.c_int]
     L.mfs_get_stream_position.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_rng_counter.argtypes = [ctypes.c_int, u64, u64, u32, u32, u32, ctypes.POINTER(u32)]
     L.mfs_philox.argtypes = [p, u64, u64, p]

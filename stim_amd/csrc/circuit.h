@@ -59,6 +59,7 @@ struct Instr {
     Gate gate;
     std::vector<double> args;
     std::vector<Target> targets;
+    uint32_t tag = 0;   // marker set by host-side circuit rewrites (0 = none); the parser leaves it 0
 };
 
 struct Block;

@@ -13,14 +13,137 @@
 
 namespace sa {
 hipError_t launch_m2d(const M2dParams &P, hipStream_t stream);
-size_t m2d_lds_bytes(uint32_t blocks);
+size_t m2d_lds_bytes(uint32_t blocks, uint32_t sweep_blocks);
 }  // namespace sa
+
+namespace {
+
+// Number of sweep bits a circuit reads: the largest sweep[k] index plus one (the reference's
+// CircuitStats::num_sweep_bits).
+uint32_t count_sweep_bits(const std::vector<const Instr *> &flat) {
+    uint64_t k = 0;
+    for (const Instr *in : flat) {
+        for (const Target &t : in->targets) {
+            if (t.kind == TK::SWEEP) k = std::max<uint64_t>(k, (uint64_t)t.value + 1);
+        }
+    }
+    if (k >= (1ull << 24)) throw std::invalid_argument("too many sweep bits for the converter");
+    return (uint32_t)k;
+}
+
+// Copies a block, replacing each sweep-controlled pair of CX/CY/CZ/XCZ/YCZ by a one-qubit error
+// instruction on its qubit, tagged k + 1 for sweep bit k: the frame flip the reference's frame
+// simulator applies when that sweep bit is set (xor_control_bit_into, frame_simulator.inl:139-150:
+// CX and XCZ flip X, CY and YCZ flip X and Z, CZ flips Z; single_cx/single_cy/do_ZCZ at :386-474,
+// do_XCZ/do_YCZ at :592-630 with their pairs reversed). Pairs of two classical bits do nothing; the
+// other pairs keep their order, split into instructions of the same gate.
+void split_sweep_controls(const Block &src, Block &dst) {
+    for (const Item &it : src.items) {
+        if (it.is_repeat) {
+            Item r;
+            r.is_repeat = true;
+            r.reps = it.reps;
+            r.body = std::make_shared<Block>();
+            split_sweep_controls(*it.body, *r.body);
+            dst.items.push_back(std::move(r));
+            continue;
+        }
+        const Instr &in = it.instr;
+        const Gate g = in.gate;
+        bool has_sweep = false;
+        for (const Target &t : in.targets) has_sweep |= t.kind == TK::SWEEP;
+        const bool controllable = g == Gate::CX || g == Gate::CY || g == Gate::CZ || g == Gate::XCZ || g == Gate::YCZ;
+        if (!has_sweep || !controllable) {
+            dst.items.push_back(it);
+            continue;
+        }
+        Instr pending{g, in.args, {}, 0};
+        auto flush = [&]() {
+            if (pending.targets.empty()) return;
+            Item x;
+            x.instr = pending;
+            dst.items.push_back(std::move(x));
+            pending.targets.clear();
+        };
+        for (size_t i = 0; i + 1 < in.targets.size(); i += 2) {
+            const Target &a = in.targets[i], &b = in.targets[i + 1];
+            if (a.kind != TK::SWEEP && b.kind != TK::SWEEP) {
+                pending.targets.push_back(a);
+                pending.targets.push_back(b);
+                continue;
+            }
+            // Control and target of the pair in the reference's single_cx / single_cy orientation.
+            const bool reversed = g == Gate::XCZ || g == Gate::YCZ;
+            const Target &ctl = reversed ? b : a, &tgt = reversed ? a : b;
+            Gate err;
+            Target q;
+            if (g == Gate::CZ) {
+                if (a.kind == TK::SWEEP && b.kind == TK::QUBIT) q = b;
+                else if (b.kind == TK::SWEEP && a.kind == TK::QUBIT) q = a;
+                else continue;
+                err = Gate::Z_ERROR;
+            } else {
+                if (ctl.kind != TK::SWEEP || tgt.kind != TK::QUBIT) {
+                    if (tgt.kind != TK::QUBIT) throw std::invalid_argument("a controlled gate had a bit as its target");
+                    continue;
+                }
+                q = tgt;
+                err = (g == Gate::CX || g == Gate::XCZ) ? Gate::X_ERROR : Gate::Y_ERROR;
+            }
+            const uint32_t k = (ctl.kind == TK::SWEEP) ? ctl.value : (a.kind == TK::SWEEP ? a.value : b.value);
+            flush();
+            Item x;
+            x.instr = Instr{err, {0.5}, {q}, k + 1};
+            dst.items.push_back(std::move(x));
+        }
+        flush();
+    }
+}
+
+// Effect of each sweep bit on the circuit's outputs: rows[k] = the detectors then observables (bit
+// packed, little-endian, D + O bits) that sweep bit k flips. Runs the frame kernel in fixed-fault
+// mode on the circuit with its sweep controls turned into error sites, one shot per sweep bit with
+// every site of that bit fired (no frame randomization, no other noise: the reference's noiseless,
+// non-randomizing frame simulator of measurements_to_detection_events.inl:57-63).
+std::vector<uint8_t> sweep_effect_rows(const Block &block, uint32_t K, uint32_t D, uint32_t O, size_t row) {
+    auto s = std::make_unique<mfs_sampler>();
+    split_sweep_controls(block, s->block);
+    s->flat = flatten(s->block);
+    s->prog = compile_program(s->flat);
+    s->key = 0;
+    std::vector<std::vector<uint32_t>> per_bit(K);
+    for (size_t i = 0; i < s->flat.size(); i++) {
+        const Instr *in = s->flat[i];
+        if (!in->tag) continue;
+        const uint32_t code = in->gate == Gate::X_ERROR ? 1u : in->gate == Gate::Z_ERROR ? 2u : 3u;
+        per_bit[in->tag - 1].insert(per_bit[in->tag - 1].end(), {(uint32_t)i, 0u, code});
+    }
+    std::vector<uint64_t> offs(K + 1, 0);
+    std::vector<uint32_t> faults;
+    for (uint32_t k = 0; k < K; k++) {
+        faults.insert(faults.end(), per_bit[k].begin(), per_bit[k].end());
+        offs[k + 1] = faults.size() / 3;
+    }
+    if (faults.empty()) return std::vector<uint8_t>((size_t)K * row, 0);
+    check(hipStreamCreate(&s->stream), "stream");
+    s->pick_tile_words(0);
+    s->upload_program();
+    s->engine_mode = 1;
+    std::vector<uint8_t> rows((size_t)K * row, 0);
+    // No error site at all (sweep bits that only meet classical bits): every effect is empty.
+    if (D + O && !faults.empty())
+        s->sample(K, MFS_BIT_PACKED | MFS_APPEND_OBSERVABLES, rows.data(), row, nullptr, 0, offs.data(), faults.data());
+    return rows;
+}
+
+}  // namespace
 
 struct mfs_m2d {
     uint32_t M = 0, D = 0, O = 0;
+    uint32_t K = 0;   // sweep bits the circuit reads
     std::vector<uint2> cols;
     std::vector<uint32_t> ovf;
-    DevBuf d_cols, d_ovf, d_in, d_out, d_obs, d_tmp;
+    DevBuf d_cols, d_ovf, d_in, d_out, d_obs, d_tmp, d_sweep;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -31,6 +154,8 @@ struct mfs_m2d {
     }
 
     uint32_t blocks() const { return (M + 31) / 32; }
+    uint32_t sweep_blocks() const { return (K + 31) / 32; }
+    size_t sweep_row() const { return (K + 7) / 8; }
     size_t in_row() const { return (M + 7) / 8; }
     size_t in_vec_stride() const { return (size_t)16 * ((blocks() + 3) / 4); }
 
@@ -52,6 +177,10 @@ struct mfs_m2d {
         P.shots = shots;
         P.cols = const_cast<DevBuf &>(d_cols).as<uint2>();
         P.ovf = const_cast<DevBuf &>(d_ovf).as<uint32_t>();
+        P.sweep = nullptr;
+        P.sweep_stride = 0;
+        P.sweep_row_bytes = (uint32_t)sweep_row();
+        P.sweep_blocks = sweep_blocks();
         return P;
     }
 
@@ -80,11 +209,13 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
         c->M = (uint32_t)st.num_measurements;
         c->D = (uint32_t)st.num_detectors;
         c->O = (uint32_t)st.num_observables;
-        if (m2d_lds_bytes(c->blocks()) > MAX_LDS) {
+        c->K = count_sweep_bits(flat);
+        if (m2d_lds_bytes(c->blocks(), c->sweep_blocks()) > MAX_LDS) {
             uint32_t max_blocks = 0;
-            while (m2d_lds_bytes(max_blocks + 1) <= MAX_LDS) max_blocks++;
-            throw std::invalid_argument("the converter keeps a tile's measurements in LDS: at most " +
-                                        std::to_string((uint64_t)max_blocks * 32) + " measurements per shot");
+            while (m2d_lds_bytes(max_blocks + 1, c->sweep_blocks()) <= MAX_LDS) max_blocks++;
+            throw std::invalid_argument("the converter keeps a tile's measurements and sweep bits in LDS: at most " +
+                                        std::to_string((uint64_t)max_blocks * 32) + " measurements per shot with " +
+                                        std::to_string(c->K) + " sweep bits");
         }
         std::vector<uint8_t> ref(c->M, 0);
         if (!skip_reference_sample) ref = reference_sample(flat);
@@ -105,9 +236,31 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
             m += num_results(*in);
         }
         for (auto &ts : obs_terms) col_terms.push_back(std::move(ts));
-        for (const auto &ts : col_terms) {
-            uint8_t f = 0;
-            for (uint32_t idx : ts) f ^= ref[idx];
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+            throw HipError("no HIP device available: the stim_amd converter runs only on the GPU");
+        }
+        // The reference parity covers measurement terms only; sweep terms (word indices past the
+        // measurements) are appended after it is formed.
+        std::vector<uint8_t> invert(col_terms.size(), 0);
+        for (size_t j = 0; j < col_terms.size(); j++) {
+            for (uint32_t idx : col_terms[j]) invert[j] ^= ref[idx];
+        }
+        if (c->K) {
+            const size_t row = (c->D + c->O + 7) / 8;
+            const std::vector<uint8_t> eff = sweep_effect_rows(b, c->K, c->D, c->O, std::max<size_t>(row, 1));
+            const uint32_t w0 = m2d_sweep_word(c->blocks());
+            for (uint32_t k = 0; k < c->K; k++) {
+                const uint8_t *r = eff.data() + (size_t)k * std::max<size_t>(row, 1);
+                for (uint32_t j = 0; j < c->D + c->O; j++) {
+                    if ((r[j >> 3] >> (j & 7)) & 1) col_terms[j].push_back(w0 + k);
+                }
+            }
+        }
+        const uint32_t zero_word = m2d_zero_word(c->blocks(), c->sweep_blocks());
+        for (size_t j = 0; j < col_terms.size(); j++) {
+            const auto &ts = col_terms[j];
+            const uint8_t f = invert[j];
             uint2 r;
             if (ts.size() <= 2) {
                 r = make_uint2(((uint32_t)ts.size() << 28) | (ts.size() >= 1 ? ts[0] : 0u), ts.size() == 2 ? ts[1] : 0u);
@@ -116,14 +269,10 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
                     throw std::invalid_argument("too many measurement terms for the converter");
                 r = make_uint2((3u << 28) | (uint32_t)ts.size(), (uint32_t)c->ovf.size());
                 c->ovf.insert(c->ovf.end(), ts.begin(), ts.end());
-                while (c->ovf.size() % 4) c->ovf.push_back(m2d_zero_word(c->blocks()));
+                while (c->ovf.size() % 4) c->ovf.push_back(zero_word);
             }
             if (f) r.x |= M2D_INVERT;
             c->cols.push_back(r);
-        }
-        int count = 0;
-        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
-            throw HipError("no HIP device available: the stim_amd converter runs only on the GPU");
         }
         check(hipStreamCreate(&c->stream), "stream");
         mfs_sampler::upload(c->d_cols, c->cols, "m2d columns");
@@ -136,17 +285,26 @@ void mfs_m2d_destroy(mfs_m2d *c) {
     delete c;
 }
 
-int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[3]) {
+int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[4]) {
     return guarded([&] {
         out[0] = c->M;
         out[1] = c->D;
         out[2] = c->O;
+        out[3] = c->K;
     });
 }
 
 int mfs_m2d_convert(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_t meas_stride, uint32_t flags,
                     uint8_t *dets, size_t dets_stride, uint8_t *obs, size_t obs_stride) {
+    return mfs_m2d_convert_sweep(c, shots, meas, meas_stride, nullptr, 0, flags, dets, dets_stride, obs, obs_stride);
+}
+
+int mfs_m2d_convert_sweep(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_t meas_stride, const uint8_t *sweep,
+                          size_t sweep_stride, uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs,
+                          size_t obs_stride) {
     return guarded([&] {
+        if (sweep && c->K && sweep_stride < c->sweep_row()) throw std::invalid_argument("sweep bit rows are too short");
+        if (!c->K) sweep = nullptr;
         const bool append = flags & MFS_APPEND_OBSERVABLES, separate = flags & MFS_SEPARATE_OBSERVABLES;
         const bool packed = flags & MFS_BIT_PACKED, device = flags & MFS_DEVICE_OUTPUT;
         if (flags & MFS_PREPEND_OBSERVABLES) throw std::invalid_argument("the converter does not prepend observables");
@@ -157,13 +315,17 @@ int mfs_m2d_convert(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_t meas
         if (n_out && (!dets || dets_stride < (packed ? out_row : n_out))) throw std::invalid_argument("detection event rows are too short");
         if (n_obs && (!obs || obs_stride < (packed ? obs_row : n_obs))) throw std::invalid_argument("observable rows are too short");
         if (device && packed) {
-            c->launch(c->params(shots, meas, meas_stride, dets, dets_stride, n_out, obs, obs_stride, n_obs));
+            M2dParams P = c->params(shots, meas, meas_stride, dets, dets_stride, n_out, obs, obs_stride, n_obs);
+            P.sweep = sweep;
+            P.sweep_stride = sweep_stride;
+            c->launch(P);
             check(hipStreamSynchronize(c->stream), "m2d");
             return;
         }
         // Staged: batches of shots through device buffers (input rows at the vector stride).
         const size_t vs = std::max<size_t>(c->in_vec_stride(), 16);
-        const size_t per_shot = vs + out_row + obs_row + (packed ? 0 : n_out + n_obs);
+        const size_t srow = sweep ? c->sweep_row() : 0;
+        const size_t per_shot = vs + srow + out_row + obs_row + (packed ? 0 : n_out + n_obs);
         const uint64_t batch = std::max<uint64_t>(32, (HOST_BATCH_BYTES / std::max<size_t>(per_shot, 1)) / 32 * 32);
         for (uint64_t s0 = 0; s0 < shots; s0 += batch) {
             const uint64_t n = std::min(batch, shots - s0);
@@ -179,8 +341,17 @@ int mfs_m2d_convert(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_t meas
                                            hipMemcpyHostToDevice, c->stream), "copy measurements");
                 }
             }
-            c->launch(c->params(n, c->d_in.as<uint8_t>(), vs, c->d_out.as<uint8_t>(), std::max<size_t>(out_row, 1),
-                                n_out, c->d_obs.as<uint8_t>(), std::max<size_t>(obs_row, 1), n_obs));
+            M2dParams P = c->params(n, c->d_in.as<uint8_t>(), vs, c->d_out.as<uint8_t>(), std::max<size_t>(out_row, 1),
+                                    n_out, c->d_obs.as<uint8_t>(), std::max<size_t>(obs_row, 1), n_obs);
+            if (sweep) {
+                c->d_sweep.ensure(n * srow);
+                check(hipMemcpy2DAsync(c->d_sweep.p, srow, sweep + s0 * sweep_stride, sweep_stride, srow, n,
+                                       device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream),
+                      "copy sweep bits");
+                P.sweep = c->d_sweep.as<uint8_t>();
+                P.sweep_stride = srow;
+            }
+            c->launch(P);
             const hipMemcpyKind back = device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
             auto deliver = [&](DevBuf &src, size_t row, uint32_t ncols, uint8_t *dst, size_t dst_stride) {
                 if (!ncols) return;

@@ -1,9 +1,13 @@
 // Measurement-to-detection-event conversion: host structures shared with the kernel.
 //
 // Reference: measurements_to_detection_events_helper (src/stim/simulators/measurements_to_detection_events.inl:29-131).
-// Without sweep data, detector d of a shot is the XOR of the shot's measurement results that d names,
-// inverted when the XOR of those measurements' reference-sample bits is 1; observable k likewise over
-// the measurement records OBSERVABLE_INCLUDE(k) names (its Pauli targets add nothing).
+// Detector d of a shot is the XOR of the shot's measurement results that d names, inverted when the
+// XOR of those measurements' reference-sample bits is 1, XOR the shot's sweep bits that reach d;
+// observable k likewise over the measurement records OBSERVABLE_INCLUDE(k) names. A sweep bit reaches
+// an output through the gates it controls (CX/CY/CZ/XCZ/YCZ with a sweep[k] control flip a qubit's
+// frame; the reference tracks them with a noiseless FrameSimulator driven by the sweep table,
+// .inl:57-75): the noiseless circuit is linear in those flips, so each sweep bit's effect on the
+// outputs is a fixed set of columns, computed once when the converter is built.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -28,6 +32,10 @@ struct M2dParams {
     uint64_t shots;
     const uint2 *cols;          // column records over the D + O columns (below)
     const uint32_t *ovf;        // overflow term lists: count, then absolute measurement indices
+    const uint8_t *sweep;       // bit-packed sweep-bit rows (shot-major), or null: every sweep bit 0
+    uint64_t sweep_stride;
+    uint32_t sweep_row_bytes;   // ceil(K / 8) for the circuit's K sweep bits
+    uint32_t sweep_blocks;      // ceil(K / 32): LDS word m2d_sweep_word(blocks) + k holds sweep bit k
 };
 
 // Column record: bits 28..29 of x = term count class c, bit 31 = the column's reference parity
@@ -39,12 +47,16 @@ constexpr uint32_t M2D_INDEX_MASK = 0x0FFFFFFFu;
 constexpr uint32_t M2D_INVERT = 1u << 31;
 
 // LDS words of a tile: measurement blocks rounded up to whole 16-byte load groups (4 blocks of 32
-// words), then 4 words of zeros.
+// words), then the sweep-bit blocks, then 4 words of zeros. Column terms are word indices: measurement
+// m is word m, sweep bit k is word m2d_sweep_word(blocks) + k.
 __host__ __device__ inline uint32_t m2d_word_blocks(uint32_t blocks) {
     return (blocks + 3) / 4 * 4;
 }
-__host__ __device__ inline uint32_t m2d_zero_word(uint32_t blocks) {
+__host__ __device__ inline uint32_t m2d_sweep_word(uint32_t blocks) {
     return m2d_word_blocks(blocks) * 32;
+}
+__host__ __device__ inline uint32_t m2d_zero_word(uint32_t blocks, uint32_t sweep_blocks) {
+    return m2d_sweep_word(blocks) + sweep_blocks * 32;
 }
 
 }  // namespace sa

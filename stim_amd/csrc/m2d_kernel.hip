@@ -128,8 +128,18 @@ __global__ __launch_bounds__(NT) void m2d_kernel(M2dParams P) {
     // Lanes past the tile's shots read the tile's first row; their words are never used.
     const uint8_t *row = P.in + (s0 + (have ? lane : 0)) * P.in_stride;
     const uint32_t nblk = P.blocks;
-    const uint32_t zero = m2d_zero_word(nblk);
+    const uint32_t zero = m2d_zero_word(nblk, P.sweep_blocks);
     if (tid < 4) words[zero + tid] = 0;
+    if (P.sweep_blocks) {
+        // Sweep-bit words: each lane reads its shot's 4 bytes of a 32-bit block (zero without sweep
+        // data), transposed like the measurement words.
+        const uint8_t *srow = P.sweep ? P.sweep + (s0 + (have ? lane : 0)) * P.sweep_stride : nullptr;
+        const uint32_t sw0 = m2d_sweep_word(nblk);
+        for (uint32_t b = half; b < P.sweep_blocks; b += HALVES) {
+            const uint32_t x = (have && srow) ? load_word_bytes(srow, P.sweep_row_bytes, b) : 0u;
+            words[sw0 + b * 32 + lane] = transpose32(x, lane);
+        }
+    }
 
     if (P.in_vec) {
         // Groups of 4 blocks (16 bytes of each row); UNROLL groups per iteration. Loads past the last
@@ -168,15 +178,15 @@ __global__ __launch_bounds__(NT) void m2d_kernel(M2dParams P) {
 
 namespace sa {
 
-// LDS of a launch: the tile's measurement words and zero words plus the staged output chunk.
-size_t m2d_lds_bytes(uint32_t blocks) {
-    return ((size_t)m2d_zero_word(blocks) + 4) * 4 + 32 * PITCH;
+// LDS of a launch: the tile's measurement, sweep and zero words plus the staged output chunk.
+size_t m2d_lds_bytes(uint32_t blocks, uint32_t sweep_blocks) {
+    return ((size_t)m2d_zero_word(blocks, sweep_blocks) + 4) * 4 + 32 * PITCH;
 }
 
 hipError_t launch_m2d(const M2dParams &P, hipStream_t stream) {
     const uint64_t tiles = (P.shots + 31) / 32;
     if (tiles == 0) return hipSuccess;
-    const size_t lds = m2d_lds_bytes(P.blocks);
+    const size_t lds = m2d_lds_bytes(P.blocks, P.sweep_blocks);
     static size_t reserved = 0;
     const hipError_t e = reserve_lds(reinterpret_cast<const void *>(&m2d_kernel), lds, reserved);
     if (e != hipSuccess) return e;

@@ -236,6 +236,20 @@ int mfs_dem_compile(const char *dem_text, size_t len, uint64_t seed, int has_see
     });
 }
 
+int mfs_dem_sample_errors(mfs_sampler *s, uint64_t shots, uint32_t flags, uint8_t *dets, size_t dets_stride,
+                          uint8_t *obs, size_t obs_stride, uint8_t *errs, size_t errs_stride) {
+    return guarded([&] {
+        if (!s->dem_mode) throw std::invalid_argument("only a detector error model sampler returns errors");
+        if (flags & (MFS_PREPEND_OBSERVABLES | MFS_APPEND_OBSERVABLES))
+            throw std::invalid_argument("errors are returned with separate observables only");
+        const uint64_t E = s->dem_col.size();
+        const size_t erow = (flags & MFS_BIT_PACKED) ? (E + 7) / 8 : E;
+        if (E && (!errs || errs_stride < erow)) throw std::invalid_argument("error rows are too short");
+        s->sample(shots, flags | MFS_SEPARATE_OBSERVABLES, dets, dets_stride, obs, obs_stride, nullptr, nullptr, 0, false,
+                  E ? errs : nullptr, errs_stride);
+    });
+}
+
 int mfs_dem_parse_stats(const char *dem_text, size_t len, uint64_t out[3]) {
     return guarded([&] {
         const DemModel model = parse_dem(dem_text, dem_text ? len : 0);
@@ -455,6 +469,13 @@ int mfs_set_symptom_tile_kb(mfs_sampler *s, int kb) {
             throw std::invalid_argument("a detector error model sampler samples only with the scan kernel");
         s->sym_scan = kb > 0;
         s->sym_tile_kb = kb > 0 ? kb : -kb;
+    });
+}
+
+int mfs_set_scan_variant(mfs_sampler *s, int variant) {
+    return guarded([&] {
+        if (variant != 0 && variant != 1) throw std::invalid_argument("scan variant must be 0 (automatic) or 1 (generic)");
+        s->sym_variant = variant;
     });
 }
 

@@ -35,7 +35,7 @@ hipError_t launch_pack_rows(const uint8_t *src, size_t src_stride, uint8_t *dst,
 hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream);
 hipError_t launch_sym_fixed(const SymDevParams &P, const uint64_t *off, const uint32_t *cols, uint32_t tiles,
                             size_t lds, hipStream_t stream);
-hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream);
+hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream, bool fast);
 hipError_t launch_sym_batch(SymDevParams *jobs, const uint64_t *prefix, const uint64_t *advance, uint32_t njobs,
                             uint32_t *counter, uint32_t grid, size_t lds, hipStream_t stream);
 int sym_batch_blocks_per_cu(size_t lds);
@@ -135,6 +135,15 @@ struct mfs_sampler {
     uint64_t sym_entries = 0;
     double sym_build_ms = 0;
     DevBuf d_classes, d_sotab, d_recs, d_ovf, d_sym_tmp, d_fix_off, d_fix_cols, d_recs32, d_ovf32, d_groups, d_thr;
+    // Detector-error-model samplers: sampled site -> mechanism index, and the fired-mechanism rows of
+    // the launch in progress (set only while a call returns errors).
+    DevBuf d_site_mech, d_err, d_err_tmp;
+    // Outcome records of the specialised scan kernel (built with the symptom table when it applies).
+    DevBuf d_orecs, d_orec_base, d_orec_ovf;
+    bool sym_fast = false;
+    int sym_variant = 0;         // 0: specialised scan kernel where it applies; 1: generic scan kernel
+    uint32_t *err_words_cur = nullptr;
+    uint64_t err_stride_cur = 0;
     // Measurement samplers: the outputs are the measurement results (reference sample bits XORed
     // into the frame's flips through det_invert); they sample on the frame engine.
     bool meas_mode = false;
@@ -311,7 +320,7 @@ struct mfs_sampler {
     // Runs one sample call into host or device rows.
     void sample(uint64_t shots, uint32_t flags, uint8_t *dets, size_t dets_stride, uint8_t *obs, size_t obs_stride,
                 const uint64_t *fault_offsets = nullptr, const uint32_t *faults = nullptr, char base_char = 0,
-                bool newline = false) {
+                bool newline = false, uint8_t *errs = nullptr, size_t errs_stride = 0) {
         Layout L = layout(flags);
         if (L.separate && (L.prepend || L.append)) {
             throw std::invalid_argument(
@@ -321,7 +330,7 @@ struct mfs_sampler {
         if (shots == 0) return;
         if (use_symptoms(faults != nullptr, shots)) {
             sample_symptom(L, shots, device_out, dets, dets_stride, obs, obs_stride, fault_offsets, faults, base_char,
-                           newline);
+                           newline, errs, errs_stride);
             return;
         }
         const uint64_t W64 = W();
@@ -555,6 +564,7 @@ struct mfs_sampler {
             sym_ovf32 = !o32.empty();
             upload(d_recs32, r32, "compact symptom records");
             upload(d_ovf32, o32, "compact symptom overflow entries");
+            build_outcome_records(offs, ents);
         } else {
             // Without compact records a hit needs its class's column layout: one group per class.
             plan.groups.clear();
@@ -579,6 +589,87 @@ struct mfs_sampler {
         upload(d_groups, plan.groups, "symptom groups");
         sym_state = 1;
         return true;
+    }
+
+    // Outcome records for the specialised scan kernel: when every sampling group is a plain one (compact
+    // records, four slots per site, no thinning) whose members draw outcomes uniformly over codes
+    // 1..nout, each (site, code) gets the symptom of that outcome precomputed: the XOR of the basis
+    // columns the code selects (bit j = column j, as the generic kernel applies them), with entries
+    // that appear an even number of times removed. A hit then reads one record instead of four.
+    void build_outcome_records(const std::vector<uint32_t> &offs, const std::vector<uint32_t> &ents) {
+        sym_fast = false;
+        const uint32_t D = prog.num_detectors;
+        for (const SymGroup &g : plan.groups) {
+            if (g.flags != 0 || g.rstride != 4) return;
+            for (uint32_t k = 0; k < g.nmem; k++)
+                if (!(g.nout[k] & SYM_SEQ) || (g.nout[k] & 0xFFFFu) > 15) return;
+        }
+        std::vector<uint64_t> orec0(plan.classes.size());
+        uint64_t total = 0;
+        for (size_t c = 0; c < plan.classes.size(); c++) {
+            orec0[c] = total;
+            total += (uint64_t)plan.classes[c].nsites * (plan.classes[c].nout & 0xFFFFu);
+        }
+        // 8 bytes per outcome; beyond 1 GiB the four-slot records are the better trade.
+        if (total >= (1ull << 31) || total * 8 > ((uint64_t)1 << 30)) return;
+        std::vector<uint2> orec(std::max<uint64_t>(total, 1), make_uint2(0, 0));
+        std::vector<uint32_t> oovf;
+        std::vector<uint32_t> acc;
+        for (size_t c = 0; c < plan.classes.size(); c++) {
+            const SymClass &cl = plan.classes[c];
+            const uint32_t nout = cl.nout & 0xFFFFu;
+            for (uint64_t st = 0; st < cl.nsites; st++) {
+                for (uint32_t code = 1; code <= nout; code++) {
+                    acc.clear();
+                    for (uint32_t j = 0; j < cl.basis; j++) {
+                        if (!((code >> j) & 1)) continue;
+                        const uint64_t col = cl.col0 + st * cl.basis + j;
+                        acc.insert(acc.end(), ents.begin() + offs[col], ents.begin() + offs[col + 1]);
+                    }
+                    std::sort(acc.begin(), acc.end());
+                    size_t w = 0;
+                    for (size_t i = 0; i < acc.size();) {
+                        if (i + 1 < acc.size() && acc[i] == acc[i + 1]) {
+                            i += 2;
+                        } else {
+                            acc[w++] = acc[i++];
+                        }
+                    }
+                    acc.resize(w);
+                    uint32_t dets[4], nd = 0, obs = 0;
+                    bool spill = false;
+                    for (uint32_t e : acc) {
+                        if (e >= D) obs |= 1u << (e - D);
+                        else if (nd < 4) dets[nd++] = e;
+                        else spill = true;
+                    }
+                    uint2 r;
+                    if (!spill) {
+                        for (uint32_t i = nd; i < 4; i++) dets[i] = 0;
+                        r.x = dets[0] | (dets[1] << 14) | (nd << 28) | ((obs & 1u) << 31);
+                        r.y = dets[2] | (dets[3] << 14) | (((obs >> 1) & 1u) << 28);
+                    } else {
+                        if (oovf.size() >= (1u << 30)) return;
+                        r = make_uint2(0, SYM_OREC_OVERFLOW | (uint32_t)oovf.size());
+                        oovf.push_back((uint32_t)acc.size());
+                        oovf.insert(oovf.end(), acc.begin(), acc.end());
+                    }
+                    orec[orec0[c] + st * nout + code - 1] = r;
+                }
+            }
+        }
+        std::vector<uint32_t> base(4 * std::max<size_t>(plan.groups.size(), 1), 0);
+        for (size_t gi = 0; gi < plan.groups.size(); gi++) {
+            const SymGroup &g = plan.groups[gi];
+            for (uint32_t k = 0; k < g.nmem; k++) {
+                const uint32_t nout = g.nout[k] & 0xFFFFu;
+                base[4 * gi + k] = (uint32_t)(orec0[g.c0 + k] - (uint64_t)g.bound[k] * nout);
+            }
+        }
+        upload(d_orecs, orec, "outcome records");
+        upload(d_orec_base, base, "outcome record bases");
+        upload(d_orec_ovf, oovf, "outcome record overflow lists");
+        sym_fast = true;
     }
 
     // Builds the symptom tables of a detector error model (reference DemSampler,
@@ -706,6 +797,11 @@ struct mfs_sampler {
         upload(d_ovf, ovf, "symptom overflow entries");
         upload(d_groups, plan.groups, "symptom groups");
         upload(d_thr, thr, "thinning thresholds");
+        {
+            std::vector<uint32_t> site_mech(std::max<uint64_t>(E, 1), 0);
+            for (uint64_t j = 0; j < E; j++) site_mech[j] = plan.columns[j].op;
+            upload(d_site_mech, site_mech, "mechanism of each site");
+        }
         if (sym_compact) {
             sym_ovf32 = !o32.empty();
             upload(d_recs32, r32, "compact symptom records");
@@ -777,6 +873,12 @@ struct mfs_sampler {
         P.tile_log2 = lg;
         P.total_sites = plan.total_sites;
         P.seed = splitmix64(key ^ 0x5A5A5A5Aull);
+        P.orecs = nullptr;
+        P.orec_base = nullptr;
+        P.orec_ovf = nullptr;
+        P.err_words = err_words_cur;
+        P.err_stride_words = err_stride_cur;
+        P.site_mech = dem_mode ? d_site_mech.as<uint32_t>() : nullptr;
         return P;
     }
 
@@ -789,6 +891,7 @@ struct mfs_sampler {
         size_t lds_scan;     // dynamic LDS of the scan kernel
         size_t lds_queue;    // dynamic LDS of the queue and fixed-fault kernels
         bool scan_ok;        // tile positions fit the scan kernel's 31-bit range
+        bool fast;           // the specialised scan kernel applies (outcome records built)
     };
 
     // The scan kernel takes tiles of up to 8192 shots (rows of a few bytes); the queue and
@@ -834,6 +937,12 @@ struct mfs_sampler {
         r.tiles = (n + S - 1) / S;
         if (r.tiles >= (1ull << 31)) throw std::invalid_argument("too many shots in one launch");
         r.scan_ok = scan_ok;
+        r.fast = scan_ok && sym_fast && sym_variant == 0 && !P.err_words;
+        if (r.fast) {
+            P.orecs = d_orecs.as<uint2>();
+            P.orec_base = d_orec_base.as<uint32_t>();
+            P.orec_ovf = d_orec_ovf.as<uint32_t>();
+        }
         return r;
     }
 
@@ -842,7 +951,7 @@ struct mfs_sampler {
                         size_t obs_stride, uint64_t tile0, const uint64_t *fix_off, const uint32_t *fix_cols) {
         const SymLaunch r = sym_prepare(L, n, det_dst, det_stride, obs_dst, obs_stride, tile0, 0, sym_scan && !fix_off);
         if (r.scan_ok) {
-            check(launch_sym_scan(r.P, (uint32_t)r.tiles, r.lds_scan, stream), "symptom scan launch");
+            check(launch_sym_scan(r.P, (uint32_t)r.tiles, r.lds_scan, stream, r.fast), "symptom scan launch");
         } else if (fix_off) {
             check(launch_sym_fixed(r.P, fix_off, fix_cols, (uint32_t)r.tiles, r.lds_queue, stream), "symptom fixed launch");
         } else {
@@ -853,8 +962,12 @@ struct mfs_sampler {
 
     void sample_symptom(const Layout &L, uint64_t shots, bool device_out, uint8_t *dets, size_t dets_stride,
                         uint8_t *obs, size_t obs_stride, const uint64_t *fault_offsets, const uint32_t *faults,
-                        char base_char, bool newline) {
+                        char base_char, bool newline, uint8_t *errs = nullptr, size_t errs_stride = 0) {
         const uint32_t O = prog.num_observables;
+        // Fired-mechanism rows (detector error models, sampled mode): 32-bit words per row on the device.
+        const bool want_err = errs && dem_mode && !faults;
+        const uint64_t E = dem_col.size();
+        const size_t ew = (E + 31) / 32, erow = L.packed ? (E + 7) / 8 : E;
         const size_t pdet = (L.ncols + 7) / 8, pobs = (O + 7) / 8;
         const bool want_obs = L.separate && obs && O;
         uint64_t batch = shots;
@@ -866,6 +979,13 @@ struct mfs_sampler {
             batch = std::max<uint64_t>(1024, HOST_BATCH_BYTES / std::max<size_t>(pdet + pobs + L.ncols + O, 1));
             batch = std::max<uint64_t>(S, batch / S * S);
             batch = arena_batch(batch, pdet + (want_obs ? pobs : 0) + (device_out ? 0 : L.ncols + O), S);
+        }
+        if (want_err) {
+            // Error rows take 32-bit words per mechanism block; batches stay whole tiles.
+            static uint8_t dummy;
+            const uint64_t S = sym_prepare(L, 1, &dummy, pdet, want_obs ? &dummy : nullptr, pobs, 0).P.shots_per_tile;
+            const size_t per = 4 * ew + (L.packed ? 0 : E) + pdet + pobs;
+            batch = std::min<uint64_t>(batch, std::max<uint64_t>(S, HOST_BATCH_BYTES / std::max<size_t>(per, 1) / S * S));
         }
         const uint64_t call_tile0 = tiles_used;
         uint64_t tiles_total = 0;
@@ -893,7 +1013,37 @@ struct mfs_sampler {
             const uint64_t *fo = nullptr;
             const uint32_t *fc = nullptr;
             if (faults) stage_symptom_faults(s0, n, fault_offsets, faults, &fo, &fc);
+            if (want_err) {
+                d_err.ensure(n * ew * 4);
+                check(hipMemsetAsync(d_err.p, 0, n * ew * 4, stream), "clear error rows");
+                err_words_cur = d_err.as<uint32_t>();
+                err_stride_cur = ew;
+            }
+            struct ClearErr {
+                mfs_sampler *s;
+                ~ClearErr() {
+                    s->err_words_cur = nullptr;
+                    s->err_stride_cur = 0;
+                }
+            } clear_err{this};
             tiles_total += sym_launch(L, n, pd, pd_stride, po, po_stride, call_tile0 + tiles_total, fo, fc);
+            if (want_err && E) {
+                // Little-endian 32-bit words hold the bits in the packed byte order.
+                const hipMemcpyKind kind = device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+                if (L.packed) {
+                    check(hipMemcpy2DAsync(errs + s0 * errs_stride, errs_stride, d_err.p, ew * 4, erow, n, kind, stream),
+                          "copy error rows");
+                } else if (device_out) {
+                    check(launch_unpack_rows(d_err.as<uint8_t>(), ew * 4, errs + s0 * errs_stride, errs_stride, n,
+                                             (uint32_t)E, 0, stream), "unpack error rows");
+                } else {
+                    d_err_tmp.ensure(n * E);
+                    check(launch_unpack_rows(d_err.as<uint8_t>(), ew * 4, d_err_tmp.as<uint8_t>(), E, n, (uint32_t)E, 0,
+                                             stream), "unpack error rows");
+                    check(hipMemcpy2DAsync(errs + s0 * errs_stride, errs_stride, d_err_tmp.p, E, E, n, kind, stream),
+                          "copy error rows");
+                }
+            }
             if (!L.packed) {
                 // Expand bits to bytes (bool) or characters.
                 uint8_t *ud = device_out ? dets + s0 * dets_stride : nullptr;

@@ -122,14 +122,22 @@ struct mfs_server {
     uint64_t *h_prefix = nullptr;
     size_t h_cap = 0;
     std::map<size_t, uint32_t> grid_for_lds;
+    // Device time of the serving launches (HIP events around each) and the row bytes they wrote.
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double kernel_ms = 0;
+    uint64_t bytes_written = 0;
 
     mfs_server(uint64_t budget_bytes, double step_cost_bytes) : budget(budget_bytes), step_cost(step_cost_bytes) {
         check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+        check(hipEventCreate(&ev0), "event");
+        check(hipEventCreate(&ev1), "event");
         d_counter.ensure(4);
     }
 
     ~mfs_server() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
         for (auto &kv : reqs) release_memory(kv.second);
         if (h_jobs) (void)hipHostFree(h_jobs);
         if (h_prefix) (void)hipHostFree(h_prefix);
@@ -281,6 +289,7 @@ struct mfs_server {
             P.obs.ptr = r.obs + s0 * r.obs_row;
             h_jobs[k] = P;
             h_prefix[k + 1] = h_prefix[k] + take[k].second;
+            bytes_written += std::min<uint64_t>(take[k].second * P.shots_per_tile, P.shots) * (r.det_row + r.obs_row);
             lds = std::max(lds, r.lds);
             r.next_tile += take[k].second;
         }
@@ -290,9 +299,14 @@ struct mfs_server {
         check(hipMemcpyAsync(d_prefix.p, h_prefix, (take.size() + 1) * sizeof(uint64_t), hipMemcpyHostToDevice, stream),
               "upload work list");
         const uint32_t g = grid(lds, tiles);
+        check(hipEventRecord(ev0, stream), "event");
         check(launch_sym_batch(d_jobs.as<SymDevParams>(), d_prefix.as<uint64_t>(), nullptr, (uint32_t)take.size(),
                                d_counter.as<uint32_t>(), g, lds, stream), "serving step");
+        check(hipEventRecord(ev1, stream), "event");
         check(hipStreamSynchronize(stream), "serving step");
+        float ms = 0;
+        check(hipEventElapsedTime(&ms, ev0, ev1), "event");
+        kernel_ms += ms;
         steps++;
         const double t = now_s();
         std::vector<uint64_t> still;
@@ -404,7 +418,7 @@ int mfs_server_release(mfs_server *v, uint64_t id) {
     return guarded([&] { v->release(id); });
 }
 
-int mfs_server_stats(mfs_server *v, uint64_t out[6]) {
+int mfs_server_stats(mfs_server *v, uint64_t out[8]) {
     return guarded([&] {
         out[0] = v->steps;
         out[1] = v->queue.size();
@@ -412,6 +426,8 @@ int mfs_server_stats(mfs_server *v, uint64_t out[6]) {
         out[3] = v->in_use;
         out[4] = v->peak_in_use;
         out[5] = v->budget;
+        out[6] = (uint64_t)(v->kernel_ms * 1000.0);
+        out[7] = v->bytes_written;
     });
 }
 

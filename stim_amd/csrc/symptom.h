@@ -130,6 +130,23 @@ struct SymDevParams {
     uint32_t obs_row_bytes;        // image pitch of separate observable rows (0 when unused)
     uint32_t obs_mode;             // 0: observables dropped, 1: separate image, 2: dets image at obs_col0
     uint32_t obs_col0;
+    // Fired-mechanism rows of a detector-error-model launch (return_errors), or null: shot s of the
+    // launch sets bit e of row s (err_stride_words 32-bit words apart, zeroed beforehand) for every
+    // mechanism e that fires. site_mech maps a sampled site to its mechanism's index in the model.
+    uint32_t *err_words;
+    uint64_t err_stride_words;
+    const uint32_t *site_mech;
+    // Outcome records of the specialised scan kernel (null when it does not apply): one 8-byte record
+    // per (site, outcome code) holding the outcome's whole symptom, the XOR of its basis columns with
+    // entries that cancel removed. Member k of group g finds the record of (group site s, code) at
+    // orec_base[4 g + k] + s * nout + code - 1 (32-bit wrapping arithmetic).
+    const uint2 *orecs;
+    const uint32_t *orec_base;
+    const uint32_t *orec_ovf;      // overflow lists: count, then entries (detector m, observable D + k)
 };
+// Outcome record (8 bytes): x = e0 | e1 << 14 | n << 28 (n = 0..4 detector entries) | obs0 << 31,
+// y = e2 | e3 << 14 | obs1 << 28; unused entries are 0. A symptom with more than four detectors has
+// n = 0, y = SYM_OREC_OVERFLOW | offset of its list in orec_ovf.
+constexpr uint32_t SYM_OREC_OVERFLOW = 1u << 31;
 
 }  // namespace sa

@@ -414,13 +414,13 @@ __device__ __forceinline__ void sink_record32(const SymDevParams &P, const Sink 
 // sites; a single 16-byte load fetches all of them when rstride is 4 (classes with more than 4 basis
 // columns load the records they use one by one).
 // thin: the group is thinned (the site's threshold is slot 1; the hit is dropped when r exceeds it).
-__device__ __forceinline__ void sink_slot32(const SymDevParams &P, const Sink &k, uint32_t slot, uint32_t rstride,
+__device__ __forceinline__ bool sink_slot32(const SymDevParams &P, const Sink &k, uint32_t slot, uint32_t rstride,
                                             uint32_t code, uint32_t shot, bool thin, uint32_t r) {
     const uint32_t dbase = shot * k.det_bits + k.det_col0;
     const uint32_t obase = k.obs_base + shot * k.obs_bits + k.obs_col0 - k.D;
     if (rstride == 4) {
         const uint4 v = *reinterpret_cast<const uint4 *>(P.recs32 + slot);
-        if (thin && r > v.y) return;
+        if (thin && r > v.y) return false;
         if (code & 1) sink_record32(P, k, v.x, dbase, obase);
         if (code & 2) sink_record32(P, k, v.y, dbase, obase);
         if (code & 4) sink_record32(P, k, v.z, dbase, obase);
@@ -432,6 +432,12 @@ __device__ __forceinline__ void sink_slot32(const SymDevParams &P, const Sink &k
             sink_record32(P, k, P.recs32[slot + j], dbase, obase);
         }
     }
+    return true;
+}
+
+// Marks mechanism `mech` as fired in row `row` of the launch's error rows.
+__device__ __forceinline__ void record_error(const SymDevParams &P, uint64_t row, uint32_t mech) {
+    atomicOr(&P.err_words[row * P.err_stride_words + (mech >> 5)], 1u << (mech & 31));
 }
 
 // One tile of the scan sampler: tile `btile` of the launch described by P, in the workgroup's LDS.
@@ -516,12 +522,18 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
                         }
                         if (G.flags & SYM_DEM_SLOTS) code = SYM_DEM_SLOT_CODE;
                         const bool thin = (G.flags & SYM_THIN) != 0;
+                        bool fired;
                         if (P.recs32) {
-                            sink_slot32(P, sink, G.rec0 + srel * G.rstride, G.rstride, code, shot, thin, oc[k]);
-                        } else if (!thin || oc[k] <= P.thr[G.site0 + srel]) {
-                            const SymClass cl = P.classes[G.c0];
-                            sink_hit(P, sink, cl.col0 + srel * cl.basis, code, shot, cl.basis);
+                            fired = sink_slot32(P, sink, G.rec0 + srel * G.rstride, G.rstride, code, shot, thin, oc[k]);
+                        } else {
+                            fired = !thin || oc[k] <= P.thr[G.site0 + srel];
+                            if (fired) {
+                                const SymClass cl = P.classes[G.c0];
+                                sink_hit(P, sink, cl.col0 + srel * cl.basis, code, shot, cl.basis);
+                            }
                         }
+                        if (P.err_words && fired && s0 + shot < P.shots)
+                            record_error(P, s0 + shot, P.site_mech[G.site0 + srel]);
                     }
                     pos += 1;
                 }
@@ -541,11 +553,152 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
                    P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
 }
 
+// Specialised scan tile for models whose groups are plain (compact records, no thinning) and whose
+// members draw outcomes uniformly over codes 1..nout: the QEC memory circuits. It draws exactly the
+// hits scan_tile draws (same counters, gaps, positions and outcome codes), so its rows are
+// bit-identical; it differs in how a hit is applied. Each (site, outcome) has one 8-byte outcome
+// record holding the outcome's whole symptom (entries that cancel between the outcome's basis columns
+// already removed), so a hit reads one record and flips at most four detector bits plus observable
+// flags, instead of testing four basis records of two entries each. Each round first forms every
+// hit's record index and shot, then issues all record loads together (a lane past the end of the
+// group reads the group's first record and applies nothing), then applies the hits.
+template <bool OBS>
+__device__ __forceinline__ void scan_tile_fast(const SymDevParams &P, uint32_t btile, uint32_t *lds) {
+    const uint32_t S = P.shots_per_tile;
+    const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
+    const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
+    uint32_t *const scratch = lds + det_words + obs_words;  // 8 words
+    const uint32_t tid = threadIdx.x;
+    {
+        uint4 *const v = reinterpret_cast<uint4 *>(lds);
+        for (uint32_t i = tid; i < (det_words + obs_words) / 4; i += NT) v[i] = make_uint4(0, 0, 0, 0);
+    }
+    const Sink sink = make_sink(P, lds, det_words);
+    __syncthreads();
+
+    const uint64_t s0 = (uint64_t)btile * S;
+    const uint64_t tile64 = P.tile0 + btile;
+    const uint32_t tile = (uint32_t)tile64, lane_word = sym_lane_word(tid, tile64);
+    const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
+    const uint32_t lg = P.tile_log2;
+    for (uint32_t gi = 0; gi < P.ngroups; gi++) {
+        const SymGroup G = P.groups[gi];
+        if (!(G.p > 0.0f)) continue;
+        const uint32_t N = G.nsites << lg;
+        const float mu = (float)N * G.p;
+        uint32_t m = (uint32_t)((mu + 3.0f * sqrtf(mu) + 4.0f) / NT) + 1;
+        if (m > SCAN_M_MAX) m = SCAN_M_MAX;
+        const uint32_t b0 = P.orec_base[4 * gi], b1 = P.orec_base[4 * gi + 1], b2 = P.orec_base[4 * gi + 2],
+                       b3 = P.orec_base[4 * gi + 3];
+        uint32_t base = 0;
+        for (uint32_t round = 0; base < N; round++) {
+            uint32_t g[SCAN_M_MAX], oc[SCAN_M_MAX];
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX / 2; k++) {
+                if (2 * k < m) {
+                    const uint4 r = philox10(make_uint4(round * (SCAN_M_MAX / 2) + k, lane_word, tile, TAG_SCAN | gi), k0, k1);
+                    g[2 * k] = gap32(r.x, G.inv_log);
+                    oc[2 * k] = r.y;
+                    g[2 * k + 1] = gap32(r.z, G.inv_log);
+                    oc[2 * k + 1] = r.w;
+                } else {
+                    g[2 * k] = g[2 * k + 1] = 0;
+                    oc[2 * k] = oc[2 * k + 1] = 0;
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
+                if (k < m) sum = sat_add(sum, g[k] + 1);
+            }
+            const uint2 sc = block_scan(sum, scratch);
+            uint32_t pos = sat_add(base, sc.x);
+            pos = pos < N ? pos : N;
+            // Pass 1: record index and shot of every hit (shot S marks no hit).
+            uint32_t idx[SCAN_M_MAX], shot[SCAN_M_MAX];
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
+                idx[k] = b0;
+                shot[k] = S;
+                if (k < m) {
+                    pos += g[k] < N ? g[k] : N;
+                    pos = pos < N ? pos : N;
+                    const uint32_t srel = pos >> lg;
+                    const bool m1 = srel >= G.bound[1], m2 = srel >= G.bound[2], m3 = srel >= G.bound[3];
+                    const uint32_t nf = m3 ? G.nout[3] : m2 ? G.nout[2] : m1 ? G.nout[1] : G.nout[0];
+                    const uint32_t bb = m3 ? b3 : m2 ? b2 : m1 ? b1 : b0;
+                    const uint32_t nout = nf & 0xFFFFu;
+                    if (pos < N) {
+                        idx[k] = bb + srel * nout + __umulhi(oc[k], nout);
+                        shot[k] = pos & (S - 1);
+                    }
+                    pos += 1;
+                }
+            }
+            // Pass 2: every record load of the round in flight together.
+            uint2 rec[SCAN_M_MAX];
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
+                if (k < m) rec[k] = P.orecs[idx[k]];
+            }
+            // Pass 3: flip the outcome's detector bits and observable flags.
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
+                if (k < m && shot[k] < S) {
+                    // Shot and row-bit counts fit 24 bits (S <= 8192, rows below 64 KB).
+                    const uint32_t dbase = __umul24(shot[k], sink.det_bits) + sink.det_col0;
+                    const uint2 r = rec[k];
+                    const uint32_t n = (r.x >> 28) & 7u;
+                    if (n >= 1) xor_bit(sink, dbase + (r.x & 0x3FFFu));
+                    if (n >= 2) xor_bit(sink, dbase + ((r.x >> 14) & 0x3FFFu));
+                    if (n >= 3) xor_bit(sink, dbase + (r.y & 0x3FFFu));
+                    if (n >= 4) xor_bit(sink, dbase + ((r.y >> 14) & 0x3FFFu));
+                    if (OBS && ((r.x >> 31) | ((r.y >> 28) & 1u))) {
+                        const uint32_t obase = sink.obs_base + __umul24(shot[k], sink.obs_bits) + sink.obs_col0;
+                        if (r.x >> 31) xor_bit(sink, obase);
+                        if ((r.y >> 28) & 1u) xor_bit(sink, obase + 1);
+                    }
+                    if (r.y & SYM_OREC_OVERFLOW) {
+                        const uint32_t obase = sink.obs_base + __umul24(shot[k], sink.obs_bits) + sink.obs_col0;
+                        // More than four detectors: the list (detector m, observable D + k).
+                        const uint32_t *list = P.orec_ovf + (r.y & ~SYM_OREC_OVERFLOW);
+                        const uint32_t cnt = list[0];
+                        for (uint32_t e = 0; e < cnt; e++) {
+                            const uint32_t v = list[1 + e];
+                            if (v < sink.D) xor_bit(sink, dbase + v);
+                            else if (OBS) xor_bit(sink, obase + (v - sink.D));
+                        }
+                    }
+                }
+            }
+            base = sat_add(base, sc.y);
+            base = base < N ? base : N;
+        }
+    }
+    __syncthreads();
+
+    const uint64_t valid = P.shots > s0 ? (P.shots - s0 < S ? P.shots - s0 : S) : 0;
+    if (P.dets.enabled)
+        write_rows(reinterpret_cast<const uint8_t *>(lds), P.det_row_bytes, (uint32_t)valid,
+                   P.dets.ptr + s0 * P.dets.stride, P.dets.stride);
+    if (P.obs.enabled && P.obs_row_bytes)
+        write_rows(reinterpret_cast<const uint8_t *>(lds + det_words), P.obs_row_bytes, (uint32_t)valid,
+                   P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
+}
+
 // Both scan kernels ask for 6 waves per SIMD (at most 80 VGPRs, no spills): the sampler is bound by
 // instruction issue and latency, so resident waves matter more than registers.
 __global__ __launch_bounds__(NT, 6) void sym_scan_kernel(SymDevParams P) {
     extern __shared__ __align__(16) uint32_t lds[];
     scan_tile(P, blockIdx.x, lds);
+}
+
+// The specialised form holds a round's record loads in registers (16 more VGPRs); at d=25 its
+// 31 KB tile image allows 5 workgroups per CU, so it asks for 5 waves per SIMD.
+template <bool OBS>
+__global__ __launch_bounds__(NT, 5) void sym_scan_fast_kernel(SymDevParams P) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    scan_tile_fast<OBS>(P, blockIdx.x, lds);
 }
 
 // Batched form: many launches (jobs, possibly of different circuits) in one kernel; job j owns global
@@ -745,8 +898,22 @@ hipError_t launch_sym_sample(const SymDevParams &P, uint32_t tiles, size_t lds, 
     return hipGetLastError();
 }
 
-hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream) {
+hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hipStream_t stream, bool fast) {
     if (!tiles) return hipSuccess;
+    if (fast) {
+        if (P.obs_mode != 0) {
+            static size_t reserved_obs = 0;
+            const hipError_t e = reserve_lds((const void *)sym_scan_fast_kernel<true>, lds, reserved_obs);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(sym_scan_fast_kernel<true>, dim3(tiles), dim3(NT), lds, stream, P);
+        } else {
+            static size_t reserved_plain = 0;
+            const hipError_t e = reserve_lds((const void *)sym_scan_fast_kernel<false>, lds, reserved_plain);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(sym_scan_fast_kernel<false>, dim3(tiles), dim3(NT), lds, stream, P);
+        }
+        return hipGetLastError();
+    }
     static size_t reserved = 0;
     const hipError_t e = reserve_lds((const void *)sym_scan_kernel, lds, reserved);
     if (e != hipSuccess) return e;

@@ -1,16 +1,18 @@
-// Stabilizer tableau for reference samples (see tableau.h).
+// Stabilizer simulation for reference samples (see tableau.h).
 //
-// Form: Aaronson & Gottesman, "Improved simulation of stabilizer circuits", PRA 70, 052328 (2004):
-// rows 0..n-1 are destabilizers, rows n..2n-1 stabilizers and row 2n is scratch; each row is an
-// n-qubit Pauli string (x bits, z bits, 64 qubits per word) with a sign bit. A Clifford gate acts on
-// the columns of its qubits through its conjugation table (U P U^dagger for every local Pauli P,
-// with sign), built from the images of X and Z that define each gate (doc/gates.md, "Stabilizer
-// Generators"). Measurement follows the paper's procedure; the random branch always takes result 0
-// (the reference's sign bias +1). The random results an instruction collapses are taken in ascending
-// qubit order and read afterwards in target order, as the reference does (tableau_simulator.inl:330-504,
-// 1267-1290): the order decides the sample when random results are correlated. Pauli-product
-// measurements and phases rotate every factor to Z, fold the product onto its lowest qubit with CX,
-// act there and undo the rotation (src/stim/circuit/gate_decomposition.cc:88-170, 163-217, 245-274).
+// Form: the inverse tableau of the circuit's Clifford U (the rows U^dagger X_q U and U^dagger Z_q U,
+// each an n-qubit Pauli string with a sign). A gate rewrites only its own qubits' rows (products of
+// old rows), a deterministic Z measurement reads one row's sign, and a random one is resolved by a
+// few input-side column operations (details at class Tableau). The reference keeps the same kind
+// of state (TableauSimulator::inv_state, tableau_simulator.h:41). Gates act through conjugation
+// tables (U P U^dagger for every local Pauli P, with sign), built from the images of X and Z that
+// define each gate (doc/gates.md, "Stabilizer Generators"). A random result is always 0 (the
+// reference's sign bias +1). The random results an instruction collapses are taken in ascending
+// qubit order and read afterwards in target order, as the reference does
+// (tableau_simulator.inl:330-504, 1267-1290): the order decides the sample when random results are
+// correlated. Pauli-product measurements and phases rotate every factor to Z, fold the product onto
+// its lowest qubit with CX, act there and undo the rotation (src/stim/circuit/gate_decomposition.cc:
+// 88-170, 163-217, 245-274).
 #include "tableau.h"
 
 #include <algorithm>
@@ -184,17 +186,161 @@ const Tables &tables() {
     return t;
 }
 
+// Inverse of a single-qubit gate's conjugation map: U P U^dagger = (-1)^neg Q gives
+// U^dagger Q U = (-1)^neg P.
+Map1 inverse1(const Map1 &m) {
+    Map1 r{};
+    for (int c = 1; c < 4; c++) {
+        r.out[m.out[c]] = (uint8_t)c;
+        r.neg[m.out[c]] = m.neg[c];
+    }
+    return r;
+}
+
+Map2 inverse2(const Map2 &m) {
+    Map2 r{};
+    for (int c = 1; c < 16; c++) {
+        const int d = m.oa[c] | (m.ob[c] << 2);
+        r.oa[d] = (uint8_t)(c & 3);
+        r.ob[d] = (uint8_t)(c >> 2);
+        r.neg[d] = m.neg[c];
+    }
+    return r;
+}
+
+// The state U|0...0> held as the inverse tableau of U: row X'_q = U^dagger X_q U and row Z'_q =
+// U^dagger Z_q U, each an n-qubit Pauli string (x bits, z bits, 64 qubits per word) with a sign.
+// Applying a gate G (U -> G U) rewrites only its qubits' rows: X'_q becomes the image of
+// G^dagger X_q G, a product of the old rows of G's qubits. Z_q is deterministic in U|0> exactly when
+// Z'_q has no X part (it is then a signed product of Z's, which fix |0...0>), and its value is the
+// sign. A random Z_q measurement picks a pivot input qubit p where Z'_q has X, clears Z'_q's other X
+// components with CX gates from p on the input side (U -> U C with C fixing |0...0>, so the state is
+// unchanged), turns p's X (or Y) into Z with H (or H_YZ) on the input side, which moves the state onto
+// the measured eigenspace, and picks result 0 with an X on input p when the sign says 1. Input-side
+// gates conjugate every row on the input qubits' columns.
 class Tableau {
 public:
     explicit Tableau(uint32_t n)
-        : n_(n), nw_((n + 63) / 64), x_((2 * (size_t)n + 1) * nw_, 0), z_(x_.size(), 0), r_(2 * (size_t)n + 1, 0) {
+        : n_(n), nw_((n + 63) / 64), x_(2 * (size_t)n * nw_, 0), z_(x_.size(), 0), r_(2 * (size_t)n, 0),
+          sx_(4 * nw_), sz_(4 * nw_) {
         for (uint32_t q = 0; q < n; q++) {
-            set(x_, q, q, true);
-            set(z_, (size_t)n + q, q, true);
+            set(x_, 2 * (size_t)q, q, true);
+            set(z_, 2 * (size_t)q + 1, q, true);
         }
     }
 
+    // Output-side gate: U -> G U. m is G's conjugation map (G P G^dagger).
     void apply1(const Map1 &m, uint32_t q) {
+        const Map1 inv = inverse1(m);
+        const size_t rx = 2 * (size_t)q, rz = rx + 1;
+        const uint8_t sx = image1(inv.out[1], rx, rz, &sx_[0], &sz_[0]) ^ inv.neg[1];
+        const uint8_t sz = image1(inv.out[2], rx, rz, &sx_[nw_], &sz_[nw_]) ^ inv.neg[2];
+        store(rx, &sx_[0], &sz_[0], sx);
+        store(rz, &sx_[nw_], &sz_[nw_], sz);
+    }
+
+    void apply2(const Map2 &m, uint32_t a, uint32_t b) {
+        const Map2 inv = inverse2(m);
+        const size_t rows[4] = {2 * (size_t)a, 2 * (size_t)a + 1, 2 * (size_t)b, 2 * (size_t)b + 1};
+        const int codes[4] = {1, 2, 1 << 2, 2 << 2};   // X_a, Z_a, X_b, Z_b
+        uint8_t sg[4];
+        bool same[4];
+        for (int k = 0; k < 4; k++) {
+            const int c = codes[k];
+            // A generator the gate leaves alone keeps its row (CX keeps Z_a and X_b).
+            same[k] = (inv.oa[c] | (inv.ob[c] << 2)) == c && !inv.neg[c];
+            if (!same[k]) sg[k] = image2(inv.oa[c], inv.ob[c], rows, &sx_[k * nw_], &sz_[k * nw_]) ^ inv.neg[c];
+        }
+        for (int k = 0; k < 4; k++)
+            if (!same[k]) store(rows[k], &sx_[k * nw_], &sz_[k * nw_], sg[k]);
+    }
+
+    // Measures Z on qubit a; a random result is taken as 0 and the state collapses onto it.
+    bool measure_z(uint32_t a) {
+        const size_t row = 2 * (size_t)a + 1;
+        const uint64_t *zx = &x_[row * nw_];
+        size_t w = 0;
+        while (w < nw_ && !zx[w]) w++;
+        if (w == nw_) return r_[row];
+        const uint32_t p = (uint32_t)(64 * w + __builtin_ctzll(zx[w]));
+        const Tables &T = tables();
+        std::copy_n(zx, nw_, sx_.data());
+        sx_[p >> 6] &= ~(1ull << (p & 63));
+        cx_fan_out(p, sx_.data());
+        input1(T.one[(size_t)(get(z_, row, p) ? Gate::H_YZ : Gate::H)], p);
+        if (r_[row]) input1(T.one[(size_t)Gate::X], p);
+        return false;
+    }
+
+private:
+    uint32_t n_;
+    size_t nw_;
+    std::vector<uint64_t> x_, z_;
+    std::vector<uint8_t> r_;
+    std::vector<uint64_t> sx_, sz_;   // new rows of the gate being applied
+
+    bool get(const std::vector<uint64_t> &v, size_t row, uint32_t q) const {
+        return (v[row * nw_ + (q >> 6)] >> (q & 63)) & 1;
+    }
+    void set(std::vector<uint64_t> &v, size_t row, uint32_t q, bool b) {
+        uint64_t &w = v[row * nw_ + (q >> 6)];
+        const uint64_t bit = 1ull << (q & 63);
+        w = b ? (w | bit) : (w & ~bit);
+    }
+    void store(size_t row, const uint64_t *x, const uint64_t *z, uint8_t sign) {
+        std::copy_n(x, nw_, &x_[row * nw_]);
+        std::copy_n(z, nw_, &z_[row * nw_]);
+        r_[row] = sign;
+    }
+
+    // acc (x, z, exponent e of i^e) <- acc * row; returns the new exponent.
+    int mul_row(uint64_t *ax, uint64_t *az, int e, size_t row) const {
+        const uint64_t *rx = &x_[row * nw_], *rz = &z_[row * nw_];
+        e += 2 * r_[row];
+        for (size_t w = 0; w < nw_; w++) {
+            const uint64_t x1 = ax[w], z1 = az[w], x2 = rx[w], z2 = rz[w];
+            const uint64_t X1 = x1 & ~z1, Y1 = x1 & z1, Z1 = ~x1 & z1;
+            const uint64_t X2 = x2 & ~z2, Y2 = x2 & z2, Z2 = ~x2 & z2;
+            e += __builtin_popcountll((X1 & Y2) | (Y1 & Z2) | (Z1 & X2));
+            e -= __builtin_popcountll((X1 & Z2) | (Y1 & X2) | (Z1 & Y2));
+            ax[w] = x1 ^ x2;
+            az[w] = z1 ^ z2;
+        }
+        return e & 3;
+    }
+
+    // Sign of the real product i^e (e even for a Hermitian image).
+    static uint8_t sign_of(int e) {
+        if (e & 1) throw std::logic_error("non-Hermitian Pauli product in the tableau");
+        return (uint8_t)((e & 3) == 2);
+    }
+
+    // Image under the inverse tableau of a single-qubit Pauli code c on the qubit whose rows are
+    // (rx, rz): X -> row X', Z -> row Z', Y = i X Z -> i X' Z'.
+    uint8_t image1(int c, size_t rx, size_t rz, uint64_t *ox, uint64_t *oz) const {
+        std::fill_n(ox, nw_, 0);
+        std::fill_n(oz, nw_, 0);
+        int e = c == 3 ? 1 : 0;
+        if (c & 1) e = mul_row(ox, oz, e, rx);
+        if (c & 2) e = mul_row(ox, oz, e, rz);
+        return sign_of(e);
+    }
+
+    // Image of the two-qubit Pauli (code ca on a, cb on b) = i^{#Y} X_a^. Z_a^. X_b^. Z_b^.
+    uint8_t image2(int ca, int cb, const size_t rows[4], uint64_t *ox, uint64_t *oz) const {
+        std::fill_n(ox, nw_, 0);
+        std::fill_n(oz, nw_, 0);
+        int e = (ca == 3) + (cb == 3);
+        if (ca & 1) e = mul_row(ox, oz, e, rows[0]);
+        if (ca & 2) e = mul_row(ox, oz, e, rows[1]);
+        if (cb & 1) e = mul_row(ox, oz, e, rows[2]);
+        if (cb & 2) e = mul_row(ox, oz, e, rows[3]);
+        return sign_of(e);
+    }
+
+    // Input-side gate (U -> U C): every row is conjugated by C on the input columns. m is C's map;
+    // C is self-inverse for every use here (H, H_YZ, X).
+    void input1(const Map1 &m, uint32_t q) {
         const size_t w = q >> 6;
         const uint64_t bit = 1ull << (q & 63);
         for (size_t i = 0; i < 2 * (size_t)n_; i++) {
@@ -208,88 +354,31 @@ public:
         }
     }
 
-    void apply2(const Map2 &m, uint32_t a, uint32_t b) {
-        const size_t wa = a >> 6, wb = b >> 6;
-        const uint64_t ba = 1ull << (a & 63), bb = 1ull << (b & 63);
+    // Input-side CX from control p to every target in `mask` (p not in it), in one pass over the rows.
+    // Per row: x_k ^= x_p for k in mask, z_p ^= parity(z & mask), and the sign flips as the CX gates
+    // applied one after another would flip it (each CX(p, k) flips it when x_p z_k (x_k + z_p + 1) is
+    // odd, with z_p already updated by the earlier targets). With a = z & mask, b = x & mask,
+    // m = |a|, that is x_p (m + |a & b| + z_p m + m (m - 1) / 2) mod 2.
+    void cx_fan_out(uint32_t p, const uint64_t *mask) {
+        const size_t wp = p >> 6;
+        const uint64_t bp = 1ull << (p & 63);
+        std::vector<uint32_t> words;   // the mask's nonzero words (usually a few)
+        for (size_t w = 0; w < nw_; w++)
+            if (mask[w]) words.push_back((uint32_t)w);
+        if (words.empty()) return;
         for (size_t i = 0; i < 2 * (size_t)n_; i++) {
             uint64_t *xr = &x_[i * nw_], *zr = &z_[i * nw_];
-            const int ca = ((xr[wa] & ba) ? 1 : 0) | ((zr[wa] & ba) ? 2 : 0);
-            const int cb = ((xr[wb] & bb) ? 1 : 0) | ((zr[wb] & bb) ? 2 : 0);
-            const int c = ca | cb << 2;
-            if (!c) continue;
-            const int oa = m.oa[c], ob = m.ob[c];
-            xr[wa] = (oa & 1) ? (xr[wa] | ba) : (xr[wa] & ~ba);
-            zr[wa] = (oa & 2) ? (zr[wa] | ba) : (zr[wa] & ~ba);
-            xr[wb] = (ob & 1) ? (xr[wb] | bb) : (xr[wb] & ~bb);
-            zr[wb] = (ob & 2) ? (zr[wb] | bb) : (zr[wb] & ~bb);
-            r_[i] ^= m.neg[c];
-        }
-    }
-
-    // Measures Z on qubit a; a random result is taken as 0 and the state collapses onto it.
-    bool measure_z(uint32_t a) {
-        const size_t n = n_;
-        size_t p = SIZE_MAX;
-        for (size_t i = n; i < 2 * n; i++) {
-            if (get(x_, i, a)) {
-                p = i;
-                break;
+            const bool xp = (xr[wp] & bp) != 0, zp = (zr[wp] & bp) != 0;
+            uint64_t m = 0, ab = 0;
+            for (uint32_t w : words) {
+                const uint64_t a = zr[w] & mask[w];
+                m += __builtin_popcountll(a);
+                ab += __builtin_popcountll(a & xr[w]);
+                if (xp) xr[w] ^= mask[w];
             }
+            if (m & 1) zr[wp] ^= bp;
+            if (xp) r_[i] ^= (uint8_t)((m + ab + (zp ? m : 0) + m * (m - 1) / 2) & 1);
         }
-        if (p != SIZE_MAX) {
-            for (size_t i = 0; i < 2 * n; i++) {
-                if (i != p && get(x_, i, a)) rowsum(i, p);
-            }
-            std::copy_n(&x_[p * nw_], nw_, &x_[(p - n) * nw_]);
-            std::copy_n(&z_[p * nw_], nw_, &z_[(p - n) * nw_]);
-            r_[p - n] = r_[p];
-            std::fill_n(&x_[p * nw_], nw_, 0);
-            std::fill_n(&z_[p * nw_], nw_, 0);
-            set(z_, p, a, true);
-            r_[p] = 0;
-            return false;
-        }
-        const size_t s = 2 * n;
-        std::fill_n(&x_[s * nw_], nw_, 0);
-        std::fill_n(&z_[s * nw_], nw_, 0);
-        r_[s] = 0;
-        for (size_t i = 0; i < n; i++) {
-            if (get(x_, i, a)) rowsum(s, i + n);
-        }
-        return r_[s];
-    }
-
-private:
-    uint32_t n_;
-    size_t nw_;
-    std::vector<uint64_t> x_, z_;
-    std::vector<uint8_t> r_;
-
-    bool get(const std::vector<uint64_t> &v, size_t row, uint32_t q) const {
-        return (v[row * nw_ + (q >> 6)] >> (q & 63)) & 1;
-    }
-    void set(std::vector<uint64_t> &v, size_t row, uint32_t q, bool b) {
-        uint64_t &w = v[row * nw_ + (q >> 6)];
-        const uint64_t bit = 1ull << (q & 63);
-        w = b ? (w | bit) : (w & ~bit);
-    }
-
-    // Row h becomes (row i) * (row h), with the sign of the product (real for commuting rows; the
-    // only non-commuting use, on the destabilizer about to be overwritten, ignores its sign).
-    void rowsum(size_t h, size_t i) {
-        uint64_t *xh = &x_[h * nw_], *zh = &z_[h * nw_];
-        const uint64_t *xi = &x_[i * nw_], *zi = &z_[i * nw_];
-        int e = 2 * (r_[h] + r_[i]);
-        for (size_t w = 0; w < nw_; w++) {
-            const uint64_t x1 = xi[w], z1 = zi[w], x2 = xh[w], z2 = zh[w];
-            const uint64_t X1 = x1 & ~z1, Y1 = x1 & z1, Z1 = ~x1 & z1;
-            const uint64_t X2 = x2 & ~z2, Y2 = x2 & z2, Z2 = ~x2 & z2;
-            e += __builtin_popcountll((X1 & Y2) | (Y1 & Z2) | (Z1 & X2));
-            e -= __builtin_popcountll((X1 & Z2) | (Y1 & X2) | (Z1 & Y2));
-            xh[w] = x1 ^ x2;
-            zh[w] = z1 ^ z2;
-        }
-        r_[h] = (((e % 4) + 4) % 4) == 2;
     }
 };
 
@@ -571,12 +660,12 @@ struct Runner {
 std::vector<uint8_t> reference_sample(const std::vector<const Instr *> &flat) {
     const Stats st = compute_stats(flat);
     if (st.num_measurements == 0) return {};
-    // The tableau holds 2n + 1 rows of n X bits and n Z bits (about n^2 / 2 bytes), as the reference's
+    // The tableau holds 2n rows of n X bits and n Z bits (about n^2 / 2 bytes), as the reference's
     // TableauSimulator does; circuits whose tableau would exceed the limit are refused up front rather
     // than failing inside the allocator.
     constexpr double MAX_TABLEAU_BYTES = 16.0 * (1ull << 30);
     const uint64_t n = std::max<uint64_t>(st.num_qubits, 1);
-    const double bytes = 2.0 * (2.0 * (double)n + 1.0) * (double)((n + 63) / 64) * 8.0;
+    const double bytes = 2.0 * (2.0 * (double)n) * (double)((n + 63) / 64) * 8.0;
     if (n >= (1ull << 31) || bytes > MAX_TABLEAU_BYTES) {
         throw std::runtime_error("the reference sample of a circuit on " + std::to_string(n) +
                                  " qubits needs a stabilizer tableau of " + std::to_string((uint64_t)(bytes / (1 << 20))) +

@@ -13,6 +13,7 @@ import os
 import numpy as np
 
 from . import _native as N
+from . import formats
 from .sampler import CompiledDetectorSampler as _CDS
 
 
@@ -93,10 +94,10 @@ class CompiledDemSampler:
 
     def sample(self, shots: int, *, bit_packed: bool = False, return_errors: bool = False,
                recorded_errors_to_replay=None):
-        """Returns (detection events, observable flips, None): bool (shots, n) arrays, or uint8
-        (shots, ceil(n/8)) bit-packed little-endian rows."""
-        if return_errors:
-            raise ValueError('return_errors=True is not supported by this sampler')
+        """Returns (detection events, observable flips, errors): bool (shots, n) arrays, or uint8
+        (shots, ceil(n/8)) bit-packed little-endian rows. errors is None unless return_errors=True; it
+        then holds which mechanisms fired in each shot (the replayed ones when replaying), as
+        src/stim/simulators/dem_sampler.pybind.cc:45-79 returns them."""
         shots = int(shots)
         if shots < 0:
             raise ValueError('shots must be non-negative')
@@ -112,7 +113,14 @@ class CompiledDemSampler:
         o_stride = max(obs.shape[1], 1)
         d_ptr = dets.ctypes.data if dets.size else None
         o_ptr = obs.ctypes.data if obs.size else None
-        if shots and (D or O):
+        E = self.num_errors
+        errs = None
+        if return_errors and recorded_errors_to_replay is None:
+            errs = np.zeros((shots, (E + 7) // 8), dtype=np.uint8) if bit_packed else np.zeros((shots, E), np.bool_)
+            if shots:
+                N.check(N.lib().mfs_dem_sample_errors(self._h, shots, flags, d_ptr, d_stride, o_ptr, o_stride,
+                                                      errs.ctypes.data if errs.size else None, max(errs.shape[1], 1)))
+        elif shots and (D or O):
             if recorded_errors_to_replay is None:
                 N.check(N.lib().mfs_sample(self._h, shots, flags, d_ptr, d_stride, o_ptr, o_stride))
             else:
@@ -122,49 +130,42 @@ class CompiledDemSampler:
                                                        d_ptr, d_stride, o_ptr, o_stride))
         elif recorded_errors_to_replay is not None:
             self._replay_faults(recorded_errors_to_replay, shots)   # shape checks only: no outputs to fill
-        return dets, obs, None
+        if return_errors and recorded_errors_to_replay is not None:
+            errs = self._errors_as(recorded_errors_to_replay, bit_packed)
+        return dets, obs, errs
+
+    def _errors_as(self, errs, bit_packed):
+        """A (validated) recorded-error array in the requested layout."""
+        a = np.asarray(errs)
+        if a.dtype == np.bool_:
+            bits = a
+        else:
+            bits = np.unpackbits(a, axis=1, bitorder='little')[:, :self.num_errors].astype(np.bool_)
+        return np.packbits(bits, axis=1, bitorder='little') if bit_packed else bits.copy()
 
     def sample_write(self, shots: int, *, det_out_file, det_out_format: str = '01', obs_out_file,
                      obs_out_format: str = '01', err_out_file=None, err_out_format: str = '01',
                      replay_err_in_file=None, replay_err_in_format: str = '01'):
-        """Samples and writes detection events and observable flips in the reference's result formats."""
-        if err_out_file is not None:
-            raise ValueError('err_out_file is not supported by this sampler')
+        """Samples and writes detection events, observable flips and (err_out_file) the fired mechanisms
+        in the reference's result formats, or replays errors read from replay_err_in_file
+        (src/stim/simulators/dem_sampler.inl:75-128: prefixes D, L and M in the dets format)."""
         shots = int(shots)
-        if replay_err_in_file is None:
+        D, O, E = self.num_detectors, self.num_observables, self.num_errors
+        if replay_err_in_file is None and err_out_file is None:
             N.check(N.lib().mfs_sample_write(self._h, shots, os.fsencode(det_out_file), det_out_format.encode(), 0, 0,
                                              os.fsencode(obs_out_file), obs_out_format.encode()))
             return
-        errs = _read_bits(replay_err_in_file, replay_err_in_format, shots, self.num_errors)
-        dets, obs, _ = self.sample(shots, recorded_errors_to_replay=errs)
-        _write_bits(det_out_file, det_out_format, dets)
-        _write_bits(obs_out_file, obs_out_format, obs)
-
-
-def _read_bits(path, fmt, shots, n):
-    """Reads shots x n bits in the '01' or 'b8' result format (doc/result_formats.md)."""
-    data = open(path, 'rb').read()
-    if fmt == '01':
-        lines = [l for l in data.decode().split('\n') if l != '']
-        if len(lines) != shots or any(len(l) != n for l in lines):
-            raise ValueError('replay data does not have the expected shape')
-        return np.array([[c == '1' for c in l] for l in lines], dtype=np.bool_).reshape(shots, n)
-    if fmt == 'b8':
-        row = (n + 7) // 8
-        if len(data) != shots * row:
-            raise ValueError('replay data does not have the expected size')
-        return np.frombuffer(data, dtype=np.uint8).reshape(shots, row).copy()
-    raise ValueError(f'replay format {fmt!r} is not supported (use 01 or b8)')
-
-
-def _write_bits(path, fmt, rows):
-    if fmt == '01':
-        with open(path, 'w') as f:
-            for r in rows:
-                f.write(''.join('1' if b else '0' for b in r) + '\n')
-    elif fmt == 'b8':
-        packed = np.packbits(np.asarray(rows, dtype=np.uint8), axis=1, bitorder='little') if len(rows) else b''
-        with open(path, 'wb') as f:
-            f.write(bytes(packed))
-    else:
-        raise ValueError(f'format {fmt!r} is not supported with replayed errors (use 01 or b8)')
+        if replay_err_in_file is not None:
+            errs = formats.read_rows(os.fspath(replay_err_in_file), replay_err_in_format, E, [('M', E)])
+            if len(errs) < shots:
+                raise ValueError('Expected more error data for the requested number of shots.')
+            errs = np.ascontiguousarray(errs[:shots])
+            dets, obs, _ = self.sample(shots, recorded_errors_to_replay=errs)
+        else:
+            dets, obs, errs = self.sample(shots, return_errors=True)
+        if err_out_file is not None:
+            formats.write_rows(os.fspath(err_out_file), err_out_format, errs, [('M', E)])
+        if obs_out_file is not None:
+            formats.write_rows(os.fspath(obs_out_file), obs_out_format, obs, [('L', O)])
+        if det_out_file is not None:
+            formats.write_rows(os.fspath(det_out_file), det_out_format, dets, [('D', D)])

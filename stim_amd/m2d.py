@@ -3,7 +3,7 @@
 Reference: stim.CompiledMeasurementsToDetectionEventsConverter
 (src/stim/simulators/measurements_to_detection_events.pybind.cc:33-166, 210-380) created by
 stim.Circuit.compile_m2d_converter (src/stim/circuit/circuit.pybind.cc:781-785). Conversion runs
-through the C ABI (mfs_m2d_*); sweep data is not supported (a call with any set sweep bit raises).
+through the C ABI (mfs_m2d_*), with sweep data (sweep_bits / sweep_bits_filepath) as in the reference.
 """
 
 import ctypes
@@ -26,9 +26,9 @@ class CompiledMeasurementsToDetectionEventsConverter:
         t = _text_bytes(str(circuit))
         N.check(N.lib().mfs_m2d_compile(t, len(t), 1 if self._skip else 0, ctypes.byref(h)))
         self._h = h
-        out = (ctypes.c_uint64 * 3)()
+        out = (ctypes.c_uint64 * 4)()
         N.check(N.lib().mfs_m2d_stats(self._h, out))
-        self.num_measurements, self.num_detectors, self.num_observables = list(out)
+        self.num_measurements, self.num_detectors, self.num_observables, self.num_sweep_bits = list(out)
 
     def __del__(self):
         h = getattr(self, '_h', None)
@@ -55,6 +55,30 @@ class CompiledMeasurementsToDetectionEventsConverter:
             return np.ascontiguousarray(m) if M else np.zeros((m.shape[0], 1), np.uint8)
         raise ValueError('measurements must have dtype np.bool_ or np.uint8 (bit packed)')
 
+    def _packed_sweep_bits(self, sweep_bits, shots):
+        """Sweep data as bit-packed rows of ceil(K/8) bytes (K = num_sweep_bits), or None when all zero.
+        Accepts bool (shots, K) or bit-packed uint8 (shots, ceil(K/8)) arrays, as the reference does
+        (numpy_array_to_transposed_simd_table, src/stim/py/numpy.pybind.cc)."""
+        sw = np.asarray(sweep_bits)
+        K = self.num_sweep_bits
+        if sw.ndim != 2:
+            raise ValueError('sweep_bits must be a 2d numpy array')
+        if sw.shape[0] != shots:
+            raise ValueError('Need sweep_bits.shape[0] == measurements.shape[0]')
+        if sw.dtype == np.bool_:
+            if sw.shape[1] != K:
+                raise ValueError(f'sweep_bits.shape[1] != num_sweep_bits ({sw.shape[1]} != {K})')
+            packed = np.packbits(sw, axis=1, bitorder='little')
+        elif sw.dtype == np.uint8:
+            if sw.shape[1] != (K + 7) // 8:
+                raise ValueError('bit packed sweep_bits.shape[1] != ceil(num_sweep_bits / 8)')
+            packed = sw
+        else:
+            raise ValueError('sweep_bits must have dtype np.bool_ or np.uint8 (bit packed)')
+        if K == 0:
+            return None
+        return np.ascontiguousarray(packed)
+
     def convert(self, *, measurements, sweep_bits=None, separate_observables=None, append_observables=None,
                 bit_packed: bool = False, bit_pack_result: bool = False):
         """Detection events (and observable flips) of measurement data: bool (shots, n) or bit-packed uint8
@@ -67,12 +91,7 @@ class CompiledMeasurementsToDetectionEventsConverter:
         packed = bool(bit_packed or bit_pack_result)
         meas = self._packed_measurements(measurements)
         shots = meas.shape[0]
-        if sweep_bits is not None:
-            sw = np.asarray(sweep_bits)
-            if sw.ndim != 2 or sw.shape[0] != shots:
-                raise ValueError('Need sweep_bits.shape[0] == measurements.shape[0]')
-            if sw.any():
-                raise ValueError('sweep bits are not supported by this converter')
+        sweep = self._packed_sweep_bits(sweep_bits, shots) if sweep_bits is not None else None
         D, O = self.num_detectors, self.num_observables
         n_out = D + (O if append else 0)
         dtype = np.uint8 if packed else np.bool_
@@ -81,8 +100,9 @@ class CompiledMeasurementsToDetectionEventsConverter:
         flags = (N.MFS_BIT_PACKED if packed else 0) | (N.MFS_APPEND_OBSERVABLES if append else 0) | \
                 (N.MFS_SEPARATE_OBSERVABLES if separate else 0)
         if shots:
-            N.check(N.lib().mfs_m2d_convert(
-                self._h, shots, meas.ctypes.data, meas.shape[1], flags,
+            N.check(N.lib().mfs_m2d_convert_sweep(
+                self._h, shots, meas.ctypes.data, meas.shape[1],
+                sweep.ctypes.data if sweep is not None else None, sweep.shape[1] if sweep is not None else 0, flags,
                 dets.ctypes.data if dets.size else None, max(dets.shape[1], 1),
                 obs.ctypes.data if obs is not None and obs.size else None,
                 max(obs.shape[1], 1) if obs is not None else 0))
@@ -95,10 +115,14 @@ class CompiledMeasurementsToDetectionEventsConverter:
                      append_observables: bool = False, obs_out_filepath=None, obs_out_format: str = '01'):
         """Reads measurements from a file, converts them, and writes detection events (and observables)."""
         rows = formats.read_rows(os.fspath(measurements_filepath), measurements_format, self.num_measurements)
+        sweep = None
         if sweep_bits_filepath is not None:
-            raise ValueError('sweep bits are not supported by this converter')
+            sweep = formats.read_rows(os.fspath(sweep_bits_filepath), sweep_bits_format, self.num_sweep_bits)
+            if len(sweep) != len(rows):
+                raise ValueError('The sweep data file holds a different number of shots than the measurement file')
         separate = obs_out_filepath is not None
-        out = self.convert(measurements=rows, separate_observables=separate, append_observables=append_observables)
+        out = self.convert(measurements=rows, sweep_bits=sweep, separate_observables=separate,
+                           append_observables=append_observables)
         dets, obs = out if separate else (out, None)
         D, O = self.num_detectors, self.num_observables
         prefixes = [('D', D)] + ([('L', O)] if append_observables else [])

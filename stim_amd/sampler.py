@@ -286,6 +286,10 @@ class CompiledDetectorSampler:
     def set_symptom_tile_kb(self, kb: int):
         N.check(N.lib().mfs_set_symptom_tile_kb(self._h, int(kb)))
 
+    def set_scan_variant(self, variant: str):
+        """'auto' (specialised symptom scan kernel where it applies) or 'generic' (rows are identical)."""
+        N.check(N.lib().mfs_set_scan_variant(self._h, {'auto': 0, 'generic': 1}[variant]))
+
     @property
     def stream_position(self) -> int:
         """Global shot-tile index of the random stream where the next call starts."""

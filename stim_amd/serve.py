@@ -87,9 +87,13 @@ class Server:
         self._samplers.pop(rid, None)
 
     def stats(self):
-        out = (ctypes.c_uint64 * 6)()
+        out = (ctypes.c_uint64 * 8)()
         N.check(N.lib().mfs_server_stats(self._h, out))
-        return dict(zip(['steps', 'queued', 'active', 'bytes_in_use', 'peak_bytes_in_use', 'budget'], list(out)))
+        v = list(out)
+        d = dict(zip(['steps', 'queued', 'active', 'bytes_in_use', 'peak_bytes_in_use', 'budget'], v[:6]))
+        d['kernel_ms'] = v[6] / 1000.0
+        d['bytes_written'] = v[7]
+        return d
 
     def replay(self, samplers, shots, seeds, arrival_s):
         """Runs a request trace in real time (arrival times in seconds from now); each result is
@@ -131,8 +135,9 @@ def serving_pool(seed=2026, n_surface=80, n_rep=80, n_clifford=40, clifford_laye
     return pool
 
 
-def bench(qps=40.0, requests=400, seed=2026, budget_bytes=200e9):
-    """Poisson request stream over the serving pool; returns the benchmark's JSON record."""
+def bench(qps=40.0, requests=400, seed=2026, budget_bytes=200e9, trace=None):
+    """Poisson request stream over the serving pool; returns the benchmark's JSON record. When `trace` is
+    a dict it receives the pool and the requests (circuit index and shots of each)."""
     import time
     import stim_amd
     t0 = time.perf_counter()
@@ -148,6 +153,8 @@ def bench(qps=40.0, requests=400, seed=2026, budget_bytes=200e9):
         arrivals.append(t)
         chosen.append(rng.randrange(len(samplers)))
         shots.append(int(math.exp(rng.uniform(math.log(1e3), math.log(1e7)))))
+    if trace is not None:
+        trace.update(pool=pool, chosen=chosen, shots=shots, samplers=samplers)
     srv = Server(budget_bytes=budget_bytes)
     lat, wall = srv.replay([samplers[c] for c in chosen], shots, list(range(requests)), arrivals)
     total_shots = sum(shots)
@@ -175,6 +182,12 @@ def bench(qps=40.0, requests=400, seed=2026, budget_bytes=200e9):
                        'mean': float(lat.mean()), 'max': float(lat.max())},
         'qps_achieved': requests / wall,
         'output_GBps': out_bytes / wall / 1e9,
+        'roofline': {'bound': 'hbm', 'achieved': st['bytes_written'] / max(st['kernel_ms'], 1e-9) * 1e-6,
+                     'peak': 8000.0, 'unit': 'GB/s',
+                     'frac': st['bytes_written'] / max(st['kernel_ms'], 1e-9) * 1e-6 / 8000.0, 'traffic': None,
+                     'kernel': 'sym_batch_kernel', 'kernel_ms': st['kernel_ms'] / max(st['steps'], 1),
+                     'algorithmic_bytes_per_launch': st['bytes_written'] / max(st['steps'], 1)},
+        'gpu_busy_frac': st['kernel_ms'] / (wall * 1e3),
         'peak_bytes_in_use': st['peak_bytes_in_use'],
         'compile_and_build_s': compile_s,
         'wall_s': wall,

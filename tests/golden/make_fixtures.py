@@ -385,6 +385,44 @@ def write_m2d_fixtures():
     json.dump(cases, open(os.path.join(HERE, 'm2d_vectors.json'), 'w'))
 
 
+def write_m2d_sweep_fixtures():
+    """Reference `stim m2d --sweep` conversions of seeded random measurement and sweep data on circuits
+    with sweep-controlled gates (zoo.sweep_zoo), with and without the reference sample, observables
+    appended (bit-packed rows as hex)."""
+    import numpy as np
+    nrng = np.random.default_rng(20261205)
+    cases = []
+    for seed in range(24):
+        n_sweep = (5, 40, 70)[seed % 3]
+        text = zoo.sweep_zoo(3000 + seed, n_qubits=5 + seed % 4, layers=14 + seed % 7, n_sweep=n_sweep)
+        s = st.stats(st.parse_flat(text))
+        m, nd, no = s['num_measurements'], s['num_detectors'], s['num_observables']
+        k = max([int(x) + 1 for x in __import__('re').findall(r'sweep\[(\d+)\]', text)] + [0])
+        shots = 64
+        bits = nrng.random((shots, m)) < 0.5
+        sweep = nrng.random((shots, k)) < 0.5
+        for skip in (False, True):
+            with tempfile.TemporaryDirectory() as td:
+                cpath = os.path.join(td, 'c.stim')
+                with open(cpath, 'w') as f:
+                    f.write(text)
+                ipath, spath, opath = (os.path.join(td, x) for x in ('m.b8', 's.b8', 'd.b8'))
+                np.packbits(bits, axis=1, bitorder='little').tofile(ipath)
+                np.packbits(sweep, axis=1, bitorder='little').tofile(spath)
+                args = ['m2d', '--circuit', cpath, '--in', ipath, '--in_format', 'b8', '--sweep', spath,
+                        '--sweep_format', 'b8', '--out', opath, '--out_format', 'b8', '--append_observables']
+                if skip:
+                    args.append('--skip_reference_sample')
+                run(args, stdin='')
+                out = np.fromfile(opath, dtype=np.uint8).reshape(shots, (nd + no + 7) // 8)
+            cases.append({'name': 'sweep_zoo_%d' % (3000 + seed), 'skip_reference_sample': skip, 'shots': shots,
+                          'circuit': text, 'num_sweep_bits': k,
+                          'measurements': np.packbits(bits, axis=1, bitorder='little').tobytes().hex(),
+                          'sweep': np.packbits(sweep, axis=1, bitorder='little').tobytes().hex(),
+                          'expected': out.tobytes().hex()})
+    json.dump(cases, open(os.path.join(HERE, 'm2d_sweep_vectors.json'), 'w'))
+
+
 def write_zoo_counts():
     """Seeded detector/observable firing counts of the reference CLI (`stim detect`, 2^20 shots) on
     gate-zoo circuits whose noise includes PAULI_CHANNEL_1/2, heralded channels and correlated-error
@@ -444,6 +482,9 @@ def write_zoo_counts():
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == '--m2d-sweep-only':
+        write_m2d_sweep_fixtures()
+        return
     if len(sys.argv) > 1 and sys.argv[1] == '--zoo-counts-only':
         write_zoo_counts()
         return

@@ -134,8 +134,6 @@ def test_replay_rejects_bad_shapes():
         s.sample(3, recorded_errors_to_replay=np.zeros((3, 3), dtype=np.bool_))
     with pytest.raises(ValueError):
         s.sample(3, recorded_errors_to_replay=np.zeros((2, 2), dtype=np.bool_))
-    with pytest.raises(ValueError):
-        s.sample(3, return_errors=True)
 
 
 # ---------------------------------------------------------------- seeded statistics
@@ -265,3 +263,104 @@ def test_d25_model_full_size_rates(golden):
 def _check_rates_counts(counts, rates, shots):
     z = _z(counts, shots, rates)
     assert np.max(np.abs(z)) < _zcrit(len(z)), np.argsort(-np.abs(z))[:5]
+
+
+# ---------------------------------------------------------------- returned errors
+
+def test_reference_docstring_example_returns_errors():
+    """The reference's own example (dem_sampler.pybind.cc:88-110): error(0) D0, error(1) D1 D2 L0."""
+    s = _sampler('error(0) D0\nerror(1) D1 D2 L0\n', seed=3)
+    d, o, e = s.sample(shots=4, return_errors=True)
+    np.testing.assert_array_equal(d, [[False, True, True]] * 4)
+    np.testing.assert_array_equal(o, [[True]] * 4)
+    np.testing.assert_array_equal(e, [[False, True]] * 4)
+    d, o, e = s.sample(shots=4, return_errors=True, bit_packed=True)
+    np.testing.assert_array_equal(e, [[2]] * 4)
+
+
+@pytest.mark.parametrize('name', ['surface_d5_r5', 'rep_d7_r5_folded', 'color_d5_r3_folded'])
+def test_returned_errors_replay_to_the_same_events(golden, name):
+    """Returned errors are exactly the mechanisms that fired: replaying them (oracle and GPU replay)
+    gives the same detection events and observables; the draws are those of a plain call."""
+    text = _dem(golden, name)
+    m = do.parse(text)
+    s = _sampler(text, seed=17)
+    plain = _sampler(text, seed=17)
+    shots = 20000
+    d, o, e = s.sample(shots, return_errors=True)
+    assert e.shape == (shots, s.num_errors) and e.dtype == np.bool_
+    pd, po, _ = plain.sample(shots)
+    np.testing.assert_array_equal(d, pd)
+    np.testing.assert_array_equal(o, po)
+    od, oo = do.replay(m, [np.nonzero(r)[0].tolist() for r in e])
+    np.testing.assert_array_equal(d, od)
+    np.testing.assert_array_equal(o, oo)
+    rd, ro, re = _sampler(text).sample(shots, recorded_errors_to_replay=e, return_errors=True)
+    np.testing.assert_array_equal(rd, d)
+    np.testing.assert_array_equal(ro, o)
+    np.testing.assert_array_equal(re, e)
+    # Bit-packed errors are the packed bool errors (next call of both samplers).
+    d2, o2, e2 = s.sample(5000, return_errors=True, bit_packed=True)
+    pd2, po2, _ = plain.sample(5000, bit_packed=True)
+    np.testing.assert_array_equal(d2, pd2)
+    bits = np.unpackbits(e2, axis=1, bitorder='little')[:, :s.num_errors].astype(bool)
+    rd2, _, _ = _sampler(text).sample(5000, recorded_errors_to_replay=bits, bit_packed=True)
+    np.testing.assert_array_equal(rd2, d2)
+
+
+def test_returned_error_rates_match_probabilities():
+    """Each mechanism's firing rate against its probability (thinned buckets over four decades)."""
+    rng = np.random.default_rng(12)
+    text = _synthetic_dem(rng, 3000, 300, 2, 1e-3, 0.4, max_dets=2,
+                          extra_lines=['error(1) D0', 'error(0) D1', 'error(0.75) D2 D3'])
+    m = do.parse(text)
+    s = _sampler(text, seed=8)
+    shots = 1 << 18
+    _, _, e = s.sample(shots, return_errors=True, bit_packed=True)
+    counts = np.zeros(((s.num_errors + 7) // 8) * 8, dtype=np.int64)
+    for b in range(8):
+        counts[b::8] = ((e >> b) & 1).sum(axis=0, dtype=np.int64)
+    probs = np.array([p for p, _, _ in m[2]])
+    _check_rates_counts_bounded(counts[:s.num_errors], probs, shots)
+
+
+def _check_rates_counts_bounded(counts, probs, shots):
+    assert np.all(counts[probs == 0] == 0) and np.all(counts[probs == 1] == shots)
+    live = (probs > 0) & (probs < 1)
+    z = _z(counts[live], shots, probs[live])
+    assert np.max(np.abs(z)) < _zcrit(int(live.sum())), np.argsort(-np.abs(z))[:5]
+
+
+def test_d25_model_returned_errors_consistent(golden):
+    """The d=25 model (356,321 mechanisms): errors returned for 4,096 shots replay to the same events."""
+    text = _d25_text(golden)
+    s = _sampler(text, seed=23)
+    d, o, e = s.sample(4096, return_errors=True, bit_packed=True)
+    assert e.shape == (4096, (356321 + 7) // 8)
+    fired = np.unpackbits(e, axis=1, bitorder='little')[:, :356321].sum(axis=1)
+    assert 60 < fired.mean() < 160
+    rd, ro, _ = _sampler(text).sample(4096, recorded_errors_to_replay=e, bit_packed=True)
+    np.testing.assert_array_equal(rd, d)
+    np.testing.assert_array_equal(ro, o)
+
+
+@pytest.mark.parametrize('fmt', ['01', 'b8', 'dets', 'hits', 'r8'])
+def test_sample_write_err_out_and_replay_files(golden, tmp_path, fmt):
+    from stim_amd import formats
+    text = _dem(golden, 'surface_d3_r3')
+    s = _sampler(text, seed=4)
+    ref = _sampler(text, seed=4)
+    shots = 640
+    D, O, E = s.num_detectors, s.num_observables, s.num_errors
+    paths = {k: str(tmp_path / f'{k}.{fmt}') for k in ('d', 'o', 'e', 'd2', 'o2')}
+    s.sample_write(shots, det_out_file=paths['d'], det_out_format=fmt, obs_out_file=paths['o'], obs_out_format=fmt,
+                   err_out_file=paths['e'], err_out_format=fmt)
+    d, o, e = ref.sample(shots, return_errors=True)
+    np.testing.assert_array_equal(formats.read_rows(paths['d'], fmt, D, [('D', D)]), d)
+    np.testing.assert_array_equal(formats.read_rows(paths['o'], fmt, O, [('L', O)]), o)
+    np.testing.assert_array_equal(formats.read_rows(paths['e'], fmt, E, [('M', E)]), e)
+    # Replaying the written errors reproduces the written events.
+    _sampler(text).sample_write(shots, det_out_file=paths['d2'], det_out_format=fmt, obs_out_file=paths['o2'],
+                                obs_out_format=fmt, replay_err_in_file=paths['e'], replay_err_in_format=fmt)
+    np.testing.assert_array_equal(formats.read_rows(paths['d2'], fmt, D, [('D', D)]), d)
+    np.testing.assert_array_equal(formats.read_rows(paths['o2'], fmt, O, [('L', O)]), o)

@@ -182,3 +182,74 @@ def test_columns_without_measurements():
     assert p.shape == (45, 1) and not p.any()
     d, o = conv.convert(measurements=ms, separate_observables=True)
     assert d.shape == (45, 2) and o.shape == (45, 3) and not d.any() and not o.any()
+
+
+# ---------------------------------------------------------------- sweep data
+
+def test_sweep_matches_reference_conversions(golden, tmp_path):
+    """Circuits with sweep-controlled CX/CY/CZ/XCZ/YCZ (also inside REPEAT, mixed with quantum pairs,
+    reaching observables through Pauli targets): bit-exact against the reference CLI's
+    `stim m2d --sweep` on random measurement and sweep data, with and without the reference sample,
+    through the array API (bool and bit-packed, appended and separate observables) and files."""
+    from stim_amd import formats
+    n = 0
+    for c in golden.load('m2d_sweep_vectors.json'):
+        conv = _converter(c['circuit'], c['skip_reference_sample'])
+        M, D, O, K = conv.num_measurements, conv.num_detectors, conv.num_observables, conv.num_sweep_bits
+        assert K == c['num_sweep_bits'], c['name']
+        meas = _rows(c['measurements'], c['shots'], M)
+        sweep = _rows(c['sweep'], c['shots'], K)
+        want = _rows(c['expected'], c['shots'], D + O)
+        got = conv.convert(measurements=meas, sweep_bits=sweep, append_observables=True)
+        np.testing.assert_array_equal(got, want, err_msg=c['name'])
+        got_p = conv.convert(measurements=np.packbits(meas, axis=1, bitorder='little'),
+                             sweep_bits=np.packbits(sweep, axis=1, bitorder='little'), append_observables=True,
+                             bit_packed=True)
+        np.testing.assert_array_equal(got_p, np.packbits(want, axis=1, bitorder='little'), err_msg=c['name'])
+        d, o = conv.convert(measurements=meas, sweep_bits=sweep, separate_observables=True)
+        np.testing.assert_array_equal(d, want[:, :D])
+        np.testing.assert_array_equal(o, want[:, D:])
+        if n % 6 == 0:
+            mp, sp, dp = tmp_path / 'm.01', tmp_path / 's.b8', tmp_path / 'd.01'
+            formats.write_rows(str(mp), '01', meas)
+            formats.write_rows(str(sp), 'b8', sweep)
+            conv.convert_file(measurements_filepath=mp, measurements_format='01', sweep_bits_filepath=sp,
+                              sweep_bits_format='b8', detection_events_filepath=dp, detection_events_format='01',
+                              append_observables=True)
+            np.testing.assert_array_equal(formats.read_rows(str(dp), '01', D + O), want)
+        n += 1
+    assert n >= 40
+
+
+def test_sweep_data_is_linear_and_checked(golden):
+    """Larger random data: conversion is affine in the sweep bits and measurements
+    (f(m, s) = f(m, 0) ^ f(0, s) ^ f(0, 0)); sweep rows on the device; shape errors."""
+    import torch
+    from stim_amd import _native as N
+    c = golden.load('m2d_sweep_vectors.json')[5]
+    conv = _converter(c['circuit'])
+    M, D, O, K = conv.num_measurements, conv.num_detectors, conv.num_observables, conv.num_sweep_bits
+    rng = np.random.default_rng(11)
+    shots = 3001
+    meas = rng.random((shots, M)) < 0.5
+    sweep = rng.random((shots, K)) < 0.5
+    f = lambda m, s: conv.convert(measurements=m, sweep_bits=s, append_observables=True)  # noqa: E731
+    zm, zs = np.zeros_like(meas), np.zeros_like(sweep)
+    full = f(meas, sweep)
+    np.testing.assert_array_equal(full, f(meas, zs) ^ f(zm, sweep) ^ f(zm, zs))
+    np.testing.assert_array_equal(f(meas, zs), conv.convert(measurements=meas, append_observables=True))
+    assert (f(zm, sweep) != f(zm, zs)).any()
+    # Device rows.
+    dm = torch.from_numpy(np.packbits(meas, axis=1, bitorder='little')).cuda()
+    ds = torch.from_numpy(np.packbits(sweep, axis=1, bitorder='little')).cuda()
+    row = (D + O + 7) // 8
+    out = torch.zeros((shots, row), dtype=torch.uint8, device='cuda:0')
+    flags = N.MFS_BIT_PACKED | N.MFS_APPEND_OBSERVABLES | N.MFS_DEVICE_OUTPUT
+    N.check(N.lib().mfs_m2d_convert_sweep(conv._h, shots, dm.data_ptr(), dm.stride(0), ds.data_ptr(), ds.stride(0),
+                                          flags, out.data_ptr(), row, None, 0))
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(out.cpu().numpy(), np.packbits(full, axis=1, bitorder='little'))
+    with pytest.raises(ValueError):
+        conv.convert(measurements=meas, sweep_bits=sweep[:, :-1], append_observables=True)
+    with pytest.raises(ValueError):
+        conv.convert(measurements=meas, sweep_bits=sweep[:-1], append_observables=True)

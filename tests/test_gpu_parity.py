@@ -435,3 +435,44 @@ def test_rates_match_exact_rates_on_channel_zoo(golden, engine):
         counts = _column_counts(d, len(exact))
         z = _z(counts, shots, exact)
         assert np.max(np.abs(z)) < _zcrit(len(z)), (name, np.argsort(-np.abs(z))[:5])
+
+
+OVERFLOW_CIRCUIT = '''R 0 1 2
+DEPOLARIZE2(0.2) 0 1
+DEPOLARIZE1(0.2) 2
+X_ERROR(0.2) 0
+M 0 1 2
+DETECTOR rec[-3]
+DETECTOR rec[-3]
+DETECTOR rec[-3]
+DETECTOR rec[-2]
+DETECTOR rec[-2]
+DETECTOR rec[-2]
+DETECTOR rec[-1] rec[-3]
+OBSERVABLE_INCLUDE(0) rec[-2]
+OBSERVABLE_INCLUDE(1) rec[-3]
+'''
+
+
+@pytest.mark.parametrize('name', ['surface_d5_r5', 'color_d5_r3', 'surface_x_d5_r3', 'rep_d7_r5', 'surface_d25_r25',
+                                  'overflow'])
+def test_specialised_scan_kernel_equals_generic(golden, name):
+    """The specialised scan kernel (one outcome record per site and outcome) draws the generic kernel's
+    hits, so rows are bit-identical in every output layout, including outcomes of more than four
+    detectors (overflow lists) and both observables."""
+    text = OVERFLOW_CIRCUIT if name == 'overflow' else golden.circuit(name)
+    shots = (1 << 14) if name == 'surface_d25_r25' else (1 << 16)
+    for kw in [dict(separate_observables=True), dict(append_observables=True), dict(prepend_observables=True),
+               dict()]:
+        a = _sampler(text, seed=31, engine='symptom')
+        b = _sampler(text, seed=31, engine='symptom')
+        b.set_scan_variant('generic')
+        for packed in (True, False):
+            ra = a.sample(shots, bit_packed=packed, **kw)
+            rb = b.sample(shots, bit_packed=packed, **kw)
+            if isinstance(ra, tuple):
+                for x, y in zip(ra, rb):
+                    np.testing.assert_array_equal(x, y, err_msg=f'{name} {kw} packed={packed}')
+            else:
+                np.testing.assert_array_equal(ra, rb, err_msg=f'{name} {kw} packed={packed}')
+            assert (ra[0] if isinstance(ra, tuple) else ra).any()

@@ -215,3 +215,74 @@ def ref_zoo(seed, n_qubits=6, layers=14):
                 lines.append('X_ERROR(0.4) ' + ' '.join(map(str, sub())))
     lines.append(rng.choice(['M', 'MX', 'MY']) + ' ' + ' '.join(inv(q) for q in qs))
     return '\n'.join(lines) + '\n'
+
+
+def sweep_zoo(seed, n_qubits=6, layers=16, n_sweep=5):
+    """Seeded circuits whose detection events depend on sweep data: sweep-controlled CX/CY/CZ/XCZ/YCZ
+    (sweep bit as control, or on either side of CZ; mixed with quantum and record-controlled pairs in
+    one instruction), random Clifford layers, measurements and resets in every basis, MPP, feedback, a
+    REPEAT block, noise (ignored by the conversion) and detectors / observables over random records,
+    including observables with Pauli targets (which sweep-driven frame flips reach)."""
+    rng = random.Random(seed)
+    qs = list(range(n_qubits))
+    lines = ['R ' + ' '.join(map(str, qs))]
+    nmeas = 0
+
+    def sweep_pair():
+        q, k = rng.choice(qs), rng.randrange(n_sweep)
+        g = rng.choice(['CX', 'CY', 'CZ', 'CZ', 'XCZ', 'YCZ'])
+        if g in ('XCZ', 'YCZ'):
+            return g, f'{q} sweep[{k}]'
+        if g == 'CZ' and rng.random() < 0.5:
+            return g, f'{q} sweep[{k}]'
+        return g, f'sweep[{k}] {q}'
+
+    def layer():
+        nonlocal nmeas
+        out = []
+        k = rng.randrange(9)
+        if k < 2:
+            out.append(rng.choice(ONE_Q) + ' ' + ' '.join(map(str, rng.sample(qs, rng.randint(1, len(qs))))))
+        elif k < 4:
+            ps = rng.sample(qs, 2 * rng.randint(1, len(qs) // 2))
+            out.append(rng.choice(TWO_Q) + ' ' + ' '.join(map(str, ps)))
+        elif k < 6:
+            g, pair = sweep_pair()
+            extra = ''
+            if rng.random() < 0.4:
+                a, b = rng.sample(qs, 2)
+                extra = f' {a} {b}' if rng.random() < 0.5 else f' sweep[{rng.randrange(n_sweep)}] {a}' \
+                    if g in ('CX', 'CY', 'CZ') else ''
+            out.append(f'{g} {pair}{extra}')
+        elif k == 6:
+            ts = rng.sample(qs, rng.randint(1, len(qs)))
+            out.append(rng.choice(['M', 'MX', 'MY', 'MR', 'MRX', 'MRY']) + ' ' + ' '.join(map(str, ts)))
+            nmeas += len(ts)
+        elif k == 7:
+            out.append('MPP ' + _product(rng, qs, rng.randint(1, 3)))
+            nmeas += 1
+        else:
+            out.append(_noise(rng, qs))
+        if nmeas and rng.random() < 0.5:
+            ks = rng.sample(range(1, nmeas + 1), k=min(nmeas, rng.randint(1, 3)))
+            out.append('DETECTOR ' + ' '.join('rec[-%d]' % k for k in ks))
+        if nmeas and rng.random() < 0.15:
+            out.append(f'CX rec[-{rng.randint(1, min(nmeas, 3))}] {rng.choice(qs)}')
+        return out
+
+    for i in range(layers):
+        lines.extend(layer())
+        if i == layers // 2:
+            lines.append('REPEAT 2 {')
+            for _ in range(3):
+                lines.extend('    ' + x for x in layer())
+            lines.append('}')
+            nmeas += 0
+    ts = list(qs)
+    lines.append('M ' + ' '.join(map(str, ts)))
+    nmeas += len(ts)
+    for q in qs[:3]:
+        lines.append(f'DETECTOR rec[-{n_qubits - q}]')
+    lines.append('OBSERVABLE_INCLUDE(0) rec[-1] ' + rng.choice('XYZ') + str(rng.choice(qs)))
+    lines.append('OBSERVABLE_INCLUDE(1) ' + rng.choice('XYZ') + str(rng.choice(qs)))
+    return '\n'.join(lines) + '\n'
