@@ -15,6 +15,7 @@ The default line also carries the metric's second half: a bounded replay of the 
     python bench.py --workload dem          # detector-error-model sampler, the d=25 r=25 model, 2^20 shots
     python bench.py --workload measure      # measurement sampler (compile_sampler), d=25 r=25, 2^20 shots
     python bench.py --workload m2d          # measurement -> detection event conversion, d=25 r=25, 2^20 shots
+    python bench.py --workload sinter       # the drop-in call shape: sample(2^20, bit_packed, separate_observables) into numpy
 """
 
 import argparse
@@ -674,13 +675,53 @@ def m2d(args):
     }
 
 
+def sinter(args):
+    """The reference's own call shape as sinter makes it (src/stim/py/compiled_detector_sampler.pybind.cc:34-86;
+    glue/sample/src/sinter/_decoding/_stim_then_decode_sampler.py:162-174): sample(2^20, bit_packed=True,
+    separate_observables=True) returning numpy arrays in host memory, d=25 r=25 p=1e-3."""
+    import numpy as np
+    import stim_amd
+    circuit = stim_amd.Circuit(open(CIRCUIT).read())
+    sampler = circuit.compile_detector_sampler(seed=2026)
+    S = args.shots
+    D, O = sampler.num_detectors, sampler.num_observables
+    dets = np.empty((S, (D + 7) // 8), dtype=np.uint8)
+    obs = np.empty((S, (O + 7) // 8), dtype=np.uint8)
+    steps = max(1, min(args.steps, 10))
+    for _ in range(max(1, min(args.warmup, 3))):
+        sampler.sample(S, bit_packed=True, separate_observables=True, dets_out=dets, obs_out=obs)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sampler.sample(S, bit_packed=True, separate_observables=True, dets_out=dets, obs_out=obs)
+    wall = (time.perf_counter() - t0) / steps
+    kernel_ms = sampler.bench_device(S, warmup=2, iters=10)
+    out_bytes = S * (dets.shape[1] + obs.shape[1])
+    return {
+        'metric': 'detector shots/s into host numpy arrays, sinter call shape, d=25 r=25 p=1e-3',
+        'value': S / wall, 'unit': 'shots/s', 'n_gpus': 1, 'steps': steps, 'warmup': 1,
+        'ms_per_step': wall * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'u32',
+        'data': 'synthetic: rotated surface code memory-Z d=25 r=25 p=1e-3, seeded Philox noise',
+        'config': {'workload': 'sinter_sample_numpy', 'shots_per_call': S, 'bit_packed': True,
+                   'separate_observables': True, 'output': 'preallocated numpy arrays (dets_out/obs_out), pageable',
+                   'parallelism': 'single GPU'},
+        'host_GBps': out_bytes / wall / 1e9,
+        'pcie_peak_GBps': 64.0,
+        'pcie_frac': out_bytes / wall / 1e9 / 64.0,
+        'kernel_ms': kernel_ms,
+        'kernel_share_of_call': kernel_ms / (wall * 1e3),
+        'cpu_baseline': None if args.no_cpu else cpu_baseline(CIRCUIT),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=50)
     ap.add_argument('--warmup', type=int, default=10)
     ap.add_argument('--shots', type=int, default=1 << 20)
-    ap.add_argument('--workload', choices=['surface_d25', 'batch64', 'serve', 'clifford', 'dem', 'measure', 'm2d'], default='surface_d25')
+    ap.add_argument('--workload', choices=['surface_d25', 'batch64', 'serve', 'clifford', 'dem', 'measure', 'm2d',
+                                           'sinter'], default='surface_d25')
     ap.add_argument('--no-cpu', action='store_true')
     ap.add_argument('--tile-words', type=int, default=0)
     ap.add_argument('--calibrate-copy', action='store_true',
@@ -712,6 +753,8 @@ def main():
         line = measure(args)
     elif args.workload == 'm2d':
         line = m2d(args)
+    elif args.workload == 'sinter':
+        line = sinter(args)
     else:
         from stim_amd import serve
         trace = {}

@@ -164,11 +164,12 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
  * No reference counterpart (the reference has one engine). */
 int mfs_set_engine(mfs_sampler *s, int engine);
 
-/* out[0..7] = engine in use for sampling (1 frame, 2 symptom), site classes, symptom columns,
- * symptom entries, symptom-table build time in microseconds, noise sites, and for the symptom engine
- * the shots per workgroup tile and the sampling kernel's LDS bytes per tile (bit-packed output with
- * separate observables). Builds the symptom table if needed. */
-int mfs_engine_info(mfs_sampler *s, uint64_t out[8]);
+/* out[0..9] = engine in use for sampling (1 frame, 2 symptom), site classes, symptom columns,
+ * symptom entries, symptom-table build time in microseconds, noise sites, for the symptom engine the
+ * shots per workgroup tile and the sampling kernel's LDS bytes per tile (bit-packed output with
+ * separate observables), and the specialised scan kernel's distinct outcome blocks and outcome
+ * records (0 when it does not apply). Builds the symptom table if needed. */
+int mfs_engine_info(mfs_sampler *s, uint64_t out[10]);
 
 /* Symptom engine: target size (KB) of one workgroup's LDS image of output rows (sets the shots per
  * workgroup). Default 32. A negative value selects the per-lane-walk queue kernel instead of the

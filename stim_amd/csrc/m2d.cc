@@ -334,10 +334,10 @@ int mfs_m2d_convert_sweep(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_
             c->d_obs.ensure(n * std::max<size_t>(obs_row, 1));
             if (c->M) {
                 if (device) {
-                    check(hipMemcpy2DAsync(c->d_in.p, vs, meas + s0 * meas_stride, meas_stride, c->in_row(), n,
+                    check(copy_rows_async(c->d_in.p, vs, meas + s0 * meas_stride, meas_stride, c->in_row(), n,
                                            hipMemcpyDeviceToDevice, c->stream), "copy measurements");
                 } else {
-                    check(hipMemcpy2DAsync(c->d_in.p, vs, meas + s0 * meas_stride, meas_stride, c->in_row(), n,
+                    check(copy_rows_async(c->d_in.p, vs, meas + s0 * meas_stride, meas_stride, c->in_row(), n,
                                            hipMemcpyHostToDevice, c->stream), "copy measurements");
                 }
             }
@@ -345,7 +345,7 @@ int mfs_m2d_convert_sweep(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_
                                     n_out, c->d_obs.as<uint8_t>(), std::max<size_t>(obs_row, 1), n_obs);
             if (sweep) {
                 c->d_sweep.ensure(n * srow);
-                check(hipMemcpy2DAsync(c->d_sweep.p, srow, sweep + s0 * sweep_stride, sweep_stride, srow, n,
+                check(copy_rows_async(c->d_sweep.p, srow, sweep + s0 * sweep_stride, sweep_stride, srow, n,
                                        device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c->stream),
                       "copy sweep bits");
                 P.sweep = c->d_sweep.as<uint8_t>();
@@ -364,7 +364,7 @@ int mfs_m2d_convert_sweep(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_
                     from_stride = ncols;
                     width = ncols;
                 }
-                check(hipMemcpy2DAsync(dst + s0 * dst_stride, dst_stride, from, from_stride, width, n, back, c->stream),
+                check(copy_rows_async(dst + s0 * dst_stride, dst_stride, from, from_stride, width, n, back, c->stream),
                       "copy results");
                 check(hipStreamSynchronize(c->stream), "m2d");
             };

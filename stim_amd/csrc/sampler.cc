@@ -440,7 +440,7 @@ int mfs_set_engine(mfs_sampler *s, int engine) {
     });
 }
 
-int mfs_engine_info(mfs_sampler *s, uint64_t out[8]) {
+int mfs_engine_info(mfs_sampler *s, uint64_t out[10]) {
     return guarded([&] {
         const bool sym = s->use_symptoms(false);
         out[0] = sym ? 2 : 1;
@@ -449,7 +449,11 @@ int mfs_engine_info(mfs_sampler *s, uint64_t out[8]) {
         out[3] = s->sym_entries;
         out[4] = (uint64_t)(s->sym_build_ms * 1000.0);
         out[5] = s->plan.total_sites;
-        out[6] = out[7] = 0;
+        out[6] = out[7] = out[8] = out[9] = 0;
+        if (sym && s->sym_fast) {
+            out[8] = s->orec_blocks;
+            out[9] = s->orec_records;
+        }
         if (sym) {
             // Tile of a bit-packed sample with separate observables.
             static uint8_t dummy;

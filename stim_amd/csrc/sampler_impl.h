@@ -17,6 +17,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/stim_amd.h"
@@ -26,6 +27,7 @@
 #include "dem.h"
 #include "symptom.h"
 #include "rng.h"
+#include "host_pipe.h"
 
 namespace sa {
 hipError_t launch_frame_kernel(int T, const DevProgram &prog, const LaunchParams &P, uint32_t num_tiles, size_t lds,
@@ -91,6 +93,8 @@ inline uint64_t splitmix64(uint64_t x) {
 
 constexpr size_t MAX_LDS = LDS_PER_WORKGROUP;
 constexpr size_t HOST_BATCH_BYTES = (size_t)1 << 30;
+// Host-bound bytes per batch of a call that returns rows in host memory (one of two pipeline slots).
+constexpr size_t PIPE_BATCH_BYTES = (size_t)64 << 20;
 
 // Records the message of a failed C-ABI call (read back by mfs_last_error) and returns its kind.
 int record_error(const std::exception &e, int kind);
@@ -139,7 +143,8 @@ struct mfs_sampler {
     // the launch in progress (set only while a call returns errors).
     DevBuf d_site_mech, d_err, d_err_tmp;
     // Outcome records of the specialised scan kernel (built with the symptom table when it applies).
-    DevBuf d_orecs, d_orec_base, d_orec_ovf;
+    DevBuf d_orecs, d_orec_base, d_orec_sites, d_orec_ovf;
+    uint64_t orec_blocks = 0, orec_records = 0;
     bool sym_fast = false;
     int sym_variant = 0;         // 0: specialised scan kernel where it applies; 1: generic scan kernel
     uint32_t *err_words_cur = nullptr;
@@ -167,6 +172,27 @@ struct mfs_sampler {
         size_t cap = 0, used = 0;
     };
     Arena *arena = nullptr;
+    // Calls that return rows in host memory: double-buffered device slots and pinned buffers.
+    HostPipe pipe;
+    // Points staging at the pipeline's next device slot for one batch (restores the previous arena and,
+    // when the call unwinds, ends what the pipeline has in flight).
+    struct SlotScope {
+        mfs_sampler *s;
+        Arena slot{};
+        Arena *prev;
+        explicit SlotScope(mfs_sampler *o) : s(o), prev(o->arena) {
+            slot.base = s->pipe.next_slot(&slot.cap);
+            s->arena = &slot;
+        }
+        ~SlotScope() { s->arena = prev; }
+    };
+    struct PipeCall {
+        mfs_sampler *s;
+        bool on;
+        ~PipeCall() {
+            if (on) s->pipe.drain_noexcept();
+        }
+    };
 
     uint8_t *staging(DevBuf &b, size_t n) {
         if (!arena) {
@@ -337,11 +363,21 @@ struct mfs_sampler {
         const size_t row_bytes = L.det_row + (L.separate ? L.obs_row : 0) + (L.via_unpacked ? L.ncols : 0);
         uint64_t batch = device_out && !L.via_unpacked ? shots
                                                         : std::max<uint64_t>(W64, (HOST_BATCH_BYTES / std::max<size_t>(row_bytes, 1)) / W64 * W64);
-        if (!(device_out && !L.via_unpacked)) {
-            const size_t staged = (L.via_unpacked ? L.ncols : 0) + (device_out ? 0 : L.det_row + (L.separate ? L.obs_row : 0));
-            batch = arena_batch(batch, staged, W64);
+        if (device_out && L.via_unpacked) batch = arena_batch(batch, L.ncols, W64);
+        const bool piped = !device_out;
+        if (piped) {
+            // Host rows travel through the pipeline: batches of about PIPE_BATCH_BYTES of host rows.
+            const size_t host_row = std::max<size_t>(L.det_row + (L.separate ? L.obs_row : 0), 1);
+            batch = std::max<uint64_t>(W64, PIPE_BATCH_BYTES / host_row / W64 * W64);
         }
         batch = std::min(batch, shots);
+        PipeCall pipe_call{this, piped};
+        if (piped) {
+            const size_t dev_row = (L.via_unpacked ? L.ncols : 0) + std::max<size_t>(L.det_row, 1) +
+                                   (L.separate ? std::max<size_t>(L.obs_row, 1) : 0);
+            const size_t host_row = L.det_row + (L.separate ? L.obs_row : 0);
+            pipe.begin(stream, batch * dev_row + 4 * 256, std::max<size_t>(batch * host_row, 256));
+        }
         const bool fixed = faults != nullptr;
         const uint64_t call_tile0 = tiles_used;
         for (uint64_t s0 = 0; s0 < shots; s0 += batch) {
@@ -353,6 +389,8 @@ struct mfs_sampler {
             P.mode = fixed ? MODE_FIXED : MODE_SAMPLED;
             P.randomize = (!fixed && randomize) ? 1 : 0;
             if (fixed) stage_faults(s0, n, tiles, fault_offsets, faults, P);
+            std::unique_ptr<SlotScope> slot;
+            if (piped) slot.reset(new SlotScope(this));
             staging_reset();
             uint8_t *det_dst, *obs_dst;
             size_t det_stride_dev, obs_stride_dev;
@@ -396,24 +434,20 @@ struct mfs_sampler {
                     launch(P, tiles);
                 }
             }
-            if (!device_out) {
-                if (L.det_row && dets) {
-                    check(hipMemcpy2DAsync(dets + s0 * dets_stride, dets_stride, det_dst, det_stride_dev, L.det_row, n,
-                                           hipMemcpyDeviceToHost, stream),
-                          "copy dets");
+            if (piped) {
+                std::vector<HostPipe::Part> parts;
+                if (dets && (L.det_row || newline)) {
+                    parts.push_back({det_dst, det_stride_dev, L.det_row, dets + s0 * dets_stride, dets_stride,
+                                     newline ? (long)L.ncols : -1L});
                 }
                 if (L.separate && obs && L.obs_row && obs_dst) {
-                    check(hipMemcpy2DAsync(obs + s0 * obs_stride, obs_stride, obs_dst, obs_stride_dev, L.obs_row, n,
-                                           hipMemcpyDeviceToHost, stream),
-                          "copy obs");
+                    parts.push_back({obs_dst, obs_stride_dev, L.obs_row, obs + s0 * obs_stride, obs_stride, -1L});
                 }
-                check(hipStreamSynchronize(stream), "sample");
-                if (newline) {
-                    for (uint64_t i = 0; i < n; i++) dets[(s0 + i) * dets_stride + L.ncols] = '\n';
-                }
+                pipe.submit(parts, n);
             }
         }
-        if (device_out) check(hipStreamSynchronize(stream), "sample");
+        if (piped) pipe.finish();
+        check(hipStreamSynchronize(stream), "sample");
         tiles_used += (shots + W64 - 1) / W64;
     }
 
@@ -595,7 +629,10 @@ struct mfs_sampler {
     // records, four slots per site, no thinning) whose members draw outcomes uniformly over codes
     // 1..nout, each (site, code) gets the symptom of that outcome precomputed: the XOR of the basis
     // columns the code selects (bit j = column j, as the generic kernel applies them), with entries
-    // that appear an even number of times removed. A hit then reads one record instead of four.
+    // that appear an even number of times removed. A hit then reads its site's entry and one record
+    // instead of four basis records. Each site's block of nout records is written relative to the
+    // site's lowest detector and stored once per distinct block, so that the table stays small enough
+    // to live in L2 (a d=25 memory circuit's 980,000 outcomes reduce to the blocks of about three rounds).
     void build_outcome_records(const std::vector<uint32_t> &offs, const std::vector<uint32_t> &ents) {
         sym_fast = false;
         const uint32_t D = prog.num_detectors;
@@ -604,21 +641,27 @@ struct mfs_sampler {
             for (uint32_t k = 0; k < g.nmem; k++)
                 if (!(g.nout[k] & SYM_SEQ) || (g.nout[k] & 0xFFFFu) > 15) return;
         }
-        std::vector<uint64_t> orec0(plan.classes.size());
-        uint64_t total = 0;
+        std::vector<uint64_t> site0(plan.classes.size());
+        uint64_t nsites = 0;
         for (size_t c = 0; c < plan.classes.size(); c++) {
-            orec0[c] = total;
-            total += (uint64_t)plan.classes[c].nsites * (plan.classes[c].nout & 0xFFFFu);
+            site0[c] = nsites;
+            nsites += plan.classes[c].nsites;
         }
-        // 8 bytes per outcome; beyond 1 GiB the four-slot records are the better trade.
-        if (total >= (1ull << 31) || total * 8 > ((uint64_t)1 << 30)) return;
-        std::vector<uint2> orec(std::max<uint64_t>(total, 1), make_uint2(0, 0));
+        if (nsites >= (1ull << 31)) return;
+        std::vector<uint2> sites(std::max<uint64_t>(nsites, 1), make_uint2(0, 0));
+        std::vector<uint2> orec;
         std::vector<uint32_t> oovf;
         std::vector<uint32_t> acc;
+        // Distinct relative blocks: key = nout, then per code its relative entries and a terminator.
+        std::unordered_map<std::string, uint32_t> blocks;
+        std::vector<std::vector<uint32_t>> site_syms;
+        std::string key;
         for (size_t c = 0; c < plan.classes.size(); c++) {
             const SymClass &cl = plan.classes[c];
             const uint32_t nout = cl.nout & 0xFFFFu;
+            site_syms.resize(nout);
             for (uint64_t st = 0; st < cl.nsites; st++) {
+                uint32_t lo = D;
                 for (uint32_t code = 1; code <= nout; code++) {
                     acc.clear();
                     for (uint32_t j = 0; j < cl.basis; j++) {
@@ -636,6 +679,33 @@ struct mfs_sampler {
                         }
                     }
                     acc.resize(w);
+                    for (uint32_t e : acc)
+                        if (e < D) lo = std::min(lo, e);
+                    site_syms[code - 1] = acc;
+                }
+                const uint32_t b = lo < D ? lo : 0;
+                key.assign(reinterpret_cast<const char *>(&nout), 4);
+                for (uint32_t code = 1; code <= nout; code++) {
+                    for (uint32_t e : site_syms[code - 1]) {
+                        const uint32_t v = e < D ? e - b : e;
+                        key.append(reinterpret_cast<const char *>(&v), 4);
+                    }
+                    const uint32_t end = 0xFFFFFFFFu;
+                    key.append(reinterpret_cast<const char *>(&end), 4);
+                }
+                auto found = blocks.find(key);
+                if (found != blocks.end()) {
+                    sites[site0[c] + st] = make_uint2(found->second, b);
+                    continue;
+                }
+                const uint64_t first = orec.size();
+                if (first + nout >= (1ull << 31) || (first + nout) * 8 > ((uint64_t)1 << 30)) return;
+                blocks.emplace(key, (uint32_t)first);
+                sites[site0[c] + st] = make_uint2((uint32_t)first, b);
+                for (uint32_t code = 1; code <= nout; code++) {
+                    acc = site_syms[code - 1];
+                    for (uint32_t &e : acc)
+                        if (e < D) e -= b;
                     uint32_t dets[4], nd = 0, obs = 0;
                     bool spill = false;
                     for (uint32_t e : acc) {
@@ -654,21 +724,22 @@ struct mfs_sampler {
                         oovf.push_back((uint32_t)acc.size());
                         oovf.insert(oovf.end(), acc.begin(), acc.end());
                     }
-                    orec[orec0[c] + st * nout + code - 1] = r;
+                    orec.push_back(r);
                 }
             }
         }
         std::vector<uint32_t> base(4 * std::max<size_t>(plan.groups.size(), 1), 0);
         for (size_t gi = 0; gi < plan.groups.size(); gi++) {
             const SymGroup &g = plan.groups[gi];
-            for (uint32_t k = 0; k < g.nmem; k++) {
-                const uint32_t nout = g.nout[k] & 0xFFFFu;
-                base[4 * gi + k] = (uint32_t)(orec0[g.c0 + k] - (uint64_t)g.bound[k] * nout);
-            }
+            for (uint32_t k = 0; k < g.nmem; k++) base[4 * gi + k] = (uint32_t)(site0[g.c0 + k] - g.bound[k]);
         }
+        if (orec.empty()) orec.push_back(make_uint2(0, 0));
         upload(d_orecs, orec, "outcome records");
-        upload(d_orec_base, base, "outcome record bases");
+        upload(d_orec_base, base, "outcome record site bases");
+        upload(d_orec_sites, sites, "outcome record sites");
         upload(d_orec_ovf, oovf, "outcome record overflow lists");
+        orec_blocks = blocks.size();
+        orec_records = orec.size();
         sym_fast = true;
     }
 
@@ -875,6 +946,7 @@ struct mfs_sampler {
         P.seed = splitmix64(key ^ 0x5A5A5A5Aull);
         P.orecs = nullptr;
         P.orec_base = nullptr;
+        P.orec_sites = nullptr;
         P.orec_ovf = nullptr;
         P.err_words = err_words_cur;
         P.err_stride_words = err_stride_cur;
@@ -941,6 +1013,7 @@ struct mfs_sampler {
         if (r.fast) {
             P.orecs = d_orecs.as<uint2>();
             P.orec_base = d_orec_base.as<uint32_t>();
+            P.orec_sites = d_orec_sites.as<uint2>();
             P.orec_ovf = d_orec_ovf.as<uint32_t>();
         }
         return r;
@@ -970,15 +1043,16 @@ struct mfs_sampler {
         const size_t ew = (E + 31) / 32, erow = L.packed ? (E + 7) / 8 : E;
         const size_t pdet = (L.ncols + 7) / 8, pobs = (O + 7) / 8;
         const bool want_obs = L.separate && obs && O;
+        const bool piped = !device_out;
         uint64_t batch = shots;
-        if (!(device_out && L.packed)) {
+        if (device_out && !L.packed) {
             // Batches are whole tiles of the launch's tile size, so the split never changes the tiles'
             // RNG streams.
             static uint8_t dummy;
             const uint64_t S = sym_prepare(L, 1, &dummy, pdet, want_obs ? &dummy : nullptr, pobs, 0).P.shots_per_tile;
             batch = std::max<uint64_t>(1024, HOST_BATCH_BYTES / std::max<size_t>(pdet + pobs + L.ncols + O, 1));
             batch = std::max<uint64_t>(S, batch / S * S);
-            batch = arena_batch(batch, pdet + (want_obs ? pobs : 0) + (device_out ? 0 : L.ncols + O), S);
+            batch = arena_batch(batch, pdet + (want_obs ? pobs : 0), S);
         }
         if (want_err) {
             // Error rows take 32-bit words per mechanism block; batches stay whole tiles.
@@ -987,6 +1061,21 @@ struct mfs_sampler {
             const size_t per = 4 * ew + (L.packed ? 0 : E) + pdet + pobs;
             batch = std::min<uint64_t>(batch, std::max<uint64_t>(S, HOST_BATCH_BYTES / std::max<size_t>(per, 1) / S * S));
         }
+        PipeCall pipe_call{this, piped};
+        if (piped) {
+            // Host rows travel through the pipeline: batches of about PIPE_BATCH_BYTES of host rows, in
+            // whole tiles so that the split never changes the tiles' RNG streams.
+            static uint8_t dummy;
+            const uint64_t S = sym_prepare(L, 1, &dummy, pdet, want_obs ? &dummy : nullptr, pobs, 0).P.shots_per_tile;
+            const size_t host_row = (L.packed ? pdet : L.ncols) + (want_obs ? (L.packed ? pobs : O) : 0) +
+                                    (want_err ? erow : 0);
+            batch = std::max<uint64_t>(S, PIPE_BATCH_BYTES / std::max<size_t>(host_row, 1) / S * S);
+            batch = std::min(batch, shots);
+            const size_t dev_row = std::max<size_t>(pdet, 1) + (want_obs ? std::max<size_t>(pobs, 1) : 0) +
+                                   (L.packed ? 0 : std::max<size_t>(L.ncols + O, 1)) +
+                                   (want_err ? 4 * ew + (L.packed ? 0 : E) : 0);
+            pipe.begin(stream, batch * dev_row + 6 * 256, std::max<size_t>(batch * host_row, 256));
+        }
         const uint64_t call_tile0 = tiles_used;
         uint64_t tiles_total = 0;
         for (uint64_t s0 = 0; s0 < shots; s0 += batch) {
@@ -994,6 +1083,8 @@ struct mfs_sampler {
             // Packed destination rows for this batch.
             uint8_t *pd, *po = nullptr;
             size_t pd_stride, po_stride = 0;
+            std::unique_ptr<SlotScope> slot;
+            if (piped) slot.reset(new SlotScope(this));
             if (L.packed && device_out) {
                 pd = dets + s0 * dets_stride;
                 pd_stride = dets_stride;
@@ -1013,10 +1104,16 @@ struct mfs_sampler {
             const uint64_t *fo = nullptr;
             const uint32_t *fc = nullptr;
             if (faults) stage_symptom_faults(s0, n, fault_offsets, faults, &fo, &fc);
+            uint8_t *err_dev = nullptr;
             if (want_err) {
-                d_err.ensure(n * ew * 4);
-                check(hipMemsetAsync(d_err.p, 0, n * ew * 4, stream), "clear error rows");
-                err_words_cur = d_err.as<uint32_t>();
+                if (piped) {
+                    err_dev = staging(d_err, n * ew * 4);
+                } else {
+                    d_err.ensure(n * ew * 4);
+                    err_dev = d_err.as<uint8_t>();
+                }
+                check(hipMemsetAsync(err_dev, 0, n * ew * 4, stream), "clear error rows");
+                err_words_cur = reinterpret_cast<uint32_t *>(err_dev);
                 err_stride_cur = ew;
             }
             struct ClearErr {
@@ -1027,21 +1124,25 @@ struct mfs_sampler {
                 }
             } clear_err{this};
             tiles_total += sym_launch(L, n, pd, pd_stride, po, po_stride, call_tile0 + tiles_total, fo, fc);
-            if (want_err && E) {
+            std::vector<HostPipe::Part> parts;
+            if (want_err && E && piped) {
                 // Little-endian 32-bit words hold the bits in the packed byte order.
-                const hipMemcpyKind kind = device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
                 if (L.packed) {
-                    check(hipMemcpy2DAsync(errs + s0 * errs_stride, errs_stride, d_err.p, ew * 4, erow, n, kind, stream),
-                          "copy error rows");
-                } else if (device_out) {
-                    check(launch_unpack_rows(d_err.as<uint8_t>(), ew * 4, errs + s0 * errs_stride, errs_stride, n,
-                                             (uint32_t)E, 0, stream), "unpack error rows");
+                    parts.push_back({err_dev, ew * 4, erow, errs + s0 * errs_stride, errs_stride, -1L});
                 } else {
-                    d_err_tmp.ensure(n * E);
-                    check(launch_unpack_rows(d_err.as<uint8_t>(), ew * 4, d_err_tmp.as<uint8_t>(), E, n, (uint32_t)E, 0,
-                                             stream), "unpack error rows");
-                    check(hipMemcpy2DAsync(errs + s0 * errs_stride, errs_stride, d_err_tmp.p, E, E, n, kind, stream),
+                    uint8_t *eu = staging(d_err_tmp, n * E);
+                    check(launch_unpack_rows(err_dev, ew * 4, eu, E, n, (uint32_t)E, 0, stream), "unpack error rows");
+                    parts.push_back({eu, E, E, errs + s0 * errs_stride, errs_stride, -1L});
+                }
+            } else if (want_err && E) {
+                // Device rows: copied (packed) or expanded to bytes in place.
+                if (L.packed) {
+                    check(copy_rows_async(errs + s0 * errs_stride, errs_stride, err_dev, ew * 4, erow, n,
+                                           hipMemcpyDeviceToDevice, stream),
                           "copy error rows");
+                } else {
+                    check(launch_unpack_rows(err_dev, ew * 4, errs + s0 * errs_stride, errs_stride, n, (uint32_t)E, 0,
+                                             stream), "unpack error rows");
                 }
             }
             if (!L.packed) {
@@ -1062,20 +1163,17 @@ struct mfs_sampler {
                 po = uo;
                 po_stride = uo_stride;
             }
-            if (!device_out) {
+            if (piped) {
                 const size_t drow = L.packed ? pdet : L.ncols, orow = L.packed ? pobs : O;
-                if (drow && dets)
-                    check(hipMemcpy2DAsync(dets + s0 * dets_stride, dets_stride, pd, pd_stride, drow, n,
-                                           hipMemcpyDeviceToHost, stream), "copy dets");
-                if (want_obs && orow)
-                    check(hipMemcpy2DAsync(obs + s0 * obs_stride, obs_stride, po, po_stride, orow, n,
-                                           hipMemcpyDeviceToHost, stream), "copy obs");
-                check(hipStreamSynchronize(stream), "sample");
-                if (newline) {
-                    for (uint64_t i = 0; i < n; i++) dets[(s0 + i) * dets_stride + L.ncols] = '\n';
+                if (dets && (drow || newline)) {
+                    parts.push_back({pd, pd_stride, drow, dets + s0 * dets_stride, dets_stride,
+                                     newline ? (long)L.ncols : -1L});
                 }
+                if (want_obs && orow) parts.push_back({po, po_stride, orow, obs + s0 * obs_stride, obs_stride, -1L});
+                pipe.submit(parts, n);
             }
         }
+        if (piped) pipe.finish();
         check(hipStreamSynchronize(stream), "sample");
         tiles_used += tiles_total;
     }

@@ -405,10 +405,10 @@ int mfs_server_copy_result(mfs_server *v, uint64_t id, uint8_t *dets, size_t det
         auto &r = v->get(id);
         if (r.state != mfs_server::DONE) throw std::invalid_argument("request has not finished");
         if (dets && r.det_row)
-            check(hipMemcpy2DAsync(dets, dets_stride, r.dets, r.det_row, r.det_row, r.shots, hipMemcpyDeviceToHost,
+            check(copy_rows_async(dets, dets_stride, r.dets, r.det_row, r.det_row, r.shots, hipMemcpyDeviceToHost,
                                    v->stream), "copy result");
         if (obs && r.obs_row)
-            check(hipMemcpy2DAsync(obs, obs_stride, r.obs, r.obs_row, r.obs_row, r.shots, hipMemcpyDeviceToHost,
+            check(copy_rows_async(obs, obs_stride, r.obs, r.obs_row, r.obs_row, r.shots, hipMemcpyDeviceToHost,
                                    v->stream), "copy result");
         check(hipStreamSynchronize(v->stream), "copy result");
     });

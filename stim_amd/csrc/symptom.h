@@ -136,17 +136,22 @@ struct SymDevParams {
     uint32_t *err_words;
     uint64_t err_stride_words;
     const uint32_t *site_mech;
-    // Outcome records of the specialised scan kernel (null when it does not apply): one 8-byte record
-    // per (site, outcome code) holding the outcome's whole symptom, the XOR of its basis columns with
-    // entries that cancel removed. Member k of group g finds the record of (group site s, code) at
-    // orec_base[4 g + k] + s * nout + code - 1 (32-bit wrapping arithmetic).
+    // Outcome records of the specialised scan kernel (null when it does not apply). A site's outcomes
+    // (code 1..nout) have symptoms: the XOR of the basis columns each code selects, with entries that
+    // cancel removed. Written relative to the site's lowest detector b, these symptoms repeat from
+    // site to site (every round of a memory circuit repeats the one before, shifted), so each distinct
+    // relative block of nout records is stored once. orec_sites holds, per site, (index of its block's
+    // first record in orecs, b); member k of group g finds group site s's entry at orec_base[4 g + k] + s
+    // (32-bit wrapping arithmetic), and the hit's detectors are b + the record's entries.
     const uint2 *orecs;
     const uint32_t *orec_base;
-    const uint32_t *orec_ovf;      // overflow lists: count, then entries (detector m, observable D + k)
+    const uint2 *orec_sites;
+    const uint32_t *orec_ovf;      // overflow lists: count, then entries (detector b + m, observable D + k)
 };
 // Outcome record (8 bytes): x = e0 | e1 << 14 | n << 28 (n = 0..4 detector entries) | obs0 << 31,
-// y = e2 | e3 << 14 | obs1 << 28; unused entries are 0. A symptom with more than four detectors has
-// n = 0, y = SYM_OREC_OVERFLOW | offset of its list in orec_ovf.
+// y = e2 | e3 << 14 | obs1 << 28; entries are relative to the site's base detector, unused entries
+// are 0. A symptom with more than four detectors has n = 0, y = SYM_OREC_OVERFLOW | offset of its list
+// in orec_ovf.
 constexpr uint32_t SYM_OREC_OVERFLOW = 1u << 31;
 
 }  // namespace sa

@@ -553,6 +553,56 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
                    P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
 }
 
+// Applies up to K hits of one lane (the specialised scan kernel's pass 2 and 3): loads the K hits'
+// site entries together, then their K outcome records together (entries past the round's hits name
+// the group's first site and are skipped), so that the compiler can wait for each load in turn; then
+// flips each hit's detector bits without branching per entry (an absent entry XORs 0 into its word)
+// and its observable flags. site[k] = the hit's site entry index, code[k] = its outcome code - 1.
+template <bool OBS, uint32_t K>
+__device__ __forceinline__ void apply_outcomes(const SymDevParams &P, const Sink &sink, const uint32_t *site,
+                                               const uint32_t *code, const uint32_t *shot, uint32_t S) {
+    uint2 se[K], rec[K];
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) se[k] = P.orec_sites[site[k]];
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) rec[k] = P.orecs[se[k].x + code[k]];
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) {
+        if (shot[k] < S) {
+            // Shot and row-bit counts fit 24 bits (S <= 8192, rows below 64 KB); the site's base
+            // detector is below D, so every entry's bit lies in the shot's row.
+            const uint32_t dbase = __umul24(shot[k], sink.det_bits) + sink.det_col0 + se[k].y;
+            const uint2 r = rec[k];
+            const uint32_t n = (r.x >> 28) & 7u;
+            // An overflow record holds its list offset where entries 2 and 3 would be (and n = 0): read
+            // those entries as 0 so that every address stays inside the tile's image.
+            const uint32_t ry = (r.y & SYM_OREC_OVERFLOW) ? 0u : r.y;
+            const uint32_t f[4] = {r.x & 0x3FFFu, (r.x >> 14) & 0x3FFFu, ry & 0x3FFFu, (ry >> 14) & 0x3FFFu};
+#pragma unroll
+            for (uint32_t e = 0; e < 4; e++) {
+                const uint32_t b = dbase + f[e];
+                atomicXor(&sink.img[b >> 5], e < n ? 1u << (b & 31) : 0u);
+            }
+            if (OBS && ((r.x >> 31) | ((r.y >> 28) & 1u))) {
+                const uint32_t obase = sink.obs_base + __umul24(shot[k], sink.obs_bits) + sink.obs_col0;
+                if (r.x >> 31) xor_bit(sink, obase);
+                if ((r.y >> 28) & 1u) xor_bit(sink, obase + 1);
+            }
+            if (r.y & SYM_OREC_OVERFLOW) {
+                const uint32_t obase = sink.obs_base + __umul24(shot[k], sink.obs_bits) + sink.obs_col0;
+                // More than four detectors: the list (detector m, observable D + k).
+                const uint32_t *list = P.orec_ovf + (r.y & ~SYM_OREC_OVERFLOW);
+                const uint32_t cnt = list[0];
+                for (uint32_t e = 0; e < cnt; e++) {
+                    const uint32_t v = list[1 + e];
+                    if (v < sink.D) xor_bit(sink, dbase + v);
+                    else if (OBS) xor_bit(sink, obase + (v - sink.D));
+                }
+            }
+        }
+    }
+}
+
 // Specialised scan tile for models whose groups are plain (compact records, no thinning) and whose
 // members draw outcomes uniformly over codes 1..nout: the QEC memory circuits. It draws exactly the
 // hits scan_tile draws (same counters, gaps, positions and outcome codes), so its rows are
@@ -614,12 +664,13 @@ __device__ __forceinline__ void scan_tile_fast(const SymDevParams &P, uint32_t b
             const uint2 sc = block_scan(sum, scratch);
             uint32_t pos = sat_add(base, sc.x);
             pos = pos < N ? pos : N;
-            // Pass 1: record index and shot of every hit (shot S marks no hit).
-            uint32_t idx[SCAN_M_MAX], shot[SCAN_M_MAX];
+            // Pass 1: site entry, outcome and shot of every hit (shot S marks no hit).
+            uint32_t idx[SCAN_M_MAX], shot[SCAN_M_MAX], code[SCAN_M_MAX];
 #pragma unroll
             for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
-                idx[k] = b0;
+                idx[k] = b0 + G.bound[0];
                 shot[k] = S;
+                code[k] = 0;
                 if (k < m) {
                     pos += g[k] < N ? g[k] : N;
                     pos = pos < N ? pos : N;
@@ -629,48 +680,17 @@ __device__ __forceinline__ void scan_tile_fast(const SymDevParams &P, uint32_t b
                     const uint32_t bb = m3 ? b3 : m2 ? b2 : m1 ? b1 : b0;
                     const uint32_t nout = nf & 0xFFFFu;
                     if (pos < N) {
-                        idx[k] = bb + srel * nout + __umulhi(oc[k], nout);
+                        idx[k] = bb + srel;
+                        code[k] = __umulhi(oc[k], nout);
                         shot[k] = pos & (S - 1);
                     }
                     pos += 1;
                 }
             }
-            // Pass 2: every record load of the round in flight together.
-            uint2 rec[SCAN_M_MAX];
-#pragma unroll
-            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
-                if (k < m) rec[k] = P.orecs[idx[k]];
-            }
-            // Pass 3: flip the outcome's detector bits and observable flags.
-#pragma unroll
-            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
-                if (k < m && shot[k] < S) {
-                    // Shot and row-bit counts fit 24 bits (S <= 8192, rows below 64 KB).
-                    const uint32_t dbase = __umul24(shot[k], sink.det_bits) + sink.det_col0;
-                    const uint2 r = rec[k];
-                    const uint32_t n = (r.x >> 28) & 7u;
-                    if (n >= 1) xor_bit(sink, dbase + (r.x & 0x3FFFu));
-                    if (n >= 2) xor_bit(sink, dbase + ((r.x >> 14) & 0x3FFFu));
-                    if (n >= 3) xor_bit(sink, dbase + (r.y & 0x3FFFu));
-                    if (n >= 4) xor_bit(sink, dbase + ((r.y >> 14) & 0x3FFFu));
-                    if (OBS && ((r.x >> 31) | ((r.y >> 28) & 1u))) {
-                        const uint32_t obase = sink.obs_base + __umul24(shot[k], sink.obs_bits) + sink.obs_col0;
-                        if (r.x >> 31) xor_bit(sink, obase);
-                        if ((r.y >> 28) & 1u) xor_bit(sink, obase + 1);
-                    }
-                    if (r.y & SYM_OREC_OVERFLOW) {
-                        const uint32_t obase = sink.obs_base + __umul24(shot[k], sink.obs_bits) + sink.obs_col0;
-                        // More than four detectors: the list (detector m, observable D + k).
-                        const uint32_t *list = P.orec_ovf + (r.y & ~SYM_OREC_OVERFLOW);
-                        const uint32_t cnt = list[0];
-                        for (uint32_t e = 0; e < cnt; e++) {
-                            const uint32_t v = list[1 + e];
-                            if (v < sink.D) xor_bit(sink, dbase + v);
-                            else if (OBS) xor_bit(sink, obase + (v - sink.D));
-                        }
-                    }
-                }
-            }
+            // Pass 2 and 3: record loads in flight together, then the flips (m is uniform).
+            static_assert(SCAN_M_MAX == 8, "two apply widths");
+            if (m > 4) apply_outcomes<OBS, 8>(P, sink, idx, code, shot, S);
+            else apply_outcomes<OBS, 4>(P, sink, idx, code, shot, S);
             base = sat_add(base, sc.y);
             base = base < N ? base : N;
         }

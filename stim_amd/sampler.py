@@ -302,11 +302,12 @@ class CompiledDetectorSampler:
         N.check(N.lib().mfs_set_stream_position(self._h, int(tiles)))
 
     def engine_info(self):
-        out = (ctypes.c_uint64 * 8)()
+        out = (ctypes.c_uint64 * 10)()
         N.check(N.lib().mfs_engine_info(self._h, out))
         v = list(out)
         return {'engine': {1: 'frame', 2: 'symptom'}[v[0]], 'classes': v[1], 'columns': v[2], 'entries': v[3],
-                'build_ms': v[4] / 1000.0, 'sites': v[5], 'tile_shots': v[6], 'lds_bytes_per_tile': v[7]}
+                'build_ms': v[4] / 1000.0, 'sites': v[5], 'tile_shots': v[6], 'lds_bytes_per_tile': v[7],
+                'outcome_blocks': v[8], 'outcome_records': v[9]}
 
     def bench_device(self, shots: int, warmup: int = 2, iters: int = 5, split: bool = False):
         """Average device time (ms) of one sample of `shots` into HBM-resident packed outputs.

@@ -476,3 +476,14 @@ def test_specialised_scan_kernel_equals_generic(golden, name):
             else:
                 np.testing.assert_array_equal(ra, rb, err_msg=f'{name} {kw} packed={packed}')
             assert (ra[0] if isinstance(ra, tuple) else ra).any()
+
+
+def test_outcome_blocks_repeat_across_rounds(golden):
+    """The specialised kernel stores each distinct block of a site's outcome records (relative to the
+    site's lowest detector) once: the d=25, 25-round memory circuit's 108,674 sites and 979,874
+    outcomes reduce to the blocks of a few rounds, small enough to stay in L2."""
+    s = _sampler(golden.circuit('surface_d25_r25'), engine='symptom')
+    info = s.engine_info()
+    assert info['sites'] == 108674
+    assert 0 < info['outcome_records'] < 979874 // 5, info
+    assert info['outcome_blocks'] < info['sites'] // 5, info
