@@ -34,7 +34,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 CIRCUIT = os.path.join(ROOT, 'tests', 'golden', 'circuits', 'surface_d25_r25.stim')
 REF_STIM = os.path.join(ROOT, 'oracle', '_ref', 'stim')
-TRAFFIC = os.path.join(ROOT, 'profiles', 'sym_scan_kernel_d25_r25_2p20_traffic.md')
+TRAFFIC = os.path.join(ROOT, 'profiles', '{kernel}_d25_r25_2p20_traffic.md')
 DEM_D25 = os.path.join(ROOT, 'tests', 'golden', 'dems', 'surface_d25_r25_folded.dem.gz')
 
 
@@ -101,11 +101,11 @@ def cpu_baseline(circuit_path):
             'sample': f'numpy oracle, 1 process x {shots} shots, wall {wall:.2f}s'}
 
 
-def measured_traffic():
+def measured_traffic(kernel):
     """HBM bytes per launch of the headline kernel, from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE
-    summary (scaled by the 1 GiB copy calibration recorded there)."""
+    summary of that kernel (scaled by the 1 GiB copy calibration recorded there)."""
     try:
-        text = open(TRAFFIC).read()
+        text = open(TRAFFIC.format(kernel=kernel)).read()
     except OSError:
         return None
     m = re.search(r'^traffic_bytes_per_launch:\s*(\d+)', text, re.M)
@@ -177,7 +177,10 @@ def headline(args):
         engines['symptom_generic_scan'] = {'shots_per_s': S / (gen_ms * 1e-3), 'kernel_ms': gen_ms}
     b_alg = S * (drow + orow)
     achieved = b_alg / (kernel_ms * 1e-3) / 1e9
-    traffic = measured_traffic() if S == 1 << 20 and engine['engine'] == 'symptom' else None
+    # The specialised scan kernel runs where the outcome table applies (outcome records built).
+    kname = ('sym_scan_fast_kernel' if engine.get('outcome_records') else 'sym_scan_kernel') \
+        if engine['engine'] == 'symptom' else 'frame_sim_kernel'
+    traffic = measured_traffic(kname) if S == 1 << 20 and engine['engine'] == 'symptom' else None
     cpu = None if args.no_cpu else cpu_baseline(CIRCUIT)
     if engine['engine'] == 'symptom':
         tile = {'tile_shots': engine['tile_shots'], 'lds_bytes_per_tile': engine['lds_bytes_per_tile']}
@@ -202,8 +205,7 @@ def headline(args):
                         'shots_per_step': S, 'detectors': D, 'observables': O, 'parallelism': 'single GPU'}, **tile),
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
                      'frac': achieved / HBM_PEAK_GBS, 'traffic': traffic,
-                     'kernel': 'sym_scan_kernel' if engine['engine'] == 'symptom' else 'frame_sim_kernel',
-                     'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
+                     'kernel': kname, 'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
         'engine': engine,
         'engines': engines,
         'cpu_baseline': cpu,
@@ -586,7 +588,8 @@ def _timed(step, args):
 
 
 def measure(args):
-    """Measurement sampler (every measurement result) on the d=25, 25-round circuit."""
+    """Measurement sampler (every measurement result) on the d=25, 25-round circuit: the expansion path
+    (detection events on the symptom engine, expanded into records) and, for comparison, the frame engine."""
     import torch
     import stim_amd
     from stim_amd import _native as N
@@ -606,27 +609,45 @@ def measure(args):
         N.check(L.mfs_sample(sampler._h, S, flags, out.data_ptr(), row, None, 0))
 
     wall = _timed(step, args)
-    kernel_ms = sampler.bench_device(S, warmup=3, iters=max(5, min(args.steps, 20)))
+    iters = max(5, min(args.steps, 20))
+    kernel_ms, last_ms = sampler.bench_device(S, warmup=3, iters=iters, split=True)
     info = sampler.info()
     engine = sampler.engine_info()
-    b_alg = S * row
-    achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+    engines = {engine['engine']: {'shots_per_s': args.steps * S / wall, 'kernel_ms': kernel_ms}}
+    if engine['engine'] == 'expansion':
+        frame = circuit.compile_sampler(seed=2027)
+        frame.set_engine('frame')
+        fms = frame.bench_device(S, warmup=2, iters=max(3, min(args.steps, 5)))
+        engines['frame'] = {'shots_per_s': S / (fms * 1e-3), 'kernel_ms': fms}
+        del frame
+        # The expansion kernel reads the padded detection-event rows and writes the records.
+        ypitch = ((circuit.num_detectors + circuit.num_observables + 7) // 8 + 15) // 16 * 16
+        b_alg = S * (ypitch + row)
+        achieved = b_alg / (last_ms * 1e-3) / 1e9
+        roof = {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'kernel': 'expand_kernel', 'kernel_ms': last_ms,
+                'algorithmic_bytes_per_launch': b_alg,
+                'symptom_kernel_ms': kernel_ms - last_ms}
+        cfg = {'engine': 'expansion', 'symptom_columns': engine['columns'], 'free_results': engine['free_results'],
+               'substitution_levels': engine['levels'], 'substitution_terms': engine['terms']}
+    else:
+        b_alg = S * row
+        achieved = b_alg / (kernel_ms * 1e-3) / 1e9
+        roof = {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'kernel': 'frame_sim_kernel', 'kernel_ms': kernel_ms,
+                'algorithmic_bytes_per_launch': b_alg}
+        cfg = {'engine': 'frame', 'tile_shots': 32 * info['tile_words'], 'lds_bytes_per_tile': info['lds_bytes']}
     return {
         'metric': 'measurement shots/s @ d=25 r=25 p=1e-3 (all 16,225 results per shot)',
         'value': args.steps * S / wall, 'unit': 'shots/s', 'n_gpus': 1, 'steps': args.steps, 'warmup': args.warmup,
         'ms_per_step': wall / args.steps * 1e3, 'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'u32',
         'data': 'synthetic: rotated surface code memory-Z d=25 r=25 p=1e-3 (stim gen text), seeded Philox noise and '
-                'frame randomization',
-        'config': {'workload': 'measure_d25_r25', 'measurements': M, 'shots_per_step': S,
-                   'engine': engine['engine'], 'symptom_columns': engine['columns'],
-                   'tile_shots': engine['tile_shots'] if engine['engine'] == 'symptom' else 32 * info['tile_words'],
-                   'lds_bytes_per_tile': engine['lds_bytes_per_tile'] if engine['engine'] == 'symptom'
-                   else info['lds_bytes'], 'parallelism': 'single GPU'},
-        'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None,
-                     'kernel': 'sym_scan_kernel' if engine['engine'] == 'symptom' else 'frame_sim_kernel',
-                     'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
+                'uniform random results',
+        'config': dict({'workload': 'measure_d25_r25', 'measurements': M, 'shots_per_step': S,
+                        'parallelism': 'single GPU'}, **cfg),
+        'roofline': roof,
+        'engines': engines,
         'reference_sample_and_compile_s': compile_s,
         'cpu_baseline': None if args.no_cpu else measure_cpu_baseline(),
     }

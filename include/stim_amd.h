@@ -115,6 +115,20 @@ int mfs_m2d_bench(mfs_m2d *c, uint64_t shots, int warmup, int iters, double out_
  * TableauSimulator::reference_sample_circuit (src/stim/simulators/tableau_simulator.inl:1435-1438). */
 int mfs_reference_sample(const char *stim_text, size_t len, uint8_t *out, size_t out_len);
 
+/* Host only, no GPU (no reference counterpart; the measurement sampler's expansion path, which stands
+ * in for the frame simulation of sample_batch_measurements, frame_simulator_util.inl:290-372):
+ * out[0..7] = whether the expansion applies (1) or not (0), measurements, pivots (independent
+ * detector/observable rows), unconstrained results, random results and random collapses of the
+ * reference run, substitution levels, substitution terms. */
+int mfs_expand_plan_stats(const char *stim_text, size_t len, uint64_t out[8]);
+
+/* Host only, a test hook (no reference counterpart): the expansion of `shots` shots computed on the
+ * host as the kernel computes it, one byte per bit: y rows hold the detector then observable values,
+ * free rows one value per unconstrained result (in record order), out rows the measurement results
+ * before reference inversion. Fails when the expansion does not apply. */
+int mfs_expand_shots_host(const char *stim_text, size_t len, uint64_t shots, const uint8_t *y, size_t y_stride,
+                          const uint8_t *free_bits, size_t free_stride, uint8_t *out, size_t out_stride);
+
 /* Samples detection events. Row strides are in bytes. dets must hold shots rows of
  * (packed ? ceil(n/8) : n) bytes where n = D + O*(prepend+append); obs (when
  * MFS_SEPARATE_OBSERVABLES) holds shots rows of (packed ? ceil(O/8) : O) bytes.
@@ -161,14 +175,20 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
  * simulation, FrameSimulator::do_circuit, frame_simulator.inl:165-170), 2 = symptom engine
  * (frame-kernel symptoms of every basis fault + sparse noise sampling; same distribution,
  * deterministic-detector circuits only). Automatic picks the symptom engine when it applies.
+ * For a measurement sampler, 2 = expansion: detection events and observable flips drawn on the
+ * symptom engine, expanded into measurement records by forward substitution with the unconstrained
+ * (random) results drawn uniformly; same distribution, for circuits whose detectors and observables
+ * fix every parity the noiseless circuit fixes and whose random collapses are all recorded.
  * No reference counterpart (the reference has one engine). */
 int mfs_set_engine(mfs_sampler *s, int engine);
 
-/* out[0..9] = engine in use for sampling (1 frame, 2 symptom), site classes, symptom columns,
- * symptom entries, symptom-table build time in microseconds, noise sites, for the symptom engine the
- * shots per workgroup tile and the sampling kernel's LDS bytes per tile (bit-packed output with
- * separate observables), and the specialised scan kernel's distinct outcome blocks and outcome
- * records (0 when it does not apply). Builds the symptom table if needed. */
+/* out[0..9] = engine in use for sampling (1 frame, 2 symptom, 3 expansion), site classes, symptom
+ * columns, symptom entries, symptom-table build time in microseconds, noise sites, for the symptom
+ * engine the shots per workgroup tile and the sampling kernel's LDS bytes per tile (bit-packed output
+ * with separate observables), and the specialised scan kernel's distinct outcome blocks and outcome
+ * records (0 when it does not apply). A measurement sampler on the expansion path (3) reports its
+ * detector sampler's symptom figures in out[1..5] and, in out[6..8], its free (random) results,
+ * substitution levels and substitution terms. Builds the symptom table or expansion if needed. */
 int mfs_engine_info(mfs_sampler *s, uint64_t out[10]);
 
 /* Symptom engine: target size (KB) of one workgroup's LDS image of output rows (sets the shots per

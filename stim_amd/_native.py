@@ -33,6 +33,7 @@ EXPORTED = [
     'mfs_m2d_compile', 'mfs_m2d_destroy', 'mfs_m2d_stats', 'mfs_m2d_convert', 'mfs_m2d_bench',
     'mfs_set_stream_position', 'mfs_get_stream_position', 'mfs_rng_counter', 'mfs_philox', 'mfs_build_id',
     'mfs_sample_noalloc', 'mfs_m2d_convert_sweep', 'mfs_dem_sample_errors', 'mfs_set_scan_variant',
+    'mfs_expand_plan_stats', 'mfs_expand_shots_host',
 ]
 
 _lib = None
@@ -60,6 +61,8 @@ def lib():
     L.mfs_circuit_stats.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_parse_stats.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(u64)]
     L.mfs_reference_sample.argtypes = [ctypes.c_char_p, sz, p, sz]
+    L.mfs_expand_plan_stats.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(u64)]
+    L.mfs_expand_shots_host.argtypes = [ctypes.c_char_p, sz, u64, p, sz, p, sz, p, sz]
     L.mfs_m2d_compile.argtypes = [ctypes.c_char_p, sz, ctypes.c_int, ctypes.POINTER(p)]
     L.mfs_m2d_destroy.argtypes = [p]
     L.mfs_m2d_destroy.restype = None

@@ -16,11 +16,12 @@ $HIPCC $DEVFLAGS -c "$HERE/frame_kernel.hip" -o "$OBJ/frame_kernel.o" & pids+=($
 $HIPCC $DEVFLAGS -c "$HERE/symptom_kernel.hip" -o "$OBJ/symptom_kernel.o" & pids+=($!)
 $HIPCC $DEVFLAGS -c "$HERE/m2d_kernel.hip" -o "$OBJ/m2d_kernel.o" & pids+=($!)
 $HIPCC $DEVFLAGS -c "$HERE/noalloc.hip" -o "$OBJ/noalloc.o" & pids+=($!)
-for f in circuit compiler symptom dem tableau sampler batch server m2d; do
+for f in circuit compiler symptom dem tableau expand sampler batch server m2d; do
   $HIPCC $HOSTFLAGS -D__HIP_PLATFORM_AMD__ -DSTIM_AMD_BUILD_ID="\"$BUILD_ID\"" -x c++ -c "$HERE/$f.cc" -o "$OBJ/$f.o" & pids+=($!)
 done
 for p in "${pids[@]}"; do wait "$p"; done
 $HIPCC --offload-arch=gfx950 -shared -fPIC -o "$OUT.tmp" "$OBJ/frame_kernel.o" "$OBJ/symptom_kernel.o" "$OBJ/m2d_kernel.o" "$OBJ/noalloc.o" "$OBJ/circuit.o" "$OBJ/compiler.o" \
-    "$OBJ/symptom.o" "$OBJ/dem.o" "$OBJ/tableau.o" "$OBJ/sampler.o" "$OBJ/batch.o" "$OBJ/server.o" "$OBJ/m2d.o"
+    "$OBJ/symptom.o" "$OBJ/dem.o" "$OBJ/tableau.o" "$OBJ/expand.o" "$OBJ/sampler.o" "$OBJ/batch.o" "$OBJ/server.o" \
+    "$OBJ/m2d.o"
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT ($BUILD_ID)"

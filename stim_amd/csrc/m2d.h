@@ -38,6 +38,26 @@ struct M2dParams {
     uint32_t sweep_blocks;      // ceil(K / 32): LDS word m2d_sweep_word(blocks) + k holds sweep bit k
 };
 
+// Expansion of detection events into measurement records (the measurement sampler's fast path; plan
+// in expand.h). One workgroup expands a tile of 32 shots: its measurement words start at zero, free
+// results get random words, each y column's word is XORed into its contribution slots, substitution
+// levels run in order, and the words leave through the identity columns of `out` (column m = word m,
+// inverted where the reference sample is 1).
+struct ExpandDev {
+    const uint8_t *in;          // y rows: detection events then observable flips, bit packed; 16-byte
+    uint64_t in_stride;         // aligned rows with a stride that is a multiple of 16
+    uint32_t in_groups;         // 16-byte groups of a y row that hold its Y bits
+    uint32_t Y;
+    const uint32_t *contrib_off, *contrib;
+    const uint32_t *free_slots;
+    uint32_t nfree;
+    uint32_t nlevels;
+    const uint32_t *level_off, *item_slot, *term_off, *terms;
+    uint64_t seed;              // Philox key of the free results
+    uint64_t tile0;             // 32-shot tile of the launch's first shot in the sampler's stream
+    M2dParams out;              // output: M identity columns over the tile's measurement words
+};
+
 // Column record: bits 28..29 of x = term count class c, bit 31 = the column's reference parity
 // (invert). c = 0..2: the column has c terms; x bits 0..27 = first measurement index, y = second
 // (unused fields hold 0). c = 3: x bits 0..27 = term count n, y = offset of the term list in the
@@ -58,5 +78,8 @@ __host__ __device__ inline uint32_t m2d_sweep_word(uint32_t blocks) {
 __host__ __device__ inline uint32_t m2d_zero_word(uint32_t blocks, uint32_t sweep_blocks) {
     return m2d_sweep_word(blocks) + sweep_blocks * 32;
 }
+
+// Dynamic LDS of a conversion or expansion launch: the tile's words plus the staged output chunk.
+size_t m2d_lds_bytes(uint32_t blocks, uint32_t sweep_blocks);
 
 }  // namespace sa

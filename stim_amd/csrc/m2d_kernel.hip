@@ -174,9 +174,88 @@ __global__ __launch_bounds__(NT) void m2d_kernel(M2dParams P) {
     if (P.n_obs) emit_columns(P, words, img, P.obs, P.obs_stride, P.n_obs, P.num_det, half, lane, s0, valid);
 }
 
+// Random words of the free results: Philox4x32-10 keyed by the sampler's expansion key, counter (group of
+// four free results, 64-bit tile index, stream tag).
+constexpr uint32_t TAG_EXPAND = 7u << 28;
+
+__global__ __launch_bounds__(NT) void expand_kernel(ExpandDev E) {
+    extern __shared__ __align__(16) uint32_t words[];
+    const M2dParams &P = E.out;
+    const uint32_t tid = threadIdx.x, half = tid >> 5, lane = tid & 31;
+    const uint64_t s0 = (uint64_t)blockIdx.x * 32;
+    const uint64_t left = P.shots - s0;
+    const uint32_t valid = left < 32 ? (uint32_t)left : 32u;
+    const bool have = lane < valid;
+    const uint32_t zero = m2d_zero_word(P.blocks, 0);
+    for (uint32_t i = tid; i < zero + 4; i += NT) words[i] = 0;
+    __syncthreads();
+    // Free results (distinct from every contribution slot, so no ordering with the loads below).
+    const uint64_t tile = E.tile0 + blockIdx.x;
+    const uint32_t k0 = (uint32_t)E.seed, k1 = (uint32_t)(E.seed >> 32);
+    for (uint32_t g = tid; g < (E.nfree + 3) / 4; g += NT) {
+        const uint4 r = philox10(make_uint4(g, (uint32_t)tile, (uint32_t)(tile >> 32), TAG_EXPAND), k0, k1);
+        const uint32_t rs[4] = {r.x, r.y, r.z, r.w};
+#pragma unroll
+        for (uint32_t k = 0; k < 4; k++) {
+            if (4 * g + k < E.nfree) words[E.free_slots[4 * g + k]] = rs[k];
+        }
+    }
+    // y words, transposed across half-waves like the measurement words of m2d_kernel, XORed into their
+    // slots. Lanes past the tile's shots read the tile's first row: those bits of every word belong to
+    // shots that are never written.
+    const uint8_t *row = E.in + (s0 + (have ? lane : 0)) * E.in_stride;
+    for (uint32_t g0 = half * UNROLL; g0 < E.in_groups; g0 += HALVES * UNROLL) {
+        uint4 v[UNROLL];
+#pragma unroll
+        for (uint32_t u = 0; u < UNROLL; u++) {
+            const uint32_t g = g0 + u < E.in_groups ? g0 + u : E.in_groups - 1;
+            v[u] = *reinterpret_cast<const uint4 *>(row + 16 * (uint64_t)g);
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < UNROLL; u++) {
+            if (g0 + u < E.in_groups) {
+                const uint32_t xs[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                for (uint32_t k = 0; k < 4; k++) {
+                    const uint32_t x = transpose32(xs[k], lane);
+                    const uint32_t y = (4 * (g0 + u) + k) * 32 + lane;
+                    if (y < E.Y) {
+                        for (uint32_t c = E.contrib_off[y]; c < E.contrib_off[y + 1]; c++) atomicXor(&words[E.contrib[c]], x);
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // Substitution: items of one level read only results of lower levels.
+    for (uint32_t l = 0; l < E.nlevels; l++) {
+        const uint32_t a = E.level_off[l], b = E.level_off[l + 1];
+        for (uint32_t i = a + tid; i < b; i += NT) {
+            const uint32_t k = E.item_slot[i];
+            uint32_t v = words[k];
+            for (uint32_t t = E.term_off[i]; t < E.term_off[i + 1]; t++) v ^= words[E.terms[t]];
+            words[k] = v;
+        }
+        __syncthreads();
+    }
+    uint8_t *const img = reinterpret_cast<uint8_t *>(words + zero + 4);
+    emit_columns(P, words, img, P.out, P.out_stride, P.n_out, 0, half, lane, s0, valid);
+}
+
 }  // namespace
 
 namespace sa {
+
+hipError_t launch_expand(const ExpandDev &E, hipStream_t stream) {
+    const uint64_t tiles = (E.out.shots + 31) / 32;
+    if (tiles == 0) return hipSuccess;
+    const size_t lds = m2d_lds_bytes(E.out.blocks, 0);
+    static size_t reserved = 0;
+    const hipError_t e = reserve_lds(reinterpret_cast<const void *>(&expand_kernel), lds, reserved);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(expand_kernel, dim3((uint32_t)tiles), dim3(NT), lds, stream, E);
+    return hipGetLastError();
+}
 
 // LDS of a launch: the tile's measurement, sweep and zero words plus the staged output chunk.
 size_t m2d_lds_bytes(uint32_t blocks, uint32_t sweep_blocks) {

@@ -200,7 +200,8 @@ int mfs_compile_measurements(const char *stim_text, size_t len, uint64_t seed, i
         }
         check(hipStreamCreate(&s->stream), "stream");
         s->meas_mode = true;
-        s->engine_mode = 1;
+        s->engine_mode = 0;
+        s->invert_words = words;
         s->pick_tile_words(0);
         s->upload_program();
         s->probe_randomization();
@@ -279,6 +280,44 @@ int mfs_parse_stats(const char *stim_text, size_t len, uint64_t out[10]) {
         out[7] = prog.num_noise_ops;
         out[8] = prog.ring;
         out[9] = lds_bytes(prog, 2);
+    });
+}
+
+int mfs_expand_plan_stats(const char *stim_text, size_t len, uint64_t out[8]) {
+    return guarded([&] {
+        std::string text(stim_text ? stim_text : "", stim_text ? len : 0);
+        Block b = parse_circuit(text.data(), text.size());
+        std::vector<const Instr *> flat = flatten(b);
+        const Stats st = compute_stats(flat);
+        RefStats rs;
+        (void)reference_sample(flat, &rs);
+        const ExpandPlan E = plan_expansion(flat, (uint32_t)st.num_measurements, (uint32_t)st.num_detectors,
+                                            (uint32_t)st.num_observables, rs.random_recorded, rs.random_total);
+        out[0] = E.complete ? 1 : 0;
+        out[1] = st.num_measurements;
+        out[2] = E.pivots;
+        out[3] = E.complete ? E.free_slots.size() : (uint64_t)st.num_measurements - E.pivots;
+        out[4] = rs.random_recorded;
+        out[5] = rs.random_total;
+        out[6] = E.level_off.empty() ? 0 : E.level_off.size() - 1;
+        out[7] = E.terms.size();
+    });
+}
+
+int mfs_expand_shots_host(const char *stim_text, size_t len, uint64_t shots, const uint8_t *y, size_t y_stride,
+                          const uint8_t *free_bits, size_t free_stride, uint8_t *out, size_t out_stride) {
+    return guarded([&] {
+        std::string text(stim_text ? stim_text : "", stim_text ? len : 0);
+        Block b = parse_circuit(text.data(), text.size());
+        std::vector<const Instr *> flat = flatten(b);
+        const Stats st = compute_stats(flat);
+        RefStats rs;
+        (void)reference_sample(flat, &rs);
+        const ExpandPlan E = plan_expansion(flat, (uint32_t)st.num_measurements, (uint32_t)st.num_detectors,
+                                            (uint32_t)st.num_observables, rs.random_recorded, rs.random_total);
+        if (!E.complete) throw std::invalid_argument("expansion unavailable: " + E.why);
+        for (uint64_t s = 0; s < shots; s++)
+            expand_shot_host(E, y + s * y_stride, free_bits + s * free_stride, out + s * out_stride);
     });
 }
 
@@ -381,8 +420,36 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
             check(hipEventCreate(&s->ev1), "event");
         }
         const uint64_t tiles = (shots + s->W() - 1) / s->W();
-        const bool sym = s->use_symptoms(false, shots);
+        const bool expand = s->meas_mode && s->use_expansion(false, shots);
+        const bool sym = !expand && s->use_symptoms(false, shots);
+        // Expansion: detection events into padded y rows, then the expansion kernel (timed alone in
+        // out_ms[1] with an event between the two).
+        DevBuf yb;
+        hipEvent_t evm = nullptr;
+        struct EvGuard {
+            hipEvent_t &e;
+            ~EvGuard() {
+                if (e) (void)hipEventDestroy(e);
+            }
+        } evm_guard{evm};
+        if (expand) {
+            yb.ensure(shots * s->det_src->row_pitch_override);
+            check(hipEventCreate(&evm), "event");
+        }
+        double expand_total = 0;
         auto one = [&]() {
+            if (expand) {
+                mfs_sampler &d = *s->det_src;
+                const mfs_sampler::Layout DL = d.layout(MFS_BIT_PACKED | MFS_APPEND_OBSERVABLES);
+                d.tiles_used += d.sym_launch(DL, shots, yb.as<uint8_t>(), d.row_pitch_override, nullptr, 0, d.tiles_used,
+                                             nullptr, nullptr);
+                check(hipEventRecord(evm, s->stream), "event");
+                check(launch_expand(s->expand_params(shots, yb.as<uint8_t>(), dd.as<uint8_t>(), L.det_row, s->expand_tiles),
+                                    s->stream),
+                      "expansion launch");
+                s->expand_tiles += (shots + 31) / 32;
+                return;
+            }
             if (sym) {
                 s->tiles_used += s->sym_launch(L, shots, dd.as<uint8_t>(), L.det_row, od.as<uint8_t>(), L.obs_row,
                                                s->tiles_used, nullptr, nullptr);
@@ -408,9 +475,14 @@ int mfs_bench_device(mfs_sampler *s, uint64_t shots, int warmup, int iters, doub
             float ms = 0;
             check(hipEventElapsedTime(&ms, s->ev0, s->ev1), "elapsed");
             kernel_total += ms;
+            if (expand) {
+                float xm = 0;
+                check(hipEventElapsedTime(&xm, evm, s->ev1), "elapsed");
+                expand_total += xm;
+            }
         }
         out_ms[0] = iters ? kernel_total / iters : 0;
-        out_ms[1] = out_ms[0];
+        out_ms[1] = expand ? (iters ? expand_total / iters : 0) : out_ms[0];
     });
 }
 
@@ -432,7 +504,10 @@ int mfs_set_engine(mfs_sampler *s, int engine) {
             return;
         }
         if (s->meas_mode) {
-            if (engine == 2) throw std::invalid_argument("the measurement sampler runs on the frame engine");
+            // 2 = expansion of detection events (the measurement sampler's symptom-engine path).
+            if (engine == 2 && !s->ensure_expansion())
+                throw std::invalid_argument("expansion unavailable: " + s->expand_why);
+            s->engine_mode = engine;
             return;
         }
         s->engine_mode = engine;
@@ -442,6 +517,21 @@ int mfs_set_engine(mfs_sampler *s, int engine) {
 
 int mfs_engine_info(mfs_sampler *s, uint64_t out[10]) {
     return guarded([&] {
+        if (s->meas_mode && s->use_expansion(false, mfs_sampler::ANY_SHOTS)) {
+            // Expansion: the symptom-engine figures are the detector sampler's.
+            mfs_sampler *d = s->det_src.get();
+            out[0] = 3;
+            out[1] = d->plan.classes.size();
+            out[2] = d->plan.columns.size();
+            out[3] = d->sym_entries;
+            out[4] = (uint64_t)(d->sym_build_ms * 1000.0);
+            out[5] = d->plan.total_sites;
+            out[6] = s->expand_free;
+            out[7] = s->expand_levels;
+            out[8] = s->expand_terms;
+            out[9] = 0;
+            return;
+        }
         const bool sym = s->use_symptoms(false);
         out[0] = sym ? 2 : 1;
         out[1] = s->plan.classes.size();

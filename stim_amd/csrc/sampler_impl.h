@@ -28,6 +28,9 @@
 #include "symptom.h"
 #include "rng.h"
 #include "host_pipe.h"
+#include "expand.h"
+#include "m2d.h"
+#include "tableau.h"
 
 namespace sa {
 hipError_t launch_frame_kernel(int T, const DevProgram &prog, const LaunchParams &P, uint32_t num_tiles, size_t lds,
@@ -48,6 +51,8 @@ hipError_t launch_sym_fill(const uint8_t *det, uint32_t det_row, const uint8_t *
 hipError_t launch_unpack_rows(const uint8_t *src, uint64_t src_stride, uint8_t *dst, uint64_t dst_stride, uint64_t rows,
                               uint32_t ncols, uint32_t base_char, hipStream_t stream);
 hipError_t launch_philox(const uint32_t *ctr, uint64_t n, uint64_t key, uint32_t *out, hipStream_t stream);
+hipError_t launch_expand(const ExpandDev &E, hipStream_t stream);
+size_t m2d_lds_bytes(uint32_t blocks, uint32_t sweep_blocks);
 }  // namespace sa
 
 using namespace sa;
@@ -150,10 +155,21 @@ struct mfs_sampler {
     uint32_t *err_words_cur = nullptr;
     uint64_t err_stride_cur = 0;
     // Measurement samplers: the outputs are the measurement results (reference sample bits XORed
-    // into the frame's flips through det_invert); they sample on the frame engine.
+    // into the frame's flips through det_invert). They sample on the frame engine, or, where the
+    // circuit's detectors and observables fix every deterministic parity, by expanding detection events
+    // drawn on the symptom engine into records (plan and semantics in expand.h).
     bool meas_mode = false;
     DevBuf d_invert;
     bool has_invert = false;
+    std::vector<uint32_t> invert_words;   // host copy of the inversion (reference sample) words
+    int expand_state = 0;                 // 0 not planned, 1 ready, -1 unavailable
+    std::string expand_why;
+    std::unique_ptr<mfs_sampler> det_src; // detector sampler of the same circuit, on this sampler's stream
+    bool borrowed_stream = false;         // the stream belongs to the sampler that owns this one
+    size_t row_pitch_override = 0;        // symptom-engine rows padded to this pitch (0: packed width)
+    uint64_t expand_key = 0, expand_tiles = 0;
+    uint32_t expand_free = 0, expand_levels = 0, expand_terms = 0;
+    DevBuf d_x_contrib_off, d_x_contrib, d_x_free, d_x_level_off, d_x_item, d_x_term_off, d_x_terms, d_x_cols, d_x_y;
     // Detector-error-model samplers: no circuit; the symptom tables come from the model's mechanisms.
     bool dem_mode = false;
     std::vector<uint32_t> dem_col;   // mechanism index (model order) -> symptom column
@@ -221,9 +237,10 @@ struct mfs_sampler {
     }
 
     ~mfs_sampler() {
+        if (det_src) det_src.reset();
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && !borrowed_stream) (void)hipStreamDestroy(stream);
     }
 
     uint32_t W() const { return 32u * (uint32_t)T; }
@@ -354,6 +371,10 @@ struct mfs_sampler {
         }
         const bool device_out = flags & MFS_DEVICE_OUTPUT;
         if (shots == 0) return;
+        if (use_expansion(faults != nullptr, shots)) {
+            sample_expand(L, shots, device_out, dets, dets_stride, base_char, newline);
+            return;
+        }
         if (use_symptoms(faults != nullptr, shots)) {
             sample_symptom(L, shots, device_out, dets, dets_stride, obs, obs_stride, fault_offsets, faults, base_char,
                            newline, errs, errs_stride);
@@ -449,6 +470,198 @@ struct mfs_sampler {
         if (piped) pipe.finish();
         check(hipStreamSynchronize(stream), "sample");
         tiles_used += (shots + W64 - 1) / W64;
+    }
+
+    // ------------------------------------------------------------------ measurement expansion
+    // Plans the measurement sampler's expansion path (semantics in expand.h) and builds its detector
+    // sampler and device tables; false (with expand_why) when it does not apply.
+    bool ensure_expansion() {
+        if (expand_state) return expand_state > 0;
+        expand_state = -1;
+        const uint32_t M = (uint32_t)prog.num_measurements;
+        if (!meas_mode || M == 0) {
+            expand_why = "no measurements to expand";
+            return false;
+        }
+        if (m2d_lds_bytes((M + 31) / 32, 0) > MAX_LDS) {
+            expand_why = "the expansion keeps a tile's measurement words in LDS: too many measurements";
+            return false;
+        }
+        RefStats rs;
+        (void)reference_sample(flat, &rs);
+        auto d = std::make_unique<mfs_sampler>();
+        d->flat = flat;
+        d->prog = compile_program(flat);
+        if (d->prog.num_measurements != M) {
+            expand_why = "record bookkeeping mismatch";
+            return false;
+        }
+        d->key = splitmix64(key ^ 0xD7EC7E5ull);
+        d->stream = stream;
+        d->borrowed_stream = true;
+        d->pick_tile_words(0);
+        d->upload_program();
+        d->probe_randomization();
+        if (d->randomize) {
+            expand_why = "the circuit has random detectors or observables";
+            return false;
+        }
+        if (!d->ensure_symptoms()) {
+            expand_why = "symptom engine unavailable: " + d->sym_why;
+            return false;
+        }
+        const uint32_t D = d->prog.num_detectors, O = d->prog.num_observables;
+        const ExpandPlan E = plan_expansion(flat, M, D, O, rs.random_recorded, rs.random_total);
+        if (!E.complete) {
+            expand_why = E.why;
+            return false;
+        }
+        // The y rows are the detector sampler's packed rows (observables appended), padded to 16 bytes.
+        d->row_pitch_override = std::max<size_t>(((D + O + 7) / 8 + 15) / 16 * 16, 16);
+        upload(d_x_contrib_off, E.contrib_off, "expansion contributions");
+        upload(d_x_contrib, E.contrib.empty() ? std::vector<uint32_t>{0} : E.contrib, "expansion contributions");
+        upload(d_x_free, E.free_slots.empty() ? std::vector<uint32_t>{0} : E.free_slots, "expansion free results");
+        upload(d_x_level_off, E.level_off, "expansion levels");
+        upload(d_x_item, E.item_slot.empty() ? std::vector<uint32_t>{0} : E.item_slot, "expansion items");
+        upload(d_x_term_off, E.term_off, "expansion terms");
+        upload(d_x_terms, E.terms.empty() ? std::vector<uint32_t>{0} : E.terms, "expansion terms");
+        // Output columns: measurement m is word m, inverted where the inversion (reference) bit is 1.
+        std::vector<uint2> cols(M);
+        for (uint32_t m = 0; m < M; m++) {
+            const bool inv = m / 32 < invert_words.size() && ((invert_words[m / 32] >> (m % 32)) & 1u);
+            cols[m] = make_uint2((1u << 28) | m | (inv ? M2D_INVERT : 0u), 0u);
+        }
+        upload(d_x_cols, cols, "expansion output columns");
+        expand_free = (uint32_t)E.free_slots.size();
+        expand_levels = (uint32_t)E.level_off.size() - 1;
+        expand_terms = (uint32_t)E.terms.size();
+        expand_key = splitmix64(key ^ 0xE8A7D5ull);
+        det_src = std::move(d);
+        expand_state = 1;
+        return true;
+    }
+
+    // Whether a measurement-sampler call expands detection events: never with fixed faults or on the
+    // frame engine setting; automatically once the detector sampler's table would cost at most two
+    // columns per shot of the call (as the symptom engine decides), or when already built.
+    bool use_expansion(bool fixed, uint64_t shots) {
+        if (!meas_mode || fixed || engine_mode == 1) return false;
+        if (engine_mode == 2) {
+            if (arena && expand_state == 0)
+                throw std::invalid_argument("the sampler's expansion is not built yet (building it allocates device "
+                                            "memory); prepare the sampler first with mfs_engine_info or one allocating call");
+            if (!ensure_expansion()) throw std::invalid_argument("expansion unavailable: " + expand_why);
+            return true;
+        }
+        if (expand_state < 0) return false;
+        if (expand_state == 0) {
+            if (shots != ANY_SHOTS) {
+                if (!plan_counted) {
+                    column_count = plan_symptoms(compile_program(flat)).columns.size();
+                    plan_counted = true;
+                }
+                if (column_count > 2 * shots) return false;
+            }
+            if (arena)
+                throw std::invalid_argument("the sampler's expansion is not built yet (building it allocates device "
+                                            "memory); prepare the sampler first with mfs_engine_info or one allocating call");
+        }
+        return ensure_expansion();
+    }
+
+    ExpandDev expand_params(uint64_t n, const uint8_t *y, uint8_t *out, size_t out_stride, uint64_t tile0) {
+        const mfs_sampler &d = *det_src;
+        const uint32_t M = (uint32_t)prog.num_measurements;
+        ExpandDev E{};
+        E.in = y;
+        E.in_stride = d.row_pitch_override;
+        E.in_groups = (uint32_t)(d.row_pitch_override / 16);
+        E.Y = d.prog.num_detectors + d.prog.num_observables;
+        E.contrib_off = d_x_contrib_off.as<uint32_t>();
+        E.contrib = d_x_contrib.as<uint32_t>();
+        E.free_slots = d_x_free.as<uint32_t>();
+        E.nfree = expand_free;
+        E.nlevels = expand_levels;
+        E.level_off = d_x_level_off.as<uint32_t>();
+        E.item_slot = d_x_item.as<uint32_t>();
+        E.term_off = d_x_term_off.as<uint32_t>();
+        E.terms = d_x_terms.as<uint32_t>();
+        E.seed = expand_key;
+        E.tile0 = tile0;
+        M2dParams &P = E.out;
+        P.blocks = (M + 31) / 32;
+        P.num_det = M;
+        P.out = out;
+        P.out_stride = out_stride;
+        P.n_out = M;
+        P.n_obs = 0;
+        P.shots = n;
+        P.cols = d_x_cols.as<uint2>();
+        P.ovf = d_x_contrib.as<uint32_t>();   // never read: every column has one term
+        return E;
+    }
+
+    // Measurement records of `shots` shots: detection events and observable flips on the detector
+    // sampler (padded y rows in device staging), then the expansion kernel into packed rows; bool or
+    // text outputs are expanded from those. Batches are whole tiles of both streams (the detector
+    // sampler's symptom tiles and the expansion's 32-shot tiles).
+    void sample_expand(const Layout &L, uint64_t shots, bool device_out, uint8_t *out, size_t out_stride,
+                       char base_char, bool newline) {
+        mfs_sampler &d = *det_src;
+        const uint32_t M = (uint32_t)prog.num_measurements;
+        const size_t ypitch = d.row_pitch_override, prow = (M + 7) / 8;
+        const Layout DL = d.layout(MFS_BIT_PACKED | MFS_APPEND_OBSERVABLES);
+        static uint8_t dummy;
+        const uint64_t S = d.sym_prepare(DL, 1, &dummy, ypitch, nullptr, 0, 0).P.shots_per_tile;
+        const uint64_t unit = std::max<uint64_t>(S, 32);
+        const bool piped = !device_out;
+        const size_t host_row = L.packed ? prow : M;
+        uint64_t batch = std::max<uint64_t>(unit, HOST_BATCH_BYTES / (ypitch + prow) / unit * unit);
+        if (piped) batch = std::max<uint64_t>(unit, PIPE_BATCH_BYTES / std::max<size_t>(host_row, 1) / unit * unit);
+        if (device_out) batch = arena_batch(batch, ypitch + (L.packed ? 0 : prow), unit);
+        batch = std::min(batch, shots);
+        PipeCall pipe_call{this, piped};
+        if (piped) {
+            const size_t dev_row = ypitch + prow + (L.packed ? 0 : M);
+            pipe.begin(stream, batch * dev_row + 6 * 256, std::max<size_t>(batch * host_row, 256));
+        }
+        const uint64_t d_tile0 = d.tiles_used, x_tile0 = expand_tiles;
+        uint64_t d_tiles = 0;
+        for (uint64_t s0 = 0; s0 < shots; s0 += batch) {
+            const uint64_t n = std::min(batch, shots - s0);
+            std::unique_ptr<SlotScope> slot;
+            if (piped) slot.reset(new SlotScope(this));
+            staging_reset();
+            uint8_t *y = staging(d_x_y, n * ypitch);
+            d_tiles += d.sym_launch(DL, n, y, ypitch, nullptr, 0, d_tile0 + d_tiles, nullptr, nullptr);
+            uint8_t *pd;
+            size_t pd_stride;
+            if (L.packed && device_out) {
+                pd = out + s0 * out_stride;
+                pd_stride = out_stride;
+            } else {
+                pd = staging(d_out_det, n * prow);
+                pd_stride = prow;
+            }
+            check(launch_expand(expand_params(n, y, pd, pd_stride, x_tile0 + s0 / 32), stream), "expansion launch");
+            if (!L.packed) {
+                uint8_t *ud = device_out ? out + s0 * out_stride : staging(d_tmp, n * std::max<size_t>(M, 1));
+                const size_t ud_stride = device_out ? out_stride : M;
+                check(launch_unpack_rows(pd, pd_stride, ud, ud_stride, n, M, (uint8_t)base_char, stream), "unpack");
+                pd = ud;
+                pd_stride = ud_stride;
+            }
+            if (piped) {
+                std::vector<HostPipe::Part> parts;
+                if (out) parts.push_back({pd, pd_stride, host_row, out + s0 * out_stride, out_stride,
+                                          newline ? (long)M : -1L});
+                pipe.submit(parts, n);
+            }
+        }
+        if (piped) pipe.finish();
+        check(hipStreamSynchronize(stream), "sample");
+        d.tiles_used += d_tiles;
+        expand_tiles += (shots + 31) / 32;
     }
 
     // ------------------------------------------------------------------ symptom engine
@@ -972,7 +1185,7 @@ struct mfs_sampler {
                           size_t obs_stride, uint64_t tile0, int tile_kb = 0, bool scan = true) {
         const uint32_t D = prog.num_detectors, O = prog.num_observables;
         const size_t obs_bytes = (L.separate && obs_dst) ? (O + 7) / 8 : 0;
-        const size_t det_bytes = (L.ncols + 7) / 8;
+        const size_t det_bytes = row_pitch_override ? row_pitch_override : (L.ncols + 7) / 8;
         const size_t row = std::max<size_t>(det_bytes + obs_bytes, 1);
         const int kb = tile_kb ? tile_kb : sym_tile_kb;
         uint32_t S = sym_tile_shots(row, kb, scan ? 8192u : 1024u);

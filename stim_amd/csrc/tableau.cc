@@ -262,6 +262,7 @@ public:
         size_t w = 0;
         while (w < nw_ && !zx[w]) w++;
         if (w == nw_) return r_[row];
+        random_++;
         const uint32_t p = (uint32_t)(64 * w + __builtin_ctzll(zx[w]));
         const Tables &T = tables();
         std::copy_n(zx, nw_, sx_.data());
@@ -272,9 +273,13 @@ public:
         return false;
     }
 
+    // Measurements so far whose result was random (collapsed onto 0).
+    uint64_t random_collapses() const { return random_; }
+
 private:
     uint32_t n_;
     size_t nw_;
+    uint64_t random_ = 0;
     std::vector<uint64_t> x_, z_;
     std::vector<uint8_t> r_;
     std::vector<uint64_t> sx_, sz_;   // new rows of the gate being applied
@@ -392,6 +397,7 @@ struct Product {
 struct Runner {
     Tableau tab;
     std::vector<uint8_t> rec;
+    uint64_t random_recorded = 0;   // random collapses whose result went into the record
     const Tables &T = tables();
 
     explicit Runner(uint32_t n) : tab(n) {}
@@ -442,7 +448,9 @@ struct Runner {
     void measure_targets(const std::vector<Target> &ts, int basis, bool record, bool reset) {
         std::vector<uint32_t> qs;
         for (const Target &t : ts) qs.push_back(t.value);
+        const uint64_t r0 = tab.random_collapses();
         collapse(qs, basis);
+        if (record) random_recorded += tab.random_collapses() - r0;
         for (const Target &t : ts) {
             const bool b = read(t.value, basis, reset);
             if (record) rec.push_back(b ^ t.inverted);
@@ -453,6 +461,7 @@ struct Runner {
     // factor's qubit with CX; the pivots are collapsed together in the collapse basis' order and read
     // in product order.
     void measure_group(const std::vector<Product> &group, int collapse_basis = 2) {
+        const uint64_t r0 = tab.random_collapses();
         std::vector<uint32_t> pivots;
         for (const Product &p : group) {
             for (const auto &t : p.terms) rotate(t.first, t.second);
@@ -468,6 +477,7 @@ struct Runner {
             for (uint32_t q : order) (void)tab.measure_z(q);
         }
         for (size_t i = 0; i < group.size(); i++) rec.push_back(tab.measure_z(pivots[i]) ^ group[i].negate);
+        random_recorded += tab.random_collapses() - r0;
         for (const Product &p : group) {
             for (size_t k = 1; k < p.terms.size(); k++) g2(Gate::CX, p.terms[k].first, p.terms[0].first);
             for (const auto &t : p.terms) rotate(t.first, t.second);
@@ -657,8 +667,9 @@ struct Runner {
 
 }  // namespace
 
-std::vector<uint8_t> reference_sample(const std::vector<const Instr *> &flat) {
+std::vector<uint8_t> reference_sample(const std::vector<const Instr *> &flat, RefStats *stats) {
     const Stats st = compute_stats(flat);
+    if (stats) *stats = RefStats{};
     if (st.num_measurements == 0) return {};
     // The tableau holds 2n rows of n X bits and n Z bits (about n^2 / 2 bytes), as the reference's
     // TableauSimulator does; circuits whose tableau would exceed the limit are refused up front rather
@@ -674,6 +685,10 @@ std::vector<uint8_t> reference_sample(const std::vector<const Instr *> &flat) {
     Runner run((uint32_t)n);
     run.rec.reserve(st.num_measurements);
     for (const Instr *in : flat) run.step(*in);
+    if (stats) {
+        stats->random_recorded = run.random_recorded;
+        stats->random_total = run.tab.random_collapses();
+    }
     return run.rec;
 }
 

@@ -16,7 +16,15 @@
 
 namespace sa {
 
+// Random collapses of a reference run: those that produced a recorded result, and all of them (a
+// reset of a qubit whose result was random also collapses). When every random collapse is recorded,
+// the noiseless record is uniform over an affine space of dimension random_recorded.
+struct RefStats {
+    uint64_t random_recorded = 0;
+    uint64_t random_total = 0;
+};
+
 // One byte (0 or 1) per measurement result, in record order.
-std::vector<uint8_t> reference_sample(const std::vector<const Instr *> &flat);
+std::vector<uint8_t> reference_sample(const std::vector<const Instr *> &flat, RefStats *stats = nullptr);
 
 }  // namespace sa

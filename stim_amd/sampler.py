@@ -376,6 +376,20 @@ class CompiledMeasurementSampler(CompiledDetectorSampler):
         N.check(N.lib().mfs_sample_write(self._h, int(shots), os.fsencode(filepath), format.encode(), 0, 0,
                                          None, None))
 
+    def set_engine(self, engine: str):
+        """'auto', 'frame' or 'expansion' (detection events from the symptom engine expanded into
+        records; see include/stim_amd.h mfs_set_engine)."""
+        N.check(N.lib().mfs_set_engine(self._h, {'auto': 0, 'frame': 1, 'expansion': 2}[engine]))
+
+    def engine_info(self):
+        out = (ctypes.c_uint64 * 10)()
+        N.check(N.lib().mfs_engine_info(self._h, out))
+        v = list(out)
+        if v[0] != 3:
+            return {'engine': 'frame'}
+        return {'engine': 'expansion', 'classes': v[1], 'columns': v[2], 'entries': v[3], 'build_ms': v[4] / 1000.0,
+                'sites': v[5], 'free_results': v[6], 'levels': v[7], 'terms': v[8]}
+
     def sample_with_faults(self, faults, *, bit_packed=False):
         """Measurement results with exactly the given faults per shot and no other noise (the frame
         simulator's randomization of random results still applies). Same fault addressing as

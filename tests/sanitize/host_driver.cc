@@ -23,6 +23,14 @@ static void run_circuit(const std::string &t, bool full) {
         (void)mfs_reference_sample(t.data(), t.size(), ref.data(), ref.size());
         mfs_m2d *m2d = nullptr;
         if (mfs_m2d_compile(t.data(), t.size(), 0, &m2d) == 0) mfs_m2d_destroy(m2d);
+        // The measurement sampler's expansion plan, and the host expansion of a few shots where it applies.
+        uint64_t xs[8] = {0};
+        if (mfs_expand_plan_stats(t.data(), t.size(), xs) == 0 && xs[0] && st[1] < (1u << 16)) {
+            const uint64_t shots = 3, Y = st[2] + st[3];
+            std::vector<uint8_t> y(shots * (Y + 1), 1), fr(shots * (xs[3] + 1), 1), m(shots * (st[1] + 1));
+            (void)mfs_expand_shots_host(t.data(), t.size(), shots, y.data(), Y + 1, fr.data(), xs[3] + 1, m.data(),
+                                        st[1] + 1);
+        }
         if (full) {
             mfs_sampler *s = nullptr;
             if (mfs_compile(t.data(), t.size(), 1, 1, &s) == 0) mfs_destroy(s);

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SRC = os.path.join(ROOT, 'stim_amd', 'csrc')
 KERNEL_OBJS = [os.path.join(ROOT, 'stim_amd', '_build', f'{k}.o')
                for k in ('frame_kernel', 'symptom_kernel', 'm2d_kernel')]
-HOST_SOURCES = ['circuit', 'compiler', 'symptom', 'dem', 'tableau', 'sampler', 'batch', 'server', 'm2d']
+HOST_SOURCES = ['circuit', 'compiler', 'symptom', 'dem', 'tableau', 'expand', 'sampler', 'batch', 'server', 'm2d']
 CLANG = '/opt/rocm/llvm/bin/clang++'
 FLAGS = ['-x', 'c++', '-O1', '-g', '-fPIC', '-std=c++17', '-D__HIP_PLATFORM_AMD__', '-I/opt/rocm/include',
          '-I' + SRC, '-I' + os.path.join(ROOT, 'include'), '-fsanitize=address,undefined',

@@ -103,6 +103,7 @@ def test_noalloc_measurement_and_dem_samplers():
     N, L = _lib()
     text = _circuit(d=5)
     ms = [stim_amd.Circuit(text).compile_sampler(seed=4) for _ in range(2)]
+    ms[1].engine_info()   # builds the expansion path (allocates) before the allocation-free calls
     M = ms[0].num_measurements
     for shots in (70000, 3000):
         want = _run('sample', ms[0], shots, 0, M, 0, False)
