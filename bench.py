@@ -629,7 +629,7 @@ def measure(args):
                 'algorithmic_bytes_per_launch': b_alg,
                 'symptom_kernel_ms': kernel_ms - last_ms}
         cfg = {'engine': 'expansion', 'symptom_columns': engine['columns'], 'free_results': engine['free_results'],
-               'substitution_levels': engine['levels'], 'substitution_terms': engine['terms']}
+               'substitution_phases': engine['phases'], 'substitution_items': engine['items']}
     else:
         b_alg = S * row
         achieved = b_alg / (kernel_ms * 1e-3) / 1e9

@@ -119,7 +119,8 @@ int mfs_reference_sample(const char *stim_text, size_t len, uint8_t *out, size_t
  * in for the frame simulation of sample_batch_measurements, frame_simulator_util.inl:290-372):
  * out[0..7] = whether the expansion applies (1) or not (0), measurements, pivots (independent
  * detector/observable rows), unconstrained results, random results and random collapses of the
- * reference run, substitution levels, substitution terms. */
+ * reference run, substitution phases (barriers of the expansion kernel's schedule), substitution
+ * depth (longest dependency chain). */
 int mfs_expand_plan_stats(const char *stim_text, size_t len, uint64_t out[8]);
 
 /* Host only, a test hook (no reference counterpart): the expansion of `shots` shots computed on the
@@ -188,7 +189,7 @@ int mfs_set_engine(mfs_sampler *s, int engine);
  * with separate observables), and the specialised scan kernel's distinct outcome blocks and outcome
  * records (0 when it does not apply). A measurement sampler on the expansion path (3) reports its
  * detector sampler's symptom figures in out[1..5] and, in out[6..8], its free (random) results,
- * substitution levels and substitution terms. Builds the symptom table or expansion if needed. */
+ * substitution phases and substitution items. Builds the symptom table or expansion if needed. */
 int mfs_engine_info(mfs_sampler *s, uint64_t out[10]);
 
 /* Symptom engine: target size (KB) of one workgroup's LDS image of output rows (sets the shots per

@@ -45,10 +45,15 @@ std::vector<uint32_t> odd_set(std::vector<uint32_t> v) {
 }  // namespace
 
 ExpandPlan plan_expansion(const std::vector<const Instr *> &flat, uint32_t M, uint32_t D, uint32_t O,
-                          uint64_t random_recorded, uint64_t random_total) {
+                          uint64_t random_recorded, uint64_t random_total, uint32_t threads) {
     ExpandPlan E;
     E.M = M;
     E.Y = D + O;
+    E.threads = threads;
+    if (M >= (1u << 16)) {
+        E.why = "more than 65,535 measurements";
+        return E;
+    }
     // Rows of A: each detector's measurement set, then each observable's (absolute record indices).
     std::vector<std::vector<uint32_t>> rows;
     std::vector<uint32_t> row_y;
@@ -127,47 +132,97 @@ ExpandPlan plan_expansion(const std::vector<const Instr *> &flat, uint32_t M, ui
         return E;
     }
 
-    // y contributions (CSR over y columns).
-    std::vector<uint32_t> count(E.Y + 1, 0);
+    // y contributions: each y column's first slot inline, further slots in a side list.
+    std::vector<std::vector<uint32_t>> slots_of(E.Y);
     for (const Reduced &x : red) {
-        for (uint32_t y : x.C) count[y + 1]++;
+        for (uint32_t y : x.C) slots_of[y].push_back(x.R.back());
     }
-    E.contrib_off.assign(E.Y + 1, 0);
-    for (uint32_t y = 0; y < E.Y; y++) E.contrib_off[y + 1] = E.contrib_off[y] + count[y + 1];
-    E.contrib.assign(E.contrib_off[E.Y], 0);
-    std::vector<uint32_t> fill(E.contrib_off.begin(), E.contrib_off.end() - 1);
-    for (const Reduced &x : red) {
-        for (uint32_t y : x.C) E.contrib[fill[y]++] = x.R.back();
+    E.prim.assign(E.Y, 0xFFFFFFFFu);
+    E.extra_off.assign(E.Y + 1, 0);
+    for (uint32_t y = 0; y < E.Y; y++) {
+        const auto &s = slots_of[y];
+        if (!s.empty()) E.prim[y] = s[0] | (s.size() > 1 ? 0x80000000u : 0u);
+        for (size_t i = 1; i < s.size(); i++) E.extra.push_back(s[i]);
+        E.extra_off[y + 1] = (uint32_t)E.extra.size();
     }
-    // Free results and substitution levels (record order is a topological order).
-    std::vector<uint32_t> level(M, 0);
-    uint32_t levels = 0;
+    // Free results; items with their owner thread and phase (record order is a topological order).
+    std::vector<int32_t> owner(M, -1);
+    std::vector<uint32_t> phase(M, 0), level(M, 0);
+    uint32_t next_thread = 0;
+    std::vector<uint32_t> item_list;
     for (uint32_t k = 0; k < M; k++) {
         if (pivot_of[k] < 0) {
             E.free_slots.push_back(k);
             continue;
         }
         const Reduced &x = red[(size_t)pivot_of[k]];
-        uint32_t l = 0;
-        for (size_t i = 0; i + 1 < x.R.size(); i++) l = std::max(l, level[x.R[i]] + 1);
-        level[k] = l;
-        levels = std::max(levels, l);
-    }
-    std::vector<std::vector<uint32_t>> by_level(levels + 1);
-    for (uint32_t k = 0; k < M; k++) {
-        if (pivot_of[k] >= 0 && level[k] > 0) by_level[level[k]].push_back(k);
-    }
-    E.level_off.push_back(0);
-    E.term_off.push_back(0);
-    for (uint32_t l = 1; l <= levels; l++) {
-        for (uint32_t k : by_level[l]) {
-            const Reduced &x = red[(size_t)pivot_of[k]];
-            E.item_slot.push_back(k);
-            E.terms.insert(E.terms.end(), x.R.begin(), x.R.end() - 1);
-            E.term_off.push_back((uint32_t)E.terms.size());
+        if (x.R.size() < 2) continue;   // no earlier terms: its value is in place after the y pass
+        int32_t own = -1;
+        uint32_t best = 0, lv = 0;
+        for (size_t i = 0; i + 1 < x.R.size(); i++) {
+            const uint32_t t = x.R[i];
+            lv = std::max(lv, level[t] + 1);
+            if (owner[t] >= 0 && (own < 0 || phase[t] > best)) {
+                own = owner[t];
+                best = phase[t];
+            }
         }
-        E.level_off.push_back((uint32_t)E.item_slot.size());
+        if (own < 0) own = (int32_t)(next_thread++ % threads);
+        uint32_t ph = 0;
+        for (size_t i = 0; i + 1 < x.R.size(); i++) {
+            const uint32_t t = x.R[i];
+            if (owner[t] >= 0) ph = std::max(ph, phase[t] + (owner[t] != own ? 1u : 0u));
+        }
+        owner[k] = own;
+        phase[k] = ph;
+        level[k] = lv;
+        E.levels = std::max(E.levels, lv);
+        E.phases = std::max(E.phases, ph + 1);
+        item_list.push_back(k);
     }
+    // Items grouped by (phase, thread), record order within a group.
+    std::stable_sort(item_list.begin(), item_list.end(), [&](uint32_t a, uint32_t b) {
+        if (phase[a] != phase[b]) return phase[a] < phase[b];
+        return owner[a] < owner[b];
+    });
+    // Records: consecutive one-term items of one (phase, thread) whose term is the previous item's
+    // result, at a constant record distance, merge into one chain record.
+    E.sched_off.assign((size_t)E.phases * threads + 1, 0);
+    size_t i = 0;
+    while (i < item_list.size()) {
+        const uint32_t k = item_list[i];
+        const Reduced &x = red[(size_t)pivot_of[k]];
+        const uint32_t n = (uint32_t)x.R.size() - 1;
+        if (n >= (1u << 16)) {
+            E.why = "a substitution with more than 65,535 terms";
+            return E;
+        }
+        const size_t group = (size_t)phase[k] * threads + (uint32_t)owner[k];
+        uint32_t len = 1, stride = 0;
+        if (n == 1) {
+            while (i + len < item_list.size()) {
+                const uint32_t k2 = item_list[i + len], prev = item_list[i + len - 1];
+                const Reduced &x2 = red[(size_t)pivot_of[k2]];
+                if ((size_t)phase[k2] * threads + (uint32_t)owner[k2] != group || x2.R.size() != 2 || x2.R[0] != prev)
+                    break;
+                if (len == 1) stride = k2 - prev;
+                else if (k2 - prev != stride) break;
+                len++;
+            }
+        }
+        E.items.push_back(k | (n << 16));
+        if (n == 1) {
+            E.items.push_back(x.R[0]);
+        } else {
+            E.items.push_back((uint32_t)E.terms.size());
+            E.terms.insert(E.terms.end(), x.R.begin(), x.R.end() - 1);
+        }
+        E.items.push_back(len);
+        E.items.push_back(stride);
+        E.sched_off[group + 1]++;
+        i += len;
+    }
+    for (size_t g = 1; g < E.sched_off.size(); g++) E.sched_off[g] += E.sched_off[g - 1];
     E.complete = true;
     return E;
 }
@@ -176,13 +231,31 @@ void expand_shot_host(const ExpandPlan &E, const uint8_t *y, const uint8_t *free
     std::fill(m, m + E.M, 0);
     for (size_t f = 0; f < E.free_slots.size(); f++) m[E.free_slots[f]] = free_bits[f] & 1;
     for (uint32_t j = 0; j < E.Y; j++) {
-        if (!(y[j] & 1)) continue;
-        for (uint32_t c = E.contrib_off[j]; c < E.contrib_off[j + 1]; c++) m[E.contrib[c]] ^= 1;
+        if (!(y[j] & 1) || E.prim[j] == 0xFFFFFFFFu) continue;
+        m[E.prim[j] & 0xFFFFu] ^= 1;
+        if (E.prim[j] >> 31) {
+            for (uint32_t c = E.extra_off[j]; c < E.extra_off[j + 1]; c++) m[E.extra[c]] ^= 1;
+        }
     }
-    for (size_t i = 0; i < E.item_slot.size(); i++) {
-        uint8_t v = m[E.item_slot[i]];
-        for (uint32_t t = E.term_off[i]; t < E.term_off[i + 1]; t++) v ^= m[E.terms[t]];
-        m[E.item_slot[i]] = v;
+    // Phases in order, threads in order within a phase: a topological order of the records.
+    const size_t nrec = E.sched_off.empty() ? 0 : E.sched_off.back();
+    for (size_t i = 0; i < nrec; i++) {
+        const uint32_t *r = &E.items[4 * i];
+        uint32_t k = r[0] & 0xFFFFu;
+        const uint32_t n = r[0] >> 16;
+        uint8_t v = m[k];
+        if (n == 1) {
+            v ^= m[r[1]];
+            m[k] = v;
+            for (uint32_t j = 1; j < r[2]; j++) {
+                k += r[3];
+                v ^= m[k];
+                m[k] = v;
+            }
+        } else {
+            for (uint32_t t = 0; t < n; t++) v ^= m[E.terms[r[1] + t]];
+            m[k] = v;
+        }
     }
 }
 

@@ -12,7 +12,14 @@
 // The plan solves A m' = y by forward substitution in record order. Gaussian elimination over GF(2)
 // gives each independent row a pivot (its largest measurement index after reduction); a pivot's result
 // is the XOR of the y values its reduced row combines and of the reduced row's other, earlier
-// measurements. Measurements that no row pivots on are free: uniformly random words. ker A = G is
+// measurements. Measurements that no row pivots on are free: uniformly random words.
+//
+// Substitution schedule: pivots with earlier terms (items) are owned by threads of the expansion
+// workgroup and run in phases separated by barriers. An item joins the thread that owns its latest
+// dependency; its phase is the largest phase of its dependencies, plus one for each dependency owned by
+// another thread. A thread runs its items of a phase in record order, so a chain of dependencies on
+// one thread needs no barrier (a memory circuit's checks repeat round after round: one chain per check,
+// three phases at d=25 instead of 48 substitution levels). ker A = G is
 // checked by dimension. G lies in ker A (the sampler checks that detectors and observables are
 // deterministic), so dim G <= nullity(A) = the free count. The reference run's random results are
 // random whatever the earlier results were (which results are random does not depend on outcomes in a
@@ -35,12 +42,18 @@ struct ExpandPlan {
     uint32_t M = 0;                   // measurements
     uint32_t Y = 0;                   // y columns: detectors, then observables
     uint32_t pivots = 0;
-    // y column j's value is XORed into the results (slots) contrib[contrib_off[j] .. contrib_off[j+1]).
-    std::vector<uint32_t> contrib_off, contrib;
+    // y column j's value is XORed into result prim[j] & 0xFFFF (none when prim[j] is 0xFFFFFFFF) and,
+    // when bit 31 is set, into results extra[extra_off[j] .. extra_off[j+1]) as well.
+    std::vector<uint32_t> prim, extra_off, extra;
     std::vector<uint32_t> free_slots;  // results drawn uniformly at random
-    // Substitution items by level: items [level_off[l], level_off[l+1]) depend only on results of
-    // lower levels; item i sets result item_slot[i] ^= XOR of results terms[term_off[i] .. term_off[i+1]).
-    std::vector<uint32_t> level_off, item_slot, term_off, terms;
+    // Substitution records, 4 words each: (x, y, z, w). With n = x >> 16 terms and k = x & 0xFFFF:
+    // n = 1: a chain of z results k, k + w, k + 2w, ...: the first is XORed with result y, each later one
+    // with the result before it in the chain (a check repeated round after round);
+    // n > 1: result k ^= the XOR of results terms[y .. y + n) (z = 1).
+    // Thread t runs records [sched_off[p * threads + t], sched_off[p * threads + t + 1]) in phase p.
+    uint32_t threads = 0, phases = 0;
+    std::vector<uint32_t> sched_off, items, terms;
+    uint32_t levels = 0;               // substitution depth (longest dependency chain), for reports
 };
 
 // Plans the expansion for a circuit with M measurements, D detectors and O observables, given the
@@ -48,7 +61,7 @@ struct ExpandPlan {
 // Pauli-frame terms are left out of A.
 // A plan whose elimination fills in beyond a fixed budget is reported incomplete.
 ExpandPlan plan_expansion(const std::vector<const Instr *> &flat, uint32_t M, uint32_t D, uint32_t O,
-                          uint64_t random_recorded, uint64_t random_total);
+                          uint64_t random_recorded, uint64_t random_total, uint32_t threads = 1024);
 
 // The expansion of one shot on the host, as the kernel computes it (one byte per bit): y = the Y
 // column values, free_bits = one value per free result in plan order; writes the M results (without

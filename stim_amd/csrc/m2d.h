@@ -39,20 +39,22 @@ struct M2dParams {
 };
 
 // Expansion of detection events into measurement records (the measurement sampler's fast path; plan
-// in expand.h). One workgroup expands a tile of 32 shots: its measurement words start at zero, free
-// results get random words, each y column's word is XORed into its contribution slots, substitution
-// levels run in order, and the words leave through the identity columns of `out` (column m = word m,
-// inverted where the reference sample is 1).
+// and table layouts in expand.h). One workgroup expands a tile of 32 shots: its measurement words start
+// at zero, free results get random words, each y column's word is XORed into its slots, the
+// substitution phases run (each thread its own items, a barrier between phases), and the words leave
+// through the identity columns of `out` (column m = word m, inverted where the reference sample is 1).
 struct ExpandDev {
     const uint8_t *in;          // y rows: detection events then observable flips, bit packed; 16-byte
     uint64_t in_stride;         // aligned rows with a stride that is a multiple of 16
     uint32_t in_groups;         // 16-byte groups of a y row that hold its Y bits
     uint32_t Y;
-    const uint32_t *contrib_off, *contrib;
+    const uint32_t *prim, *extra_off, *extra;
     const uint32_t *free_slots;
     uint32_t nfree;
-    uint32_t nlevels;
-    const uint32_t *level_off, *item_slot, *term_off, *terms;
+    uint32_t nphases;
+    const uint32_t *sched_off;  // nphases * 1024 + 1 record offsets (phase-major, then thread)
+    const uint4 *items;
+    const uint32_t *terms;
     uint64_t seed;              // Philox key of the free results
     uint64_t tile0;             // 32-shot tile of the launch's first shot in the sampler's stream
     M2dParams out;              // output: M identity columns over the tile's measurement words

@@ -219,22 +219,40 @@ __global__ __launch_bounds__(NT) void expand_kernel(ExpandDev E) {
                 for (uint32_t k = 0; k < 4; k++) {
                     const uint32_t x = transpose32(xs[k], lane);
                     const uint32_t y = (4 * (g0 + u) + k) * 32 + lane;
-                    if (y < E.Y) {
-                        for (uint32_t c = E.contrib_off[y]; c < E.contrib_off[y + 1]; c++) atomicXor(&words[E.contrib[c]], x);
+                    const uint32_t pr = y < E.Y ? E.prim[y] : 0xFFFFFFFFu;
+                    if (pr != 0xFFFFFFFFu) {
+                        atomicXor(&words[pr & 0xFFFFu], x);
+                        if (pr >> 31) {
+                            for (uint32_t c = E.extra_off[y]; c < E.extra_off[y + 1]; c++) atomicXor(&words[E.extra[c]], x);
+                        }
                     }
                 }
             }
         }
     }
     __syncthreads();
-    // Substitution: items of one level read only results of lower levels.
-    for (uint32_t l = 0; l < E.nlevels; l++) {
-        const uint32_t a = E.level_off[l], b = E.level_off[l + 1];
-        for (uint32_t i = a + tid; i < b; i += NT) {
-            const uint32_t k = E.item_slot[i];
+    // Substitution: each thread runs its own items of a phase in record order (its chains need no
+    // barrier); items read results of earlier phases or of the same thread.
+    for (uint32_t p = 0; p < E.nphases; p++) {
+        const uint32_t a = E.sched_off[p * NT + tid], b = E.sched_off[p * NT + tid + 1];
+        for (uint32_t i = a; i < b; i++) {
+            const uint4 r = E.items[i];
+            uint32_t k = r.x & 0xFFFFu;
+            const uint32_t n = r.x >> 16;
             uint32_t v = words[k];
-            for (uint32_t t = E.term_off[i]; t < E.term_off[i + 1]; t++) v ^= words[E.terms[t]];
-            words[k] = v;
+            if (n == 1) {
+                // A chain: each later result XORs the one before it, kept in v.
+                v ^= words[r.y];
+                words[k] = v;
+                for (uint32_t j = 1; j < r.z; j++) {
+                    k += r.w;
+                    v ^= words[k];
+                    words[k] = v;
+                }
+            } else {
+                for (uint32_t t = 0; t < n; t++) v ^= words[E.terms[r.y + t]];
+                words[k] = v;
+            }
         }
         __syncthreads();
     }

@@ -299,8 +299,8 @@ int mfs_expand_plan_stats(const char *stim_text, size_t len, uint64_t out[8]) {
         out[3] = E.complete ? E.free_slots.size() : (uint64_t)st.num_measurements - E.pivots;
         out[4] = rs.random_recorded;
         out[5] = rs.random_total;
-        out[6] = E.level_off.empty() ? 0 : E.level_off.size() - 1;
-        out[7] = E.terms.size();
+        out[6] = E.phases;
+        out[7] = E.levels;
     });
 }
 
@@ -527,8 +527,8 @@ int mfs_engine_info(mfs_sampler *s, uint64_t out[10]) {
             out[4] = (uint64_t)(d->sym_build_ms * 1000.0);
             out[5] = d->plan.total_sites;
             out[6] = s->expand_free;
-            out[7] = s->expand_levels;
-            out[8] = s->expand_terms;
+            out[7] = s->expand_phases;
+            out[8] = s->expand_items;
             out[9] = 0;
             return;
         }

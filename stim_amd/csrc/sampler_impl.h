@@ -98,6 +98,8 @@ inline uint64_t splitmix64(uint64_t x) {
 
 constexpr size_t MAX_LDS = LDS_PER_WORKGROUP;
 constexpr size_t HOST_BATCH_BYTES = (size_t)1 << 30;
+// Threads of an expansion workgroup (m2d_kernel.hip's workgroup size), the width of its schedule.
+constexpr uint32_t EXPAND_THREADS = 1024;
 // Host-bound bytes per batch of a call that returns rows in host memory (one of two pipeline slots).
 constexpr size_t PIPE_BATCH_BYTES = (size_t)64 << 20;
 
@@ -168,8 +170,8 @@ struct mfs_sampler {
     bool borrowed_stream = false;         // the stream belongs to the sampler that owns this one
     size_t row_pitch_override = 0;        // symptom-engine rows padded to this pitch (0: packed width)
     uint64_t expand_key = 0, expand_tiles = 0;
-    uint32_t expand_free = 0, expand_levels = 0, expand_terms = 0;
-    DevBuf d_x_contrib_off, d_x_contrib, d_x_free, d_x_level_off, d_x_item, d_x_term_off, d_x_terms, d_x_cols, d_x_y;
+    uint32_t expand_free = 0, expand_phases = 0, expand_items = 0;
+    DevBuf d_x_prim, d_x_extra_off, d_x_extra, d_x_free, d_x_sched, d_x_items, d_x_terms, d_x_cols, d_x_y;
     // Detector-error-model samplers: no circuit; the symptom tables come from the model's mechanisms.
     bool dem_mode = false;
     std::vector<uint32_t> dem_col;   // mechanism index (model order) -> symptom column
@@ -511,20 +513,21 @@ struct mfs_sampler {
             return false;
         }
         const uint32_t D = d->prog.num_detectors, O = d->prog.num_observables;
-        const ExpandPlan E = plan_expansion(flat, M, D, O, rs.random_recorded, rs.random_total);
+        const ExpandPlan E = plan_expansion(flat, M, D, O, rs.random_recorded, rs.random_total, EXPAND_THREADS);
         if (!E.complete) {
             expand_why = E.why;
             return false;
         }
         // The y rows are the detector sampler's packed rows (observables appended), padded to 16 bytes.
         d->row_pitch_override = std::max<size_t>(((D + O + 7) / 8 + 15) / 16 * 16, 16);
-        upload(d_x_contrib_off, E.contrib_off, "expansion contributions");
-        upload(d_x_contrib, E.contrib.empty() ? std::vector<uint32_t>{0} : E.contrib, "expansion contributions");
-        upload(d_x_free, E.free_slots.empty() ? std::vector<uint32_t>{0} : E.free_slots, "expansion free results");
-        upload(d_x_level_off, E.level_off, "expansion levels");
-        upload(d_x_item, E.item_slot.empty() ? std::vector<uint32_t>{0} : E.item_slot, "expansion items");
-        upload(d_x_term_off, E.term_off, "expansion terms");
-        upload(d_x_terms, E.terms.empty() ? std::vector<uint32_t>{0} : E.terms, "expansion terms");
+        auto nonempty = [](const std::vector<uint32_t> &v) { return v.empty() ? std::vector<uint32_t>{0, 0} : v; };
+        upload(d_x_prim, nonempty(E.prim), "expansion y slots");
+        upload(d_x_extra_off, E.extra_off, "expansion y slots");
+        upload(d_x_extra, nonempty(E.extra), "expansion y slots");
+        upload(d_x_free, nonempty(E.free_slots), "expansion free results");
+        upload(d_x_sched, E.sched_off, "expansion schedule");
+        upload(d_x_items, E.items.empty() ? std::vector<uint32_t>(4, 0) : E.items, "expansion items");
+        upload(d_x_terms, nonempty(E.terms), "expansion terms");
         // Output columns: measurement m is word m, inverted where the inversion (reference) bit is 1.
         std::vector<uint2> cols(M);
         for (uint32_t m = 0; m < M; m++) {
@@ -533,8 +536,8 @@ struct mfs_sampler {
         }
         upload(d_x_cols, cols, "expansion output columns");
         expand_free = (uint32_t)E.free_slots.size();
-        expand_levels = (uint32_t)E.level_off.size() - 1;
-        expand_terms = (uint32_t)E.terms.size();
+        expand_phases = E.phases;
+        expand_items = (uint32_t)(E.items.size() / 4);
         expand_key = splitmix64(key ^ 0xE8A7D5ull);
         det_src = std::move(d);
         expand_state = 1;
@@ -577,14 +580,14 @@ struct mfs_sampler {
         E.in_stride = d.row_pitch_override;
         E.in_groups = (uint32_t)(d.row_pitch_override / 16);
         E.Y = d.prog.num_detectors + d.prog.num_observables;
-        E.contrib_off = d_x_contrib_off.as<uint32_t>();
-        E.contrib = d_x_contrib.as<uint32_t>();
+        E.prim = d_x_prim.as<uint32_t>();
+        E.extra_off = d_x_extra_off.as<uint32_t>();
+        E.extra = d_x_extra.as<uint32_t>();
         E.free_slots = d_x_free.as<uint32_t>();
         E.nfree = expand_free;
-        E.nlevels = expand_levels;
-        E.level_off = d_x_level_off.as<uint32_t>();
-        E.item_slot = d_x_item.as<uint32_t>();
-        E.term_off = d_x_term_off.as<uint32_t>();
+        E.nphases = expand_phases;
+        E.sched_off = d_x_sched.as<uint32_t>();
+        E.items = d_x_items.as<uint4>();
         E.terms = d_x_terms.as<uint32_t>();
         E.seed = expand_key;
         E.tile0 = tile0;
@@ -597,7 +600,7 @@ struct mfs_sampler {
         P.n_obs = 0;
         P.shots = n;
         P.cols = d_x_cols.as<uint2>();
-        P.ovf = d_x_contrib.as<uint32_t>();   // never read: every column has one term
+        P.ovf = d_x_prim.as<uint32_t>();   // never read: every column has one term
         return E;
     }
 

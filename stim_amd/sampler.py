@@ -388,7 +388,7 @@ class CompiledMeasurementSampler(CompiledDetectorSampler):
         if v[0] != 3:
             return {'engine': 'frame'}
         return {'engine': 'expansion', 'classes': v[1], 'columns': v[2], 'entries': v[3], 'build_ms': v[4] / 1000.0,
-                'sites': v[5], 'free_results': v[6], 'levels': v[7], 'terms': v[8]}
+                'sites': v[5], 'free_results': v[6], 'phases': v[7], 'items': v[8]}
 
     def sample_with_faults(self, faults, *, bit_packed=False):
         """Measurement results with exactly the given faults per shot and no other noise (the frame

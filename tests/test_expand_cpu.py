@@ -28,7 +28,7 @@ def _stats(text):
     out = (ctypes.c_uint64 * 8)()
     b = text.encode()
     N.check(L.mfs_expand_plan_stats(b, len(b), out))
-    return dict(zip(['complete', 'M', 'pivots', 'free', 'random', 'random_total', 'levels', 'terms'], list(out)))
+    return dict(zip(['complete', 'M', 'pivots', 'free', 'random', 'random_total', 'phases', 'depth'], list(out)))
 
 
 def _expand(text, y, free):
@@ -60,6 +60,8 @@ def test_plan_applies_to_the_headline_circuit(golden):
     s = _stats(golden.circuit('surface_d25_r25'))
     assert s['complete'] == 1, s
     assert s['M'] == 16225 and s['random'] == 624 and s['pivots'] == 16225 - 624
+    # One chain per check through the 25 rounds: three barrier phases for a dependency depth of 48.
+    assert s['phases'] == 3 and s['depth'] == 48, s
 
 
 @pytest.mark.parametrize('name', MEMORY)
