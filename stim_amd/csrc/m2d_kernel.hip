@@ -203,13 +203,21 @@ __global__ __launch_bounds__(NT) void expand_kernel(ExpandDev E) {
     // y words, transposed across half-waves like the measurement words of m2d_kernel, XORed into their
     // slots. Lanes past the tile's shots read the tile's first row: those bits of every word belong to
     // shots that are never written.
+    // Each lane's slot of its y column is loaded with the rows (it depends on nothing computed), so
+    // a round's loads are all in flight together before the transposes and the XORs.
     const uint8_t *row = E.in + (s0 + (have ? lane : 0)) * E.in_stride;
     for (uint32_t g0 = half * UNROLL; g0 < E.in_groups; g0 += HALVES * UNROLL) {
         uint4 v[UNROLL];
+        uint32_t pr[UNROLL][4];
 #pragma unroll
         for (uint32_t u = 0; u < UNROLL; u++) {
             const uint32_t g = g0 + u < E.in_groups ? g0 + u : E.in_groups - 1;
             v[u] = *reinterpret_cast<const uint4 *>(row + 16 * (uint64_t)g);
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) {
+                const uint32_t y = (4 * (g0 + u) + k) * 32 + lane;
+                pr[u][k] = (g0 + u < E.in_groups && y < E.Y) ? E.prim[y] : 0xFFFFFFFFu;
+            }
         }
 #pragma unroll
         for (uint32_t u = 0; u < UNROLL; u++) {
@@ -217,14 +225,12 @@ __global__ __launch_bounds__(NT) void expand_kernel(ExpandDev E) {
                 const uint32_t xs[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
 #pragma unroll
                 for (uint32_t k = 0; k < 4; k++) {
+                    // Every lane of the half-wave takes part in the transpose.
                     const uint32_t x = transpose32(xs[k], lane);
-                    const uint32_t y = (4 * (g0 + u) + k) * 32 + lane;
-                    const uint32_t pr = y < E.Y ? E.prim[y] : 0xFFFFFFFFu;
-                    if (pr != 0xFFFFFFFFu) {
-                        atomicXor(&words[pr & 0xFFFFu], x);
-                        if (pr >> 31) {
-                            for (uint32_t c = E.extra_off[y]; c < E.extra_off[y + 1]; c++) atomicXor(&words[E.extra[c]], x);
-                        }
+                    if (pr[u][k] != 0xFFFFFFFFu) atomicXor(&words[pr[u][k] & 0xFFFFu], x);
+                    if (pr[u][k] != 0xFFFFFFFFu && (pr[u][k] >> 31)) {
+                        const uint32_t y = (4 * (g0 + u) + k) * 32 + lane;
+                        for (uint32_t c = E.extra_off[y]; c < E.extra_off[y + 1]; c++) atomicXor(&words[E.extra[c]], x);
                     }
                 }
             }
