@@ -1,5 +1,8 @@
 // C-ABI entry points of the stim_amd sampler: compile, sample, sample_write, fixed-fault sampling,
 // and the result-format writers.
+#include <array>
+#include <thread>
+
 #include "sampler_impl.h"
 #include "tableau.h"
 
@@ -38,18 +41,47 @@ void write_rows(FILE *out, Fmt fmt, const uint8_t *rows, size_t stride, uint64_t
     } else if (prefix1 == prefix2 || transition >= ncols) {
         transition = ncols;
     }
-    std::string line;
+    const size_t row_bytes = (ncols + 7) / 8;
+    if (fmt == Fmt::B8 && stride == row_bytes) {
+        // Contiguous rows are the file's bytes as they are.
+        fwrite(rows, 1, nshots * row_bytes, out);
+        return;
+    }
+    if (fmt == Fmt::F01) {
+        // Eight characters per packed byte from a table (bit 0 first), whole chunk per write.
+        static const std::array<uint64_t, 256> chars = [] {
+            std::array<uint64_t, 256> t{};
+            for (uint32_t b = 0; b < 256; b++) {
+                for (uint32_t k = 0; k < 8; k++) t[b] |= (uint64_t)('0' + ((b >> k) & 1)) << (8 * k);
+            }
+            return t;
+        }();
+        const size_t line = (size_t)ncols + 1;
+        const uint64_t per = std::max<uint64_t>(1, ((size_t)16 << 20) / line);
+        std::vector<char> text;
+        for (uint64_t s0 = 0; s0 < nshots; s0 += per) {
+            const uint64_t n = std::min(per, nshots - s0);
+            text.resize(n * line + 8);
+            for (uint64_t s = 0; s < n; s++) {
+                const uint8_t *r = rows + (s0 + s) * stride;
+                char *o = text.data() + s * line;
+                const uint32_t full = ncols / 8;
+                for (uint32_t b = 0; b < full; b++) std::memcpy(o + 8 * b, &chars[r[b]], 8);
+                for (uint32_t m = 8 * full; m < ncols; m++) o[m] = (char)('0' + ((r[m >> 3] >> (m & 7)) & 1));
+                o[ncols] = '\n';
+            }
+            fwrite(text.data(), 1, n * line, out);
+        }
+        return;
+    }
     for (uint64_t s = 0; s < nshots; s++) {
         const uint8_t *r = rows + s * stride;
         auto bit = [&](uint32_t m) { return (r[m >> 3] >> (m & 7)) & 1; };
         switch (fmt) {
             case Fmt::B8:
-                fwrite(r, 1, (ncols + 7) / 8, out);
+                fwrite(r, 1, row_bytes, out);
                 break;
             case Fmt::F01:
-                line.assign(ncols + 1, '\n');
-                for (uint32_t m = 0; m < ncols; m++) line[m] = (char)('0' + bit(m));
-                fwrite(line.data(), 1, line.size(), out);
                 break;
             case Fmt::HITS: {
                 bool first = true;
@@ -386,7 +418,6 @@ int mfs_sample_write(mfs_sampler *s, uint64_t shots, const char *path, const cha
         const uint32_t ncols = D + O * ((prepend_observables ? 1 : 0) + (append_observables ? 1 : 0));
         const size_t drow = (ncols + 7) / 8, orow = (O + 7) / 8;
         const uint64_t chunk = std::max<uint64_t>(64, ((64ull << 20) / std::max<size_t>(drow + orow, 1)) / 64 * 64);
-        std::vector<uint8_t> dbuf, obuf;
         char p1 = s->meas_mode ? 'M' : 'D', p2 = 'L';
         uint32_t transition = D;
         if (prepend_observables) {
@@ -394,17 +425,35 @@ int mfs_sample_write(mfs_sampler *s, uint64_t shots, const char *path, const cha
             p2 = 'D';
             transition = O;
         }
-        for (uint64_t s0 = 0; s0 < shots; s0 += chunk) {
-            const uint64_t n = std::min(chunk, shots - s0);
-            dbuf.assign(n * std::max<size_t>(drow, 1), 0);
-            obuf.assign(n * std::max<size_t>(orow, 1), 0);
-            s->sample(n, flags, dbuf.data(), std::max<size_t>(drow, 1), obuf.data(), std::max<size_t>(orow, 1));
-            if (fmt == Fmt::PTB64) write_ptb64(out.get(), dbuf.data(), std::max<size_t>(drow, 1), n, ncols);
-            else write_rows(out.get(), fmt, dbuf.data(), std::max<size_t>(drow, 1), n, ncols, p1, p2, transition);
-            if (has_obs) {
-                if (ofmt == Fmt::PTB64) write_ptb64(oout.get(), obuf.data(), std::max<size_t>(orow, 1), n, O);
-                else write_rows(oout.get(), ofmt, obuf.data(), std::max<size_t>(orow, 1), n, O, 'L', 'L', O);
+        // A writer thread formats and writes chunk c while the GPU samples chunk c + 1 into the other
+        // buffer pair.
+        std::vector<uint8_t> dbufs[2], obufs[2];
+        std::thread writer;
+        struct Join {
+            std::thread &t;
+            ~Join() {
+                if (t.joinable()) t.join();
             }
+        } join_writer{writer};
+        const size_t dstride = std::max<size_t>(drow, 1), ostride = std::max<size_t>(orow, 1);
+        uint32_t c = 0;
+        for (uint64_t s0 = 0; s0 < shots; s0 += chunk, c ^= 1) {
+            const uint64_t n = std::min(chunk, shots - s0);
+            std::vector<uint8_t> &db = dbufs[c], &ob = obufs[c];
+            db.assign(n * dstride, 0);
+            ob.assign(n * ostride, 0);
+            s->sample(n, flags, db.data(), dstride, ob.data(), ostride);
+            if (writer.joinable()) writer.join();
+            writer = std::thread([&, n, c] {
+                FILE *f = out.get();
+                const std::vector<uint8_t> &d = dbufs[c], &o = obufs[c];
+                if (fmt == Fmt::PTB64) write_ptb64(f, d.data(), dstride, n, ncols);
+                else write_rows(f, fmt, d.data(), dstride, n, ncols, p1, p2, transition);
+                if (has_obs) {
+                    if (ofmt == Fmt::PTB64) write_ptb64(oout.get(), o.data(), ostride, n, O);
+                    else write_rows(oout.get(), ofmt, o.data(), ostride, n, O, 'L', 'L', O);
+                }
+            });
         }
     });
 }
