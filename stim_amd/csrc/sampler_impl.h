@@ -911,13 +911,13 @@ struct mfs_sampler {
                 }
                 auto found = blocks.find(key);
                 if (found != blocks.end()) {
-                    sites[site0[c] + st] = make_uint2(found->second, b);
+                    sites[site0[c] + st] = make_uint2(found->second | (nout << 28), b);
                     continue;
                 }
                 const uint64_t first = orec.size();
-                if (first + nout >= (1ull << 31) || (first + nout) * 8 > ((uint64_t)1 << 30)) return;
+                if (first + nout >= (1ull << 28) || (first + nout) * 8 > ((uint64_t)1 << 30)) return;
                 blocks.emplace(key, (uint32_t)first);
-                sites[site0[c] + st] = make_uint2((uint32_t)first, b);
+                sites[site0[c] + st] = make_uint2((uint32_t)first | (nout << 28), b);
                 for (uint32_t code = 1; code <= nout; code++) {
                     acc = site_syms[code - 1];
                     for (uint32_t &e : acc)
@@ -947,7 +947,11 @@ struct mfs_sampler {
         std::vector<uint32_t> base(4 * std::max<size_t>(plan.groups.size(), 1), 0);
         for (size_t gi = 0; gi < plan.groups.size(); gi++) {
             const SymGroup &g = plan.groups[gi];
-            for (uint32_t k = 0; k < g.nmem; k++) base[4 * gi + k] = (uint32_t)(site0[g.c0 + k] - g.bound[k]);
+            // Member k's sites follow member k - 1's in the site table, as in the group's site order.
+            for (uint32_t k = 0; k < g.nmem; k++) {
+                base[4 * gi + k] = (uint32_t)(site0[g.c0 + k] - g.bound[k]);
+                if (base[4 * gi + k] != base[4 * gi]) return;
+            }
         }
         if (orec.empty()) orec.push_back(make_uint2(0, 0));
         upload(d_orecs, orec, "outcome records");

@@ -141,8 +141,8 @@ struct SymDevParams {
     // cancel removed. Written relative to the site's lowest detector b, these symptoms repeat from
     // site to site (every round of a memory circuit repeats the one before, shifted), so each distinct
     // relative block of nout records is stored once. orec_sites holds, per site, (index of its block's
-    // first record in orecs, b); member k of group g finds group site s's entry at orec_base[4 g + k] + s
-    // (32-bit wrapping arithmetic), and the hit's detectors are b + the record's entries.
+    // first record in orecs | nout << 28, b); group g's site s has its entry at orec_base[4 g] + s (the
+    // group's members are consecutive in the table), and the hit's detectors are b + the record's entries.
     const uint2 *orecs;
     const uint32_t *orec_base;
     const uint2 *orec_sites;

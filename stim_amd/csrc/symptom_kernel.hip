@@ -557,15 +557,16 @@ __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile,
 // site entries together, then their K outcome records together (entries past the round's hits name
 // the group's first site and are skipped), so that the compiler can wait for each load in turn; then
 // flips each hit's detector bits without branching per entry (an absent entry XORs 0 into its word)
-// and its observable flags. site[k] = the hit's site entry index, code[k] = its outcome code - 1.
+// and its observable flags. site[k] = the hit's site entry index, oc[k] = its outcome draw: the code is
+// mulhi(oc, nout) + 1 with the site's nout from its entry, as the generic kernel computes it.
 template <bool OBS, uint32_t K>
 __device__ __forceinline__ void apply_outcomes(const SymDevParams &P, const Sink &sink, const uint32_t *site,
-                                               const uint32_t *code, const uint32_t *shot, uint32_t S) {
+                                               const uint32_t *oc, const uint32_t *shot, uint32_t S) {
     uint2 se[K], rec[K];
 #pragma unroll
     for (uint32_t k = 0; k < K; k++) se[k] = P.orec_sites[site[k]];
 #pragma unroll
-    for (uint32_t k = 0; k < K; k++) rec[k] = P.orecs[se[k].x + code[k]];
+    for (uint32_t k = 0; k < K; k++) rec[k] = P.orecs[(se[k].x & 0x0FFFFFFFu) + __umulhi(oc[k], se[k].x >> 28)];
 #pragma unroll
     for (uint32_t k = 0; k < K; k++) {
         if (shot[k] < S) {
@@ -638,8 +639,7 @@ __device__ __forceinline__ void scan_tile_fast(const SymDevParams &P, uint32_t b
         const float mu = (float)N * G.p;
         uint32_t m = (uint32_t)((mu + 3.0f * sqrtf(mu) + 4.0f) / NT) + 1;
         if (m > SCAN_M_MAX) m = SCAN_M_MAX;
-        const uint32_t b0 = P.orec_base[4 * gi], b1 = P.orec_base[4 * gi + 1], b2 = P.orec_base[4 * gi + 2],
-                       b3 = P.orec_base[4 * gi + 3];
+        const uint32_t b0 = P.orec_base[4 * gi];
         uint32_t base = 0;
         for (uint32_t round = 0; base < N; round++) {
             uint32_t g[SCAN_M_MAX], oc[SCAN_M_MAX];
@@ -664,24 +664,18 @@ __device__ __forceinline__ void scan_tile_fast(const SymDevParams &P, uint32_t b
             const uint2 sc = block_scan(sum, scratch);
             uint32_t pos = sat_add(base, sc.x);
             pos = pos < N ? pos : N;
-            // Pass 1: site entry, outcome and shot of every hit (shot S marks no hit).
-            uint32_t idx[SCAN_M_MAX], shot[SCAN_M_MAX], code[SCAN_M_MAX];
+            // Pass 1: site entry and shot of every hit (shot S marks no hit; its entry is the group's
+            // first site's, which is read and not applied).
+            uint32_t idx[SCAN_M_MAX], shot[SCAN_M_MAX];
 #pragma unroll
             for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
-                idx[k] = b0 + G.bound[0];
+                idx[k] = b0;
                 shot[k] = S;
-                code[k] = 0;
                 if (k < m) {
                     pos += g[k] < N ? g[k] : N;
                     pos = pos < N ? pos : N;
-                    const uint32_t srel = pos >> lg;
-                    const bool m1 = srel >= G.bound[1], m2 = srel >= G.bound[2], m3 = srel >= G.bound[3];
-                    const uint32_t nf = m3 ? G.nout[3] : m2 ? G.nout[2] : m1 ? G.nout[1] : G.nout[0];
-                    const uint32_t bb = m3 ? b3 : m2 ? b2 : m1 ? b1 : b0;
-                    const uint32_t nout = nf & 0xFFFFu;
                     if (pos < N) {
-                        idx[k] = bb + srel;
-                        code[k] = __umulhi(oc[k], nout);
+                        idx[k] = b0 + (pos >> lg);
                         shot[k] = pos & (S - 1);
                     }
                     pos += 1;
@@ -689,8 +683,8 @@ __device__ __forceinline__ void scan_tile_fast(const SymDevParams &P, uint32_t b
             }
             // Pass 2 and 3: record loads in flight together, then the flips (m is uniform).
             static_assert(SCAN_M_MAX == 8, "two apply widths");
-            if (m > 4) apply_outcomes<OBS, 8>(P, sink, idx, code, shot, S);
-            else apply_outcomes<OBS, 4>(P, sink, idx, code, shot, S);
+            if (m > 4) apply_outcomes<OBS, 8>(P, sink, idx, oc, shot, S);
+            else apply_outcomes<OBS, 4>(P, sink, idx, oc, shot, S);
             base = sat_add(base, sc.y);
             base = base < N ? base : N;
         }
