@@ -4,7 +4,7 @@
 // qubit, shots along the bits), the measurement-record ring, 256 staged detector rows and the
 // observable rows live in LDS for the entire circuit. The workgroup walks the device program
 // (program.h) in order; each op spreads its items (targets, pairs, detectors, noise chunks) over
-// the 256 lanes, and ops that touch a row an earlier unsynchronized op touched are separated by a
+// the workgroup's lanes, and ops that touch a row an earlier unsynchronized op touched are separated by a
 // workgroup barrier (placed by the host compiler).
 //
 // Reference semantics restated here:
@@ -30,7 +30,10 @@ using namespace sa;
 
 namespace {
 
-constexpr int NT = 256;
+// Workgroup widths: 256 lanes, or 1024 for circuits whose tile fills most of a CU's LDS (one
+// workgroup per CU), so that the CU still has 16 waves to run.
+constexpr int NT_SMALL = 256;
+constexpr int NT_LARGE = 1024;
 
 template <int T>
 struct alignas(4 * T) Wd {
@@ -162,7 +165,7 @@ __device__ __forceinline__ void apply_hit(const Tile<T> &t, const DevOp &op, con
 }
 
 // Writes staged rows [0, n) (entries base .. base + n) of the tile to one output.
-template <int T>
+template <int T, int NT>
 __device__ void flush_rows(const uint32_t *rows, uint32_t n, uint32_t base, const DevOut &out, uint64_t shot0,
                            uint64_t shots_valid) {
     const uint32_t tid = threadIdx.x;
@@ -208,7 +211,7 @@ __device__ __forceinline__ Wd<T> rand_words(const Rng &rng, uint32_t op_i, uint3
     return out;
 }
 
-template <int T>
+template <int T, int NT>
 __global__ __launch_bounds__(NT) void frame_sim_kernel(const uint32_t *__restrict__ blob,
                                                        const uint32_t *__restrict__ ntgt,
                                                        const uint32_t *__restrict__ otab, LaunchParams P) {
@@ -476,10 +479,10 @@ __global__ __launch_bounds__(NT) void frame_sim_kernel(const uint32_t *__restric
                     }
                     break;
                 case OP_FLUSH:
-                    if (P.dets.enabled) flush_rows<T>(t.STG, op.n, op.base, P.dets, shot0, P.shots);
+                    if (P.dets.enabled) flush_rows<T, NT>(t.STG, op.n, op.base, P.dets, shot0, P.shots);
                     break;
                 case OP_FLUSH_OBS:
-                    if (P.obs.enabled) flush_rows<T>(t.OBS, op.n, 0, P.obs, shot0, P.shots);
+                    if (P.obs.enabled) flush_rows<T, NT>(t.OBS, op.n, 0, P.obs, shot0, P.shots);
                     break;
                 case OP_CHAINCLR:
                     if (tid < T) t.CHAIN[tid] = 0;
@@ -592,18 +595,23 @@ namespace sa {
 hipError_t launch_frame_kernel(int T, const DevProgram &prog, const LaunchParams &P, uint32_t num_tiles, size_t lds,
                                hipStream_t stream) {
     if (num_tiles == 0) return hipSuccess;
-    dim3 grid(num_tiles), block(NT);
-#define STIM_AMD_LAUNCH(TW)                                                                                     \
+    // A tile of more than 64 KB leaves room for at most two workgroups per CU: use the wide workgroup.
+    const bool wide = lds > 64 * 1024;
+    dim3 grid(num_tiles), block(wide ? NT_LARGE : NT_SMALL);
+#define STIM_AMD_LAUNCH(TW, NB)                                                                                 \
     {                                                                                                           \
         static size_t reserved = 0;                                                                             \
-        const hipError_t e = reserve_lds((const void *)frame_sim_kernel<TW>, lds, reserved);                    \
+        const hipError_t e = reserve_lds((const void *)frame_sim_kernel<TW, NB>, lds, reserved);                \
         if (e != hipSuccess) return e;                                                                          \
     }                                                                                                           \
-    hipLaunchKernelGGL(frame_sim_kernel<TW>, grid, block, lds, stream, prog.blob, prog.ntgt, prog.otab, P);
-    switch (T) {
-        case 1: STIM_AMD_LAUNCH(1) break;
-        case 2: STIM_AMD_LAUNCH(2) break;
-        case 4: STIM_AMD_LAUNCH(4) break;
+    hipLaunchKernelGGL((frame_sim_kernel<TW, NB>), grid, block, lds, stream, prog.blob, prog.ntgt, prog.otab, P);
+    switch (T * 2 + (wide ? 1 : 0)) {
+        case 2: STIM_AMD_LAUNCH(1, NT_SMALL) break;
+        case 3: STIM_AMD_LAUNCH(1, NT_LARGE) break;
+        case 4: STIM_AMD_LAUNCH(2, NT_SMALL) break;
+        case 5: STIM_AMD_LAUNCH(2, NT_LARGE) break;
+        case 8: STIM_AMD_LAUNCH(4, NT_SMALL) break;
+        case 9: STIM_AMD_LAUNCH(4, NT_LARGE) break;
         default: return hipErrorInvalidValue;
     }
 #undef STIM_AMD_LAUNCH
