@@ -490,7 +490,13 @@ struct mfs_sampler {
             return false;
         }
         RefStats rs;
-        (void)reference_sample(flat, &rs);
+        try {
+            (void)reference_sample(flat, &rs);
+        } catch (const std::exception &e) {
+            // A circuit too large for the host tableau keeps the frame engine.
+            expand_why = e.what();
+            return false;
+        }
         auto d = std::make_unique<mfs_sampler>();
         d->flat = flat;
         d->prog = compile_program(flat);
