@@ -108,6 +108,17 @@ int mfs_m2d_convert_sweep(mfs_m2d *c, uint64_t shots, const uint8_t *meas, size_
 /* Benchmark helper (no reference counterpart): average kernel time in ms of converting `shots`
  * device-resident rows (observables appended), measured with HIP events on the converter's stream. */
 int mfs_m2d_bench(mfs_m2d *c, uint64_t shots, int warmup, int iters, double out_ms[1]);
+/* Kernel selection (no reference counterpart; results are identical either way). Circuits without
+ * sweep bits whose detectors only reach back a bounded number of results (memory experiments: one
+ * round) can be converted by a windowed kernel that keeps a ring of at most 8,192 measurement words
+ * per 32-shot tile in LDS instead of the whole record. kernel: 0 = windowed where it applies and needs
+ * less than half the LDS (default), 1 = whole-record kernel, 2 = windowed (conversions it does not
+ * apply to then fail with MFS_ERR_INVALID_ARGUMENT). */
+int mfs_m2d_set_kernel(mfs_m2d *c, int kernel);
+/* out[0..1] = ring words of the windowed kernel with observables appended / with detectors only in
+ * the output rows (0: not applicable), out[2] = LDS bytes per tile of the whole-record kernel,
+ * out[3] = of the windowed kernel (appended layout; 0 when not applicable). */
+int mfs_m2d_kernel_info(const mfs_m2d *c, uint64_t out[4]);
 
 /* Reference sample of a circuit (host only, no GPU): the noiseless run with every random measurement
  * result taken as 0, written bit-packed little-endian into out[0 .. ceil(num_measurements / 8)).

@@ -32,7 +32,7 @@ EXPORTED = [
     'mfs_dem_compile', 'mfs_dem_parse_stats', 'mfs_reference_sample', 'mfs_compile_measurements',
     'mfs_m2d_compile', 'mfs_m2d_destroy', 'mfs_m2d_stats', 'mfs_m2d_convert', 'mfs_m2d_bench',
     'mfs_set_stream_position', 'mfs_get_stream_position', 'mfs_rng_counter', 'mfs_philox', 'mfs_build_id',
-    'mfs_sample_noalloc', 'mfs_m2d_convert_sweep', 'mfs_dem_sample_errors', 'mfs_set_scan_variant',
+    'mfs_sample_noalloc', 'mfs_m2d_convert_sweep', 'mfs_m2d_set_kernel', 'mfs_m2d_kernel_info', 'mfs_dem_sample_errors', 'mfs_set_scan_variant',
     'mfs_expand_plan_stats', 'mfs_expand_shots_host',
 ]
 
@@ -70,6 +70,8 @@ def lib():
     L.mfs_m2d_convert.argtypes = [p, u64, p, sz, u32, p, sz, p, sz]
     L.mfs_m2d_convert_sweep.argtypes = [p, u64, p, sz, p, sz, u32, p, sz, p, sz]
     L.mfs_m2d_bench.argtypes = [p, u64, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    L.mfs_m2d_set_kernel.argtypes = [p, ctypes.c_int]
+    L.mfs_m2d_kernel_info.argtypes = [p, ctypes.POINTER(u64)]
     L.mfs_compile_measurements.argtypes = [ctypes.c_char_p, sz, u64, ctypes.c_int, ctypes.c_int, p, sz,
                                            ctypes.POINTER(p)]
     L.mfs_sample.argtypes = [p, u64, u32, p, sz, p, sz]

@@ -14,6 +14,8 @@
 namespace sa {
 hipError_t launch_m2d(const M2dParams &P, hipStream_t stream);
 size_t m2d_lds_bytes(uint32_t blocks, uint32_t sweep_blocks);
+hipError_t launch_m2d_win(const M2dParams &P, const M2dWindow &Wn, hipStream_t stream);
+size_t m2d_win_lds_bytes(uint32_t ring_words, uint32_t nchunks);
 }  // namespace sa
 
 namespace {
@@ -136,6 +138,46 @@ std::vector<uint8_t> sweep_effect_rows(const Block &block, uint32_t K, uint32_t 
     return rows;
 }
 
+// Largest ring of the windowed kernel (words): 32 KB of LDS, with the staging image about 41 KB per
+// 256-lane workgroup.
+constexpr uint32_t MAX_RING_WORDS = 8192;
+
+// Window of one output layout: part 0 = columns [0, n0), emitted block by block as their terms arrive;
+// columns [n0, cols) (separate observables) after the last chunk. ring_words = 0 when some column's
+// terms lie further apart than MAX_RING_WORDS allows.
+struct WinPlan {
+    uint32_t ring_words = 0;
+    std::vector<uint32_t> chunk_end;   // per chunk: part-0 blocks whose last term it holds, cumulative
+};
+
+WinPlan plan_window(const std::vector<std::vector<uint32_t>> &col_terms, uint32_t n0, uint32_t M) {
+    WinPlan W;
+    const uint32_t nchunks = (M + M2D_WIN_CHUNK - 1) / M2D_WIN_CHUNK;
+    uint64_t need = M2D_WIN_CHUNK;   // a ring holds at least one chunk
+    uint32_t prev = 0;
+    W.chunk_end.assign(std::max<uint32_t>(nchunks, 1), 0);
+    for (uint32_t b = 0; b * 32 < n0; b++) {
+        uint32_t hi = 0;
+        for (uint32_t j = 32 * b; j < std::min(n0, 32 * b + 32); j++) {
+            for (uint32_t t : col_terms[j]) hi = std::max(hi, t / M2D_WIN_CHUNK);
+        }
+        prev = std::max(prev, hi);
+        W.chunk_end[prev] = b + 1;
+        // After chunk `prev` is loaded the ring holds results [(prev + 1) * CHUNK - ring, (prev + 1) * CHUNK).
+        for (uint32_t j = 32 * b; j < std::min(n0, 32 * b + 32); j++) {
+            for (uint32_t t : col_terms[j]) need = std::max<uint64_t>(need, (uint64_t)(prev + 1) * M2D_WIN_CHUNK - t);
+        }
+    }
+    for (size_t j = n0; j < col_terms.size(); j++) {
+        for (uint32_t t : col_terms[j]) need = std::max<uint64_t>(need, (uint64_t)nchunks * M2D_WIN_CHUNK - t);
+    }
+    uint32_t ring = M2D_WIN_CHUNK;
+    while (ring < need && ring <= MAX_RING_WORDS / 2) ring *= 2;
+    if (ring >= need && nchunks <= M2D_WIN_MAX_CHUNKS) W.ring_words = ring;
+    for (size_t ch = 1; ch < W.chunk_end.size(); ch++) W.chunk_end[ch] = std::max(W.chunk_end[ch], W.chunk_end[ch - 1]);
+    return W;
+}
+
 }  // namespace
 
 struct mfs_m2d {
@@ -144,6 +186,13 @@ struct mfs_m2d {
     std::vector<uint2> cols;
     std::vector<uint32_t> ovf;
     DevBuf d_cols, d_ovf, d_in, d_out, d_obs, d_tmp, d_sweep;
+    // Windowed kernel (no sweep bits): layout 0 = observables appended to part 0, layout 1 = detectors
+    // only in part 0 (observables separate or absent). kernel_mode: 0 = the whole-record kernel where
+    // the tile's record fits in LDS, else windowed; 1 = whole-record kernel; 2 = windowed (refused
+    // where it does not apply).
+    WinPlan win[2];
+    DevBuf d_chunk_end[2];
+    int kernel_mode = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -184,11 +233,36 @@ struct mfs_m2d {
         return P;
     }
 
+    // Windowed layout usable for a launch, or -1.
+    int window_for(const M2dParams &P) const {
+        if (K || !P.in_vec || !M) return -1;
+        const int l = P.n_out == D + O ? 0 : 1;
+        if (l == 1 && P.n_out != D) return -1;
+        return win[l].ring_words ? l : -1;
+    }
+    bool whole_fits() const { return m2d_lds_bytes(blocks(), sweep_blocks()) <= MAX_LDS; }
+    bool use_window(const M2dParams &P) const {
+        if (kernel_mode == 1 || window_for(P) < 0) return false;
+        return kernel_mode == 2 || !whole_fits();
+    }
+
     void launch(const M2dParams &P) {
         if ((P.shots + 31) / 32 >= (1ull << 31)) throw std::invalid_argument("too many shots in one conversion");
         // The kernel addresses a tile's output rows with 32-bit offsets.
         if (P.out_stride >= (1ull << 30) || P.obs_stride >= (1ull << 30))
             throw std::invalid_argument("output row strides must be below 2^30 bytes");
+        if ((kernel_mode == 2 || !whole_fits()) && window_for(P) < 0)
+            throw std::invalid_argument(kernel_mode == 2
+                ? "the windowed conversion kernel does not apply to this circuit or layout"
+                : "the measurement record does not fit in LDS, and the windowed conversion kernel does not apply "
+                  "to this output layout (observables must be appended, or their measurements must come last)");
+        if (use_window(P)) {
+            const int l = window_for(P);
+            M2dWindow Wn{win[l].ring_words, (M + M2D_WIN_CHUNK - 1) / M2D_WIN_CHUNK,
+                         const_cast<DevBuf &>(d_chunk_end[l]).as<uint32_t>(), D + O};
+            check(launch_m2d_win(P, Wn, stream), "m2d launch");
+            return;
+        }
         check(launch_m2d(P, stream), "m2d launch");
     }
 };
@@ -210,13 +284,6 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
         c->D = (uint32_t)st.num_detectors;
         c->O = (uint32_t)st.num_observables;
         c->K = count_sweep_bits(flat);
-        if (m2d_lds_bytes(c->blocks(), c->sweep_blocks()) > MAX_LDS) {
-            uint32_t max_blocks = 0;
-            while (m2d_lds_bytes(max_blocks + 1, c->sweep_blocks()) <= MAX_LDS) max_blocks++;
-            throw std::invalid_argument("the converter keeps a tile's measurements and sweep bits in LDS: at most " +
-                                        std::to_string((uint64_t)max_blocks * 32) + " measurements per shot with " +
-                                        std::to_string(c->K) + " sweep bits");
-        }
         std::vector<uint8_t> ref(c->M, 0);
         if (!skip_reference_sample) ref = reference_sample(flat);
         // Columns: detectors in order, then observables; terms are absolute measurement indices.
@@ -236,6 +303,20 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
             m += num_results(*in);
         }
         for (auto &ts : obs_terms) col_terms.push_back(std::move(ts));
+        if (!c->K) {
+            c->win[0] = plan_window(col_terms, c->D + c->O, c->M);
+            c->win[1] = plan_window(col_terms, c->D, c->M);
+        }
+        // The whole-record kernel keeps a tile's measurements and sweep bits in LDS; the windowed one
+        // only a ring of recent results.
+        if (!c->whole_fits() && !c->win[0].ring_words && !c->win[1].ring_words) {
+            uint32_t max_blocks = 0;
+            while (m2d_lds_bytes(max_blocks + 1, c->sweep_blocks()) <= MAX_LDS) max_blocks++;
+            throw std::invalid_argument("the converter keeps a tile's measurements and sweep bits in LDS: at most " +
+                                        std::to_string((uint64_t)max_blocks * 32) + " measurements per shot with " +
+                                        std::to_string(c->K) + " sweep bits, unless every detector reaches back "
+                                        "at most about 7,000 results and the circuit has no sweep bits");
+        }
         int count = 0;
         if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
             throw HipError("no HIP device available: the stim_amd converter runs only on the GPU");
@@ -274,9 +355,15 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
             if (f) r.x |= M2D_INVERT;
             c->cols.push_back(r);
         }
+        // A zero record after the columns (no terms, no inversion): the windowed kernel loads it for
+        // lanes past the last column, so no loaded record needs a select.
+        c->cols.push_back(make_uint2(0, 0));
         check(hipStreamCreate(&c->stream), "stream");
         mfs_sampler::upload(c->d_cols, c->cols, "m2d columns");
         mfs_sampler::upload(c->d_ovf, c->ovf, "m2d overflow terms");
+        for (int l = 0; l < 2; l++) {
+            if (c->win[l].ring_words) mfs_sampler::upload(c->d_chunk_end[l], c->win[l].chunk_end, "m2d window");
+        }
         *out = c.release();
     });
 }
@@ -291,6 +378,26 @@ int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[4]) {
         out[1] = c->D;
         out[2] = c->O;
         out[3] = c->K;
+    });
+}
+
+int mfs_m2d_set_kernel(mfs_m2d *c, int kernel) {
+    return guarded([&] {
+        if (kernel < 0 || kernel > 2) throw std::invalid_argument("kernel must be 0 (auto), 1 (whole record) or 2 (windowed)");
+        if (kernel == 2 && !c->win[0].ring_words && !c->win[1].ring_words)
+            throw std::invalid_argument("the windowed conversion kernel does not apply to this circuit");
+        if (kernel == 1 && !c->whole_fits())
+            throw std::invalid_argument("the whole-record conversion kernel does not fit this circuit's record in LDS");
+        c->kernel_mode = kernel;
+    });
+}
+
+int mfs_m2d_kernel_info(const mfs_m2d *c, uint64_t out[4]) {
+    return guarded([&] {
+        out[0] = c->win[0].ring_words;
+        out[1] = c->win[1].ring_words;
+        out[2] = m2d_lds_bytes(c->blocks(), c->sweep_blocks());
+        out[3] = c->win[0].ring_words ? m2d_win_lds_bytes(c->win[0].ring_words, (c->M + M2D_WIN_CHUNK - 1) / M2D_WIN_CHUNK) : 0;
     });
 }
 

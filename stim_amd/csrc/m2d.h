@@ -68,6 +68,22 @@ struct ExpandDev {
 constexpr uint32_t M2D_INDEX_MASK = 0x0FFFFFFFu;
 constexpr uint32_t M2D_INVERT = 1u << 31;
 
+// Windowed conversion (circuits whose columns reach back a bounded number of results, the memory
+// experiments among them): a 256-lane workgroup streams its tile's measurements through a ring of
+// ring_words words (a power of two) a chunk of M2D_WIN_CHUNK measurements at a time, and emits each
+// block of 32 output columns once its last term has passed. Output blocks of part 0 (columns
+// [0, n_out)) are emitted in order: blocks [chunk_end[ch - 1], chunk_end[ch]) after chunk ch is loaded;
+// part 1 (the n_obs separate observable columns) after the last chunk. The host checks that every
+// block's terms are still in the ring when it is emitted. Without sweep bits only.
+constexpr uint32_t M2D_WIN_CHUNK = 1024;
+constexpr uint32_t M2D_WIN_MAX_CHUNKS = 1024;   // chunk_end is copied to LDS
+struct M2dWindow {
+    uint32_t ring_words;
+    uint32_t nchunks;           // ceil(M / M2D_WIN_CHUNK), at most M2D_WIN_MAX_CHUNKS
+    const uint32_t *chunk_end;  // per chunk: part-0 blocks emitted once it is loaded (non-decreasing)
+    uint32_t zero_rec;          // index in cols of a zero record (no terms, no inversion)
+};
+
 // LDS words of a tile: measurement blocks rounded up to whole 16-byte load groups (4 blocks of 32
 // words), then the sweep-bit blocks, then 4 words of zeros. Column terms are word indices: measurement
 // m is word m, sweep bit k is word m2d_sweep_word(blocks) + k.
@@ -83,5 +99,7 @@ __host__ __device__ inline uint32_t m2d_zero_word(uint32_t blocks, uint32_t swee
 
 // Dynamic LDS of a conversion or expansion launch: the tile's words plus the staged output chunk.
 size_t m2d_lds_bytes(uint32_t blocks, uint32_t sweep_blocks);
+// Dynamic LDS of a windowed launch: the ring, the staged output blocks and the chunk_end table.
+size_t m2d_win_lds_bytes(uint32_t ring_words, uint32_t nchunks);
 
 }  // namespace sa

@@ -58,6 +58,33 @@ __device__ __forceinline__ uint32_t column_word(const M2dParams &P, const uint32
     return (r.x & M2D_INVERT) ? ~v : v;
 }
 
+// Copies staged rows [0, valid) (pitch P bytes in LDS) to bytes [off, off + nbytes) of shot rows s0 + r
+// as 4-byte stores at any byte alignment (the HSA runtime enables unaligned global access; the
+// compiler emits global_store_dword for an align-1 word): lane w of a wave takes bytes [4w, 4w + 4),
+// so one store instruction writes 256 contiguous bytes of a row; the wave's rows step by NTHREADS /
+// 64. A tail of 1 to 3 bytes at the end of a row is written byte by byte.
+typedef uint32_t unaligned_u32 __attribute__((aligned(1)));
+template <uint32_t NTHREADS, uint32_t P>
+__device__ __forceinline__ void flush_words(const uint8_t *img, uint8_t *dst, uint64_t stride, uint32_t off,
+                                            uint32_t nbytes, uint64_t s0, uint32_t valid) {
+    constexpr uint32_t RG = NTHREADS / 64;
+    static_assert(P % 4 == 0 && NTHREADS % 64 == 0, "word-aligned staging rows, whole waves");
+    const uint32_t b = 4 * (threadIdx.x % 64), r0 = threadIdx.x / 64;
+    if (b >= nbytes) return;
+    uint8_t *const base = dst + s0 * stride + off + b;
+    const uint8_t *const src = img + b;
+    if (b + 4 <= nbytes) {
+        for (uint32_t r = r0; r < valid; r += RG) {
+            const uint32_t x = *reinterpret_cast<const uint32_t *>(src + r * P);
+            __builtin_nontemporal_store(x, reinterpret_cast<unaligned_u32 *>(base + r * stride));
+        }
+    } else {
+        for (uint32_t r = r0; r < valid; r += RG) {
+            for (uint32_t k = 0; b + k < nbytes; k++) __builtin_nontemporal_store(src[r * P + k], base + r * stride + k);
+        }
+    }
+}
+
 // Copies the staged chunk to the rows: staged row r's bytes [0, nbytes) go to bytes [off, off + nbytes)
 // of shot row s0 + r, with coalesced byte stores (rows need no alignment). Thread t copies byte
 // t % CHUNK_BYTES of rows t / CHUNK_BYTES + k * ROWS_PER_PASS: per pass the row base is the same for
@@ -113,7 +140,7 @@ __device__ __forceinline__ void emit_columns(const M2dParams &P, const uint32_t 
         __syncthreads();
         const uint32_t off = 4 * cb0;
         const uint32_t nbytes = pitch - off < CHUNK_BYTES ? pitch - off : CHUNK_BYTES;
-        flush_chunk(img, dst, stride, off, nbytes, s0, valid);
+        flush_words<NT, PITCH>(img, dst, stride, off, nbytes, s0, valid);
         __syncthreads();
     }
 }
@@ -266,9 +293,186 @@ __global__ __launch_bounds__(NT) void expand_kernel(ExpandDev E) {
     emit_columns(P, words, img, P.out, P.out_stride, P.n_out, 0, half, lane, s0, valid);
 }
 
+// ------------------------------------------------------------------ windowed conversion
+constexpr int NTW = 256;
+constexpr uint32_t WIN_HALVES = NTW / 32;
+constexpr uint32_t WIN_STAGE_BLOCKS = 64;                 // output blocks staged per flush (256 bytes per row)
+constexpr uint32_t WIN_PITCH = WIN_STAGE_BLOCKS * 4 + 4;  // LDS bytes per staged row
+
+// A column word from the ring: term m is ring word m & mask; overflow padding (an index >= M) is 0.
+__device__ __forceinline__ uint32_t ring_column_word(const M2dParams &P, const uint32_t *ring, uint32_t mask,
+                                                     uint32_t M, uint2 r) {
+    const uint32_t c = (r.x >> 28) & 3u;
+    uint32_t v;
+    if (c == 3) {
+        v = 0;
+        const uint32_t n = r.x & M2D_INDEX_MASK;
+        const uint4 *q = reinterpret_cast<const uint4 *>(P.ovf + r.y);
+        for (uint32_t t = 0; t < n; t += 4) {
+            const uint4 a = q[t / 4];
+            v ^= (a.x < M ? ring[a.x & mask] : 0u) ^ (a.y < M ? ring[a.y & mask] : 0u) ^
+                 (a.z < M ? ring[a.z & mask] : 0u) ^ (a.w < M ? ring[a.w & mask] : 0u);
+        }
+    } else {
+        const uint32_t w0 = ring[(r.x & M2D_INDEX_MASK) & mask], w1 = ring[r.y & mask];
+        v = (c >= 1 ? w0 : 0u) ^ (c == 2 ? w1 : 0u);
+    }
+    return (r.x & M2D_INVERT) ? ~v : v;
+}
+
+// Writes staged blocks [0, nb) (bytes [0, nbytes) of each staged row) to bytes [off, off + nbytes) of
+// the tile's rows: thread t copies byte t of every row (coalesced across the workgroup per row).
+__device__ __forceinline__ void win_flush(const uint8_t *img, uint8_t *dst, uint64_t stride, uint32_t off,
+                                          uint32_t nbytes, uint64_t s0, uint32_t valid) {
+    const uint32_t c = threadIdx.x;
+    if (c >= nbytes) return;
+    uint8_t *base = dst + s0 * stride + off + c;
+    for (uint32_t r = 0; r < valid; r++) __builtin_nontemporal_store(img[r * WIN_PITCH + c], base + r * stride);
+}
+
+// Output blocks are staged a pass at a time: up to WIN_PASS blocks, block b + u * WIN_HALVES + half
+// on half-wave `half` for u < WIN_UNROLL, each lane producing one column of its block.
+constexpr uint32_t WIN_UNROLL = 4;
+constexpr uint32_t WIN_PASS = WIN_HALVES * WIN_UNROLL;
+
+// Column records of a pass of n blocks from block b; lanes past n or past the part's columns load the
+// zero record (an address select, so the loads stay in flight until the records are used).
+__device__ __forceinline__ void win_records(const M2dParams &P, uint2 (&r)[WIN_UNROLL], uint32_t ncols,
+                                            uint32_t col0, uint32_t zero_rec, uint32_t b, uint32_t n) {
+    const uint32_t half = threadIdx.x >> 5, lane = threadIdx.x & 31;
+#pragma unroll
+    for (uint32_t u = 0; u < WIN_UNROLL; u++) {
+        const uint32_t i = u * WIN_HALVES + half;
+        const uint32_t c = (b + i) * 32 + lane;
+        r[u] = P.cols[(i < n && c < ncols) ? col0 + c : zero_rec];
+    }
+}
+
+// Emission state of one part: `staged` blocks in the staging image since the last flush, `flushed`
+// blocks already written to the rows.
+struct WinPart {
+    uint8_t *dst;
+    uint64_t stride;
+    uint32_t ncols, col0, flushed, staged;
+};
+
+// Blocks of the next pass from block b of [b, b1): bounded by the staging room and WIN_PASS.
+__device__ __forceinline__ uint32_t win_pass_blocks(const WinPart &W, uint32_t b, uint32_t b1) {
+    const uint32_t room = WIN_STAGE_BLOCKS - W.staged, left = b1 - b;
+    const uint32_t n = left < room ? left : room;
+    return n < WIN_PASS ? n : WIN_PASS;
+}
+
+// Emits blocks [b, b1) of a part, the first pass from the records r (loaded by the caller for the
+// pass win_pass_blocks gives), and flushes the staging image whenever it fills and, with final_flush,
+// at the end. Uniform across the workgroup.
+__device__ __forceinline__ void win_emit(const M2dParams &P, const uint32_t *ring, uint32_t mask, uint32_t M,
+                                         uint32_t zero_rec, uint8_t *img, WinPart &W, uint32_t b, uint32_t b1,
+                                         uint2 (&r)[WIN_UNROLL], uint64_t s0, uint32_t valid, bool final_flush) {
+    const uint32_t half = threadIdx.x >> 5, lane = threadIdx.x & 31;
+    const uint32_t pitch = (W.ncols + 7) / 8;
+    uint32_t *const slot = reinterpret_cast<uint32_t *>(img + lane * WIN_PITCH);
+    bool first = true;
+    while (b < b1 || (final_flush && W.staged)) {
+        const uint32_t n = win_pass_blocks(W, b, b1);
+        if (!first) win_records(P, r, W.ncols, W.col0, zero_rec, b, n);
+        first = false;
+        // Every lane of a half-wave takes part in the transposes; the zero record yields 0.
+#pragma unroll
+        for (uint32_t u = 0; u < WIN_UNROLL; u++) {
+            const uint32_t i = u * WIN_HALVES + half;
+            const uint32_t x = transpose32(ring_column_word(P, ring, mask, M, r[u]), lane);
+            if (i < n) slot[W.staged + i] = x;
+        }
+        b += n;
+        W.staged += n;
+        if (W.staged == WIN_STAGE_BLOCKS || (final_flush && b == b1)) {
+            __syncthreads();
+            const uint32_t off = 4 * W.flushed;
+            const uint32_t nbytes = pitch - off < 4 * W.staged ? pitch - off : 4 * W.staged;
+            flush_words<NTW, WIN_PITCH>(img, W.dst, W.stride, off, nbytes, s0, valid);
+            __syncthreads();
+            W.flushed += W.staged;
+            W.staged = 0;
+        }
+    }
+}
+
+// Windowed conversion of a 32-shot tile (semantics in m2d.h). Per chunk: its 16-byte groups (one per
+// half-wave, loaded during the previous chunk) go into the ring; the first pass's column records are
+// loaded, then the next chunk's groups (loads complete in order, so waiting for the records leaves
+// the row load in flight); then the chunk's ready blocks are emitted.
+__global__ __launch_bounds__(NTW) void m2d_win_kernel(M2dParams P, M2dWindow Wn) {
+    extern __shared__ __align__(16) uint32_t ring[];
+    const uint32_t tid = threadIdx.x, half = tid >> 5, lane = tid & 31;
+    const uint64_t s0 = (uint64_t)blockIdx.x * 32;
+    const uint64_t left = P.shots - s0;
+    const uint32_t valid = left < 32 ? (uint32_t)left : 32u;
+    const bool have = lane < valid;
+    // Lanes past the tile's shots read the tile's first row; their words are never used.
+    const uint4 *row = reinterpret_cast<const uint4 *>(P.in + (s0 + (have ? lane : 0)) * P.in_stride);
+    const uint32_t mask = Wn.ring_words - 1, M = P.blocks * 32;
+    const uint32_t ngrp = (P.blocks + 3) / 4;
+    uint8_t *const img = reinterpret_cast<uint8_t *>(ring + Wn.ring_words);
+    uint32_t *const chunk_end = reinterpret_cast<uint32_t *>(img + 32 * WIN_PITCH);
+    for (uint32_t i = tid; i < Wn.nchunks; i += NTW) chunk_end[i] = Wn.chunk_end[i];
+    const uint32_t nb0 = (P.n_out + 31) / 32;
+    constexpr uint32_t GPC = M2D_WIN_CHUNK / 128;   // 16-byte groups per chunk, one per half-wave
+    static_assert(GPC == WIN_HALVES, "one group per half-wave and chunk");
+    // Loads past the row's last group re-read it (their words are not stored).
+    uint4 v = row[half < ngrp ? half : ngrp - 1];
+    WinPart W0{P.out, P.out_stride, P.n_out, 0, 0, 0};
+    uint32_t next0 = 0;
+    uint2 r[WIN_UNROLL];
+    for (uint32_t ch = 0; ch < Wn.nchunks; ch++) {
+        const uint32_t g = ch * GPC + half;
+        __syncthreads();   // the previous chunk's columns are read before its ring slots are reused
+        {
+            const uint32_t xs[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (uint32_t k = 0; k < 4; k++) {
+                const uint32_t x = transpose32(xs[k], lane);
+                if (g < ngrp) ring[((4 * g + k) * 32 + lane) & mask] = x;
+            }
+        }
+        const uint32_t b1 = chunk_end[ch];
+        win_records(P, r, W0.ncols, 0, Wn.zero_rec, next0, win_pass_blocks(W0, next0, b1));
+        const uint32_t gn = g + GPC;
+        v = row[gn < ngrp ? gn : ngrp - 1];
+        __syncthreads();
+        if (b1 > next0) win_emit(P, ring, mask, M, Wn.zero_rec, img, W0, next0, b1, r, s0, valid, false);
+        next0 = b1;
+    }
+    if (P.n_out) {
+        win_records(P, r, W0.ncols, 0, Wn.zero_rec, next0, win_pass_blocks(W0, next0, nb0));
+        win_emit(P, ring, mask, M, Wn.zero_rec, img, W0, next0, nb0, r, s0, valid, true);
+    }
+    if (P.n_obs) {
+        WinPart W1{P.obs, P.obs_stride, P.n_obs, P.num_det, 0, 0};
+        const uint32_t nb1 = (P.n_obs + 31) / 32;
+        win_records(P, r, W1.ncols, W1.col0, Wn.zero_rec, 0, win_pass_blocks(W1, 0, nb1));
+        win_emit(P, ring, mask, M, Wn.zero_rec, img, W1, 0, nb1, r, s0, valid, true);
+    }
+}
+
 }  // namespace
 
 namespace sa {
+
+size_t m2d_win_lds_bytes(uint32_t ring_words, uint32_t nchunks) {
+    return (size_t)ring_words * 4 + 32 * WIN_PITCH + ((size_t)nchunks + 3) / 4 * 16;
+}
+
+hipError_t launch_m2d_win(const M2dParams &P, const M2dWindow &Wn, hipStream_t stream) {
+    const uint64_t tiles = (P.shots + 31) / 32;
+    if (tiles == 0) return hipSuccess;
+    const size_t lds = m2d_win_lds_bytes(Wn.ring_words, Wn.nchunks);
+    static size_t reserved = 0;
+    const hipError_t e = reserve_lds(reinterpret_cast<const void *>(&m2d_win_kernel), lds, reserved);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(m2d_win_kernel, dim3((uint32_t)tiles), dim3(NTW), lds, stream, P, Wn);
+    return hipGetLastError();
+}
 
 hipError_t launch_expand(const ExpandDev &E, hipStream_t stream) {
     const uint64_t tiles = (E.out.shots + 31) / 32;

@@ -130,6 +130,22 @@ class CompiledMeasurementsToDetectionEventsConverter:
         if separate:
             formats.write_rows(os.fspath(obs_out_filepath), obs_out_format, obs, [('L', O)])
 
+    def set_kernel(self, kernel: str) -> None:
+        """'auto' (windowed where it applies and needs less than half the LDS), 'whole' (the tile's whole
+        record in LDS) or 'windowed' (a ring of recent results; conversions it does not apply to then
+        raise). Results are identical; this only selects the GPU kernel."""
+        modes = {'auto': 0, 'whole': 1, 'windowed': 2}
+        if kernel not in modes:
+            raise ValueError(f'kernel must be one of {sorted(modes)}')
+        N.check(N.lib().mfs_m2d_set_kernel(self._h, modes[kernel]))
+
+    def kernel_info(self) -> dict:
+        """Ring words of the windowed kernel per output layout (0: not applicable) and LDS bytes per tile."""
+        out = (ctypes.c_uint64 * 4)()
+        N.check(N.lib().mfs_m2d_kernel_info(self._h, out))
+        return {'ring_words_appended': out[0], 'ring_words_detectors': out[1], 'lds_whole': out[2],
+                'lds_windowed': out[3]}
+
     def bench_device(self, shots: int, warmup: int = 2, iters: int = 5) -> float:
         """Average kernel time (ms) of converting `shots` device-resident rows (observables appended)."""
         out = (ctypes.c_double * 1)()

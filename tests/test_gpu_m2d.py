@@ -57,6 +57,93 @@ def test_random_data_matches_oracle(golden, name):
     np.testing.assert_array_equal(conv.convert(measurements=meas, append_observables=True), mo.convert(text, meas, ref))
 
 
+def test_windowed_kernel_matches_reference_conversions(golden):
+    """The windowed kernel (a ring of recent results per tile) forced on every vector circuit it
+    applies to, bit-exact against `stim m2d` (the d=25 circuit among them)."""
+    n, saw_d25 = 0, False
+    for c in golden.load('m2d_vectors.json'):
+        text = c['circuit'] or golden.circuit(c['name'])
+        conv = _converter(text, c['skip_reference_sample'])
+        info = conv.kernel_info()
+        if not info['ring_words_appended'] or not info['ring_words_detectors']:
+            continue
+        conv.set_kernel('windowed')
+        M, D, O = conv.num_measurements, conv.num_detectors, conv.num_observables
+        meas = _rows(c['measurements'], c['shots'], M)
+        want = _rows(c['expected'], c['shots'], D + O)
+        np.testing.assert_array_equal(conv.convert(measurements=meas, append_observables=True), want,
+                                      err_msg=c['name'])
+        d, o = conv.convert(measurements=meas, separate_observables=True)
+        np.testing.assert_array_equal(d, want[:, :D], err_msg=c['name'])
+        np.testing.assert_array_equal(o, want[:, D:], err_msg=c['name'])
+        n += 1
+        saw_d25 |= c['name'] == 'surface_d25_r25'
+    assert n >= 20 and saw_d25
+
+
+@pytest.mark.parametrize('name', ['surface_d5_r5', 'rep_d7_r5', 'color_d5_r3', 'surface_d25_r25'])
+def test_windowed_kernel_equals_whole_record_kernel(golden, name):
+    """Both kernels on the same random rows, every output layout, a ragged last tile."""
+    text = golden.circuit(name)
+    conv = _converter(text)
+    info = conv.kernel_info()
+    assert info['ring_words_appended'] and info['ring_words_detectors'], info
+    assert info['lds_windowed'] <= 8192 * 4 + 32 * 260
+    M = conv.num_measurements
+    shots = 1000 if M > 4000 else 4099
+    rng = np.random.default_rng(len(name))
+    packed = np.packbits(rng.random((shots, M)) < 0.3, axis=1, bitorder='little')
+    got = {}
+    for kernel in ('whole', 'windowed'):
+        conv.set_kernel(kernel)
+        got[kernel] = (conv.convert(measurements=packed, bit_packed=True, append_observables=True),
+                       conv.convert(measurements=packed, bit_packed=True, separate_observables=True),
+                       conv.convert(measurements=packed, bit_packed=True, append_observables=False))
+    for a, b in zip(got['whole'], got['windowed']):
+        for x, y in zip(a if isinstance(a, tuple) else (a,), b if isinstance(b, tuple) else (b,)):
+            np.testing.assert_array_equal(x, y)
+    assert got['whole'][0].any()
+
+
+def test_records_longer_than_lds_use_the_windowed_kernel():
+    """d=9, 600 rounds: 48,081 measurements per shot do not fit one tile's LDS, so only the windowed
+    kernel converts them; against the oracle, every output layout, a ragged last tile."""
+    import stim_amd
+    from stim_amd import gen
+    text = gen.generate('surface_code:rotated_memory_z', distance=9, rounds=600, after_clifford_depolarization=1e-3,
+                        before_measure_flip_probability=1e-3)
+    conv = _converter(text)
+    M, D = conv.num_measurements, conv.num_detectors
+    assert M == 48081 and D == 48000
+    info = conv.kernel_info()
+    assert info['lds_whole'] > 160 * 1024 and info['ring_words_appended'] and info['ring_words_detectors'], info
+    with pytest.raises(ValueError, match='whole-record'):
+        conv.set_kernel('whole')
+    rng = np.random.default_rng(21)
+    meas = rng.random((75, M)) < 0.3
+    want = mo.convert(text, meas, stim_amd.Circuit(text).reference_sample())
+    np.testing.assert_array_equal(conv.convert(measurements=meas, append_observables=True), want)
+    d, o = conv.convert(measurements=meas, separate_observables=True)
+    np.testing.assert_array_equal(d, want[:, :D])
+    np.testing.assert_array_equal(o, want[:, D:])
+
+
+def test_windowed_kernel_refused_where_it_does_not_apply(golden):
+    # A detector reaching back over more than the largest ring.
+    text = 'M 0\n' + 'M 1\n' * 9000 + 'DETECTOR rec[-1] rec[-9001]\n'
+    conv = _converter(text)
+    assert conv.kernel_info()['ring_words_appended'] == 0
+    with pytest.raises(ValueError, match='windowed'):
+        conv.set_kernel('windowed')
+    conv.set_kernel('auto')
+    meas = np.zeros((3, conv.num_measurements), dtype=bool)
+    meas[1, 0] = True
+    np.testing.assert_array_equal(conv.convert(measurements=meas, append_observables=False)[:, 0], [False, True, False])
+    # Sweep bits keep the whole-record kernel.
+    sw = _converter('CX sweep[0] 0\nM 0\nDETECTOR rec[-1]\n')
+    assert sw.kernel_info()['ring_words_appended'] == 0
+
+
 def test_device_rows_aligned_and_unaligned(golden):
     import torch
     from stim_amd import _native as N
