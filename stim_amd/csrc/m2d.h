@@ -42,7 +42,8 @@ struct M2dParams {
 // and table layouts in expand.h). One workgroup expands a tile of 32 shots: its measurement words start
 // at zero, free results get random words, each y column's word is XORed into its slots, the
 // substitution phases run (each thread its own items, a barrier between phases), and the words leave
-// through the identity columns of `out` (column m = word m, inverted where the reference sample is 1).
+// as the columns of `out` (column m = word m, inverted where the reference sample is 1: bit m % 32 of
+// inv[m / 32]).
 struct ExpandDev {
     const uint8_t *in;          // y rows: detection events then observable flips, bit packed; 16-byte
     uint64_t in_stride;         // aligned rows with a stride that is a multiple of 16
@@ -57,7 +58,8 @@ struct ExpandDev {
     const uint32_t *terms;
     uint64_t seed;              // Philox key of the free results
     uint64_t tile0;             // 32-shot tile of the launch's first shot in the sampler's stream
-    M2dParams out;              // output: M identity columns over the tile's measurement words
+    const uint32_t *inv;        // ceil(M / 32) inversion words, zero past the last measurement
+    M2dParams out;              // output rows (out, out_stride, n_out = M, shots, blocks)
 };
 
 // Column record: bits 28..29 of x = term count class c, bit 31 = the column's reference parity

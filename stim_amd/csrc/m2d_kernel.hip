@@ -201,6 +201,35 @@ __global__ __launch_bounds__(NT) void m2d_kernel(M2dParams P) {
     if (P.n_obs) emit_columns(P, words, img, P.obs, P.obs_stride, P.n_obs, P.num_det, half, lane, s0, valid);
 }
 
+// The expansion's output: column m of the rows is measurement word m, inverted where the reference
+// sample is 1. Column inversion flips bit m of every shot's row, which after the transpose is one XOR
+// of the block's 32 inversion bits (inv[b], the same for every lane) into each lane's output word, so
+// no column records are read. Words past the last measurement are zero, as are their inversion bits.
+__device__ __forceinline__ void emit_identity(const uint32_t *words, const uint32_t *inv, uint8_t *img, uint8_t *dst,
+                                              uint64_t stride, uint32_t ncols, uint32_t half, uint32_t lane,
+                                              uint64_t s0, uint32_t valid) {
+    constexpr uint32_t CB = CHUNK_BLOCKS / HALVES;
+    const uint32_t nb = (ncols + 31) / 32, pitch = (ncols + 7) / 8;
+    uint32_t *const slot = reinterpret_cast<uint32_t *>(img + lane * PITCH);
+    for (uint32_t cb0 = 0; cb0 < nb; cb0 += CHUNK_BLOCKS) {
+        uint32_t w[CB], iv[CB];
+#pragma unroll
+        for (uint32_t u = 0; u < CB; u++) {
+            const uint32_t b = cb0 + half * CB + u;
+            const uint32_t bc = b < nb ? b : nb - 1;
+            w[u] = words[bc * 32 + lane];
+            iv[u] = inv[bc];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < CB; u++) slot[half * CB + u] = transpose32(w[u], lane) ^ iv[u];
+        __syncthreads();
+        const uint32_t off = 4 * cb0;
+        const uint32_t nbytes = pitch - off < CHUNK_BYTES ? pitch - off : CHUNK_BYTES;
+        flush_words<NT, PITCH>(img, dst, stride, off, nbytes, s0, valid);
+        __syncthreads();
+    }
+}
+
 // Random words of the free results: Philox4x32-10 keyed by the sampler's expansion key, counter (group of
 // four free results, 64-bit tile index, stream tag).
 constexpr uint32_t TAG_EXPAND = 7u << 28;
@@ -290,7 +319,7 @@ __global__ __launch_bounds__(NT) void expand_kernel(ExpandDev E) {
         __syncthreads();
     }
     uint8_t *const img = reinterpret_cast<uint8_t *>(words + zero + 4);
-    emit_columns(P, words, img, P.out, P.out_stride, P.n_out, 0, half, lane, s0, valid);
+    emit_identity(words, E.inv, img, P.out, P.out_stride, P.n_out, half, lane, s0, valid);
 }
 
 // ------------------------------------------------------------------ windowed conversion

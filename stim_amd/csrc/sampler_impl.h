@@ -171,7 +171,7 @@ struct mfs_sampler {
     size_t row_pitch_override = 0;        // symptom-engine rows padded to this pitch (0: packed width)
     uint64_t expand_key = 0, expand_tiles = 0;
     uint32_t expand_free = 0, expand_phases = 0, expand_items = 0;
-    DevBuf d_x_prim, d_x_extra_off, d_x_extra, d_x_free, d_x_sched, d_x_items, d_x_terms, d_x_cols, d_x_y;
+    DevBuf d_x_prim, d_x_extra_off, d_x_extra, d_x_free, d_x_sched, d_x_items, d_x_terms, d_x_inv, d_x_y;
     // Detector-error-model samplers: no circuit; the symptom tables come from the model's mechanisms.
     bool dem_mode = false;
     std::vector<uint32_t> dem_col;   // mechanism index (model order) -> symptom column
@@ -534,13 +534,12 @@ struct mfs_sampler {
         upload(d_x_sched, E.sched_off, "expansion schedule");
         upload(d_x_items, E.items.empty() ? std::vector<uint32_t>(4, 0) : E.items, "expansion items");
         upload(d_x_terms, nonempty(E.terms), "expansion terms");
-        // Output columns: measurement m is word m, inverted where the inversion (reference) bit is 1.
-        std::vector<uint2> cols(M);
+        // Output inversion: bit m of the words is the reference bit of measurement m (zero past M).
+        std::vector<uint32_t> inv((M + 31) / 32 + 1, 0);
         for (uint32_t m = 0; m < M; m++) {
-            const bool inv = m / 32 < invert_words.size() && ((invert_words[m / 32] >> (m % 32)) & 1u);
-            cols[m] = make_uint2((1u << 28) | m | (inv ? M2D_INVERT : 0u), 0u);
+            if (m / 32 < invert_words.size() && ((invert_words[m / 32] >> (m % 32)) & 1u)) inv[m / 32] |= 1u << (m % 32);
         }
-        upload(d_x_cols, cols, "expansion output columns");
+        upload(d_x_inv, inv, "expansion output inversion");
         expand_free = (uint32_t)E.free_slots.size();
         expand_phases = E.phases;
         expand_items = (uint32_t)(E.items.size() / 4);
@@ -605,8 +604,7 @@ struct mfs_sampler {
         P.n_out = M;
         P.n_obs = 0;
         P.shots = n;
-        P.cols = d_x_cols.as<uint2>();
-        P.ovf = d_x_prim.as<uint32_t>();   // never read: every column has one term
+        E.inv = d_x_inv.as<uint32_t>();
         return E;
     }
 
