@@ -85,33 +85,6 @@ __device__ __forceinline__ void flush_words(const uint8_t *img, uint8_t *dst, ui
     }
 }
 
-// Copies the staged chunk to the rows: staged row r's bytes [0, nbytes) go to bytes [off, off + nbytes)
-// of shot row s0 + r, with coalesced byte stores (rows need no alignment). Thread t copies byte
-// t % CHUNK_BYTES of rows t / CHUNK_BYTES + k * ROWS_PER_PASS: per pass the row base is the same for
-// the whole workgroup and each thread's offset from it is fixed, so a store costs no address
-// arithmetic. The offset (3 rows + 255 bytes) fits 32 bits because strides are below 2^30.
-constexpr uint32_t ROWS_PER_PASS = NT / CHUNK_BYTES;
-static_assert(NT % CHUNK_BYTES == 0 && 32 % ROWS_PER_PASS == 0, "flush passes cover whole tiles");
-
-__device__ __forceinline__ void flush_chunk(const uint8_t *img, uint8_t *dst, uint64_t stride, uint32_t off,
-                                            uint32_t nbytes, uint64_t s0, uint32_t valid) {
-    const uint32_t c = threadIdx.x % CHUNK_BYTES, r0 = threadIdx.x / CHUNK_BYTES;
-    if (c >= nbytes) return;
-    const uint32_t voff = r0 * (uint32_t)stride + c;
-    const uint8_t *const src = img + r0 * PITCH + c;
-    uint8_t *const base = dst + s0 * stride + off;
-    if (valid == 32) {
-#pragma unroll
-        for (uint32_t k = 0; k < 32 / ROWS_PER_PASS; k++) {
-            __builtin_nontemporal_store(src[k * ROWS_PER_PASS * PITCH], base + k * ROWS_PER_PASS * stride + voff);
-        }
-    } else {
-        for (uint32_t k = 0; r0 + k * ROWS_PER_PASS < valid; k++) {
-            __builtin_nontemporal_store(src[k * ROWS_PER_PASS * PITCH], base + k * ROWS_PER_PASS * stride + voff);
-        }
-    }
-}
-
 // Produces columns [c0, c0 + ncols) of the tile (ncols >= 1), transposed, into shot rows dst (bit 0 =
 // column c0), CHUNK_BLOCKS column blocks at a time: lane l of a half-wave gets shot l's 4 bytes of a
 // 32-column block and puts them in the chunk's LDS image (PITCH bytes per shot); the chunk is then
@@ -234,7 +207,43 @@ __device__ __forceinline__ void emit_identity(const uint32_t *words, const uint3
 // four free results, 64-bit tile index, stream tag).
 constexpr uint32_t TAG_EXPAND = 7u << 28;
 
-__global__ __launch_bounds__(NT) void expand_kernel(ExpandDev E) {
+// One substitution record (layout in expand.h).
+__device__ __forceinline__ void expand_record(const ExpandDev &E, uint32_t *words, uint4 r) {
+    uint32_t k = r.x & 0xFFFFu;
+    const uint32_t n = r.x >> 16;
+    uint32_t v = words[k];
+    if (n == 1) {
+        // A chain: each later result XORs the one before it, kept in v. Its results are distinct
+        // (stride w > 0) and only this record writes them, so the words it reads are the values
+        // before the chain: they are read 8 at a time ahead of the writes, which turns the chain's
+        // dependent LDS round trips into one per 8 steps.
+        v ^= words[r.y];
+        words[k] = v;
+        uint32_t j = 1;
+        for (; j + 8 <= r.z; j += 8) {
+            uint32_t t[8];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) t[u] = words[k + (u + 1) * r.w];
+#pragma unroll
+            for (uint32_t u = 0; u < 8; u++) {
+                v ^= t[u];
+                words[k + (u + 1) * r.w] = v;
+            }
+            k += 8 * r.w;
+        }
+        for (; j < r.z; j++) {
+            k += r.w;
+            v ^= words[k];
+            words[k] = v;
+        }
+    } else {
+        for (uint32_t t = 0; t < n; t++) v ^= words[E.terms[r.y + t]];
+        words[k] = v;
+    }
+}
+
+// Two workgroups per CU (their LDS words allow two): 8 waves per SIMD, at most 64 VGPRs.
+__global__ __launch_bounds__(NT, 8) void expand_kernel(ExpandDev E) {
     extern __shared__ __align__(16) uint32_t words[];
     const M2dParams &P = E.out;
     const uint32_t tid = threadIdx.x, half = tid >> 5, lane = tid & 31;
@@ -243,6 +252,9 @@ __global__ __launch_bounds__(NT) void expand_kernel(ExpandDev E) {
     const uint32_t valid = left < 32 ? (uint32_t)left : 32u;
     const bool have = lane < valid;
     const uint32_t zero = m2d_zero_word(P.blocks, 0);
+    // Phase 0's record range, loaded ahead (phase p + 1's is loaded while phase p runs).
+    // (sched_off holds nphases * NT + 1 offsets: no range without a phase.)
+    uint32_t na = E.nphases ? E.sched_off[tid] : 0u, nb = E.nphases ? E.sched_off[tid + 1] : 0u;
     for (uint32_t i = tid; i < zero + 4; i += NT) words[i] = 0;
     __syncthreads();
     // Free results (distinct from every contribution slot, so no ordering with the loads below).
@@ -292,30 +304,25 @@ __global__ __launch_bounds__(NT) void expand_kernel(ExpandDev E) {
             }
         }
     }
+    // Phase 0's first record (record 0 for an empty range; items holds at least one), loaded while
+    // the other waves finish the y pass.
+    uint4 nr = E.items[na < nb ? na : 0];
     __syncthreads();
     // Substitution: each thread runs its own items of a phase in record order (its chains need no
-    // barrier); items read results of earlier phases or of the same thread.
+    // barrier); items read results of earlier phases or of the same thread. Each phase's record range
+    // is loaded while the previous phase runs and its first record before the previous barrier.
     for (uint32_t p = 0; p < E.nphases; p++) {
-        const uint32_t a = E.sched_off[p * NT + tid], b = E.sched_off[p * NT + tid + 1];
-        for (uint32_t i = a; i < b; i++) {
-            const uint4 r = E.items[i];
-            uint32_t k = r.x & 0xFFFFu;
-            const uint32_t n = r.x >> 16;
-            uint32_t v = words[k];
-            if (n == 1) {
-                // A chain: each later result XORs the one before it, kept in v.
-                v ^= words[r.y];
-                words[k] = v;
-                for (uint32_t j = 1; j < r.z; j++) {
-                    k += r.w;
-                    v ^= words[k];
-                    words[k] = v;
-                }
-            } else {
-                for (uint32_t t = 0; t < n; t++) v ^= words[E.terms[r.y + t]];
-                words[k] = v;
-            }
+        const uint32_t a = na, b = nb;
+        const uint4 r0 = nr;
+        if (p + 1 < E.nphases) {
+            na = E.sched_off[(p + 1) * NT + tid];
+            nb = E.sched_off[(p + 1) * NT + tid + 1];
         }
+        if (a < b) {
+            expand_record(E, words, r0);
+            for (uint32_t i = a + 1; i < b; i++) expand_record(E, words, E.items[i]);
+        }
+        if (p + 1 < E.nphases) nr = E.items[na < nb ? na : 0];
         __syncthreads();
     }
     uint8_t *const img = reinterpret_cast<uint8_t *>(words + zero + 4);
