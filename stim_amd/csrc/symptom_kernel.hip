@@ -715,6 +715,17 @@ __global__ __launch_bounds__(NT, 5) void sym_scan_fast_kernel(SymDevParams P) {
     scan_tile_fast<OBS>(P, blockIdx.x, lds);
 }
 
+// A batch job's tile: the specialised scan for jobs whose sampler has outcome records (the job's
+// plain launch runs that kernel too; rows are identical either way), the generic scan otherwise.
+__device__ __forceinline__ void batch_tile(const SymDevParams &P, uint32_t btile, uint32_t *lds) {
+    if (P.orecs) {
+        if (P.obs_mode != 0) scan_tile_fast<true>(P, btile, lds);
+        else scan_tile_fast<false>(P, btile, lds);
+    } else {
+        scan_tile(P, btile, lds);
+    }
+}
+
 // Batched form: many launches (jobs, possibly of different circuits) in one kernel; job j owns global
 // tiles [prefix[j], prefix[j+1]). Persistent schedule (grid smaller than the tile count): workgroups
 // claim chunks of BATCH_CHUNK consecutive tiles from a counter, caching the current job so the search
@@ -740,7 +751,7 @@ __global__ __launch_bounds__(NT, 6) void sym_batch_kernel(const SymDevParams *__
         const uint32_t j = __builtin_amdgcn_readfirstlane(a);
         if (t < total) {
             const SymDevParams P = jobs[j];
-            scan_tile(P, (uint32_t)(t - prefix[j]), lds);
+            batch_tile(P, (uint32_t)(t - prefix[j]), lds);
         }
         return;
     }
@@ -767,7 +778,7 @@ __global__ __launch_bounds__(NT, 6) void sym_batch_kernel(const SymDevParams *__
             }
             if (t != t0) __syncthreads();   // the previous tile's rows are written before the image is reused
             const SymDevParams P = jobs[j];
-            scan_tile(P, (uint32_t)(t - lo), lds);
+            batch_tile(P, (uint32_t)(t - lo), lds);
         }
     }
 }
