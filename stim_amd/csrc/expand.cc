@@ -210,15 +210,18 @@ ExpandPlan plan_expansion(const std::vector<const Instr *> &flat, uint32_t M, ui
                 len++;
             }
         }
-        E.items.push_back(k | (n << 16));
+        uint32_t rec[EXPAND_RECORD_WORDS] = {k | (n << 16), 0, 0, 0, 0, 0, 0, 0};
         if (n == 1) {
-            E.items.push_back(x.R[0]);
+            rec[1] = x.R[0];
+            rec[2] = len;
+            rec[3] = stride;
+        } else if (n <= EXPAND_INLINE_TERMS) {
+            for (uint32_t t = 0; t < n; t++) rec[1 + t] = x.R[t];
         } else {
-            E.items.push_back((uint32_t)E.terms.size());
+            rec[1] = (uint32_t)E.terms.size();
             E.terms.insert(E.terms.end(), x.R.begin(), x.R.end() - 1);
         }
-        E.items.push_back(len);
-        E.items.push_back(stride);
+        E.items.insert(E.items.end(), rec, rec + EXPAND_RECORD_WORDS);
         E.sched_off[group + 1]++;
         i += len;
     }
@@ -240,7 +243,7 @@ void expand_shot_host(const ExpandPlan &E, const uint8_t *y, const uint8_t *free
     // Phases in order, threads in order within a phase: a topological order of the records.
     const size_t nrec = E.sched_off.empty() ? 0 : E.sched_off.back();
     for (size_t i = 0; i < nrec; i++) {
-        const uint32_t *r = &E.items[4 * i];
+        const uint32_t *r = &E.items[EXPAND_RECORD_WORDS * i];
         uint32_t k = r[0] & 0xFFFFu;
         const uint32_t n = r[0] >> 16;
         uint8_t v = m[k];
@@ -252,6 +255,9 @@ void expand_shot_host(const ExpandPlan &E, const uint8_t *y, const uint8_t *free
                 v ^= m[k];
                 m[k] = v;
             }
+        } else if (n <= EXPAND_INLINE_TERMS) {
+            for (uint32_t t = 0; t < n; t++) v ^= m[r[1 + t]];
+            m[k] = v;
         } else {
             for (uint32_t t = 0; t < n; t++) v ^= m[E.terms[r[1] + t]];
             m[k] = v;

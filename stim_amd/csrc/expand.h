@@ -33,6 +33,7 @@
 #include <vector>
 
 #include "circuit.h"
+#include "m2d.h"
 
 namespace sa {
 
@@ -46,11 +47,15 @@ struct ExpandPlan {
     // when bit 31 is set, into results extra[extra_off[j] .. extra_off[j+1]) as well.
     std::vector<uint32_t> prim, extra_off, extra;
     std::vector<uint32_t> free_slots;  // results drawn uniformly at random
-    // Substitution records, 4 words each: (x, y, z, w). With n = x >> 16 terms and k = x & 0xFFFF:
-    // n = 1: a chain of z results k, k + w, k + 2w, ...: the first is XORed with result y, each later one
-    // with the result before it in the chain (a check repeated round after round);
-    // n > 1: result k ^= the XOR of results terms[y .. y + n) (z = 1).
-    // Thread t runs records [sched_off[p * threads + t], sched_off[p * threads + t + 1]) in phase p.
+    // Substitution records, 8 words each (EXPAND_RECORD_WORDS). With n = word 0 >> 16 terms and
+    // k = word 0 & 0xFFFF:
+    // n = 1: a chain of (word 2) results k, k + w, k + 2w, ... (w = word 3): the first is XORed with
+    // result word 1, each later one with the result before it in the chain (a check repeated round
+    // after round);
+    // 2 <= n <= 7: result k ^= the XOR of results words 1 .. n (the terms inline);
+    // n >= 8: result k ^= the XOR of results terms[word 1 .. word 1 + n).
+    // Unused words are 0. Thread t runs records [sched_off[p * threads + t], sched_off[p * threads + t +
+    // 1]) in phase p.
     uint32_t threads = 0, phases = 0;
     std::vector<uint32_t> sched_off, items, terms;
     uint32_t levels = 0;               // substitution depth (longest dependency chain), for reports

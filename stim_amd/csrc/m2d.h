@@ -44,6 +44,11 @@ struct M2dParams {
 // substitution phases run (each thread its own items, a barrier between phases), and the words leave
 // as the columns of `out` (column m = word m, inverted where the reference sample is 1: bit m % 32 of
 // inv[m / 32]).
+// Substitution records of the expansion: 8 words (two uint4) each, up to 7 terms inline (layout in
+// expand.h).
+constexpr uint32_t EXPAND_RECORD_WORDS = 8;
+constexpr uint32_t EXPAND_INLINE_TERMS = 7;
+
 struct ExpandDev {
     const uint8_t *in;          // y rows: detection events then observable flips, bit packed; 16-byte
     uint64_t in_stride;         // aligned rows with a stride that is a multiple of 16
@@ -54,7 +59,7 @@ struct ExpandDev {
     uint32_t nfree;
     uint32_t nphases;
     const uint32_t *sched_off;  // nphases * 1024 + 1 record offsets (phase-major, then thread)
-    const uint4 *items;
+    const uint4 *items;         // two uint4 per record (8 words, layout in expand.h)
     const uint32_t *terms;
     uint64_t seed;              // Philox key of the free results
     uint64_t tile0;             // 32-shot tile of the launch's first shot in the sampler's stream

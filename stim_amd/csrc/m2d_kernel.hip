@@ -207,8 +207,8 @@ __device__ __forceinline__ void emit_identity(const uint32_t *words, const uint3
 // four free results, 64-bit tile index, stream tag).
 constexpr uint32_t TAG_EXPAND = 7u << 28;
 
-// One substitution record (layout in expand.h).
-__device__ __forceinline__ void expand_record(const ExpandDev &E, uint32_t *words, uint4 r) {
+// One substitution record (layout in expand.h): words r.x .. r.w, q.x .. q.w.
+__device__ __forceinline__ void expand_record(const ExpandDev &E, uint32_t *words, uint4 r, uint4 q) {
     uint32_t k = r.x & 0xFFFFu;
     const uint32_t n = r.x >> 16;
     uint32_t v = words[k];
@@ -236,6 +236,15 @@ __device__ __forceinline__ void expand_record(const ExpandDev &E, uint32_t *word
             v ^= words[k];
             words[k] = v;
         }
+    } else if (n <= EXPAND_INLINE_TERMS) {
+        // Terms inline: all seven read together (unused ones name result 0 and are masked).
+        const uint32_t t[EXPAND_INLINE_TERMS] = {r.y, r.z, r.w, q.x, q.y, q.z, q.w};
+        uint32_t w[EXPAND_INLINE_TERMS];
+#pragma unroll
+        for (uint32_t u = 0; u < EXPAND_INLINE_TERMS; u++) w[u] = words[t[u]];
+#pragma unroll
+        for (uint32_t u = 0; u < EXPAND_INLINE_TERMS; u++) v ^= u < n ? w[u] : 0u;
+        words[k] = v;
     } else {
         for (uint32_t t = 0; t < n; t++) v ^= words[E.terms[r.y + t]];
         words[k] = v;
@@ -306,23 +315,32 @@ __global__ __launch_bounds__(NT, 8) void expand_kernel(ExpandDev E) {
     }
     // Phase 0's first record (record 0 for an empty range; items holds at least one), loaded while
     // the other waves finish the y pass.
-    uint4 nr = E.items[na < nb ? na : 0];
+    uint4 nr = E.items[2 * (na < nb ? na : 0)], nq = E.items[2 * (na < nb ? na : 0) + 1];
     __syncthreads();
     // Substitution: each thread runs its own items of a phase in record order (its chains need no
     // barrier); items read results of earlier phases or of the same thread. Each phase's record range
     // is loaded while the previous phase runs and its first record before the previous barrier.
     for (uint32_t p = 0; p < E.nphases; p++) {
         const uint32_t a = na, b = nb;
-        const uint4 r0 = nr;
+        uint4 r = nr, q = nq;
         if (p + 1 < E.nphases) {
             na = E.sched_off[(p + 1) * NT + tid];
             nb = E.sched_off[(p + 1) * NT + tid + 1];
         }
-        if (a < b) {
-            expand_record(E, words, r0);
-            for (uint32_t i = a + 1; i < b; i++) expand_record(E, words, E.items[i]);
+        // A thread's records of a phase may depend on each other (through results in LDS only): the
+        // next record's load is issued before the current one runs.
+        for (uint32_t i = a; i < b; i++) {
+            const uint32_t nx = i + 1 < b ? i + 1 : i;
+            const uint4 r2 = E.items[2 * nx], q2 = E.items[2 * nx + 1];
+            expand_record(E, words, r, q);
+            r = r2;
+            q = q2;
         }
-        if (p + 1 < E.nphases) nr = E.items[na < nb ? na : 0];
+        if (p + 1 < E.nphases) {
+            const uint32_t f = na < nb ? na : 0;
+            nr = E.items[2 * f];
+            nq = E.items[2 * f + 1];
+        }
         __syncthreads();
     }
     uint8_t *const img = reinterpret_cast<uint8_t *>(words + zero + 4);

@@ -532,7 +532,7 @@ struct mfs_sampler {
         upload(d_x_extra, nonempty(E.extra), "expansion y slots");
         upload(d_x_free, nonempty(E.free_slots), "expansion free results");
         upload(d_x_sched, E.sched_off, "expansion schedule");
-        upload(d_x_items, E.items.empty() ? std::vector<uint32_t>(4, 0) : E.items, "expansion items");
+        upload(d_x_items, E.items.empty() ? std::vector<uint32_t>(EXPAND_RECORD_WORDS, 0) : E.items, "expansion items");
         upload(d_x_terms, nonempty(E.terms), "expansion terms");
         // Output inversion: bit m of the words is the reference bit of measurement m (zero past M).
         std::vector<uint32_t> inv((M + 31) / 32 + 1, 0);
@@ -542,7 +542,7 @@ struct mfs_sampler {
         upload(d_x_inv, inv, "expansion output inversion");
         expand_free = (uint32_t)E.free_slots.size();
         expand_phases = E.phases;
-        expand_items = (uint32_t)(E.items.size() / 4);
+        expand_items = (uint32_t)(E.items.size() / EXPAND_RECORD_WORDS);
         expand_key = splitmix64(key ^ 0xE8A7D5ull);
         det_src = std::move(d);
         expand_state = 1;
