@@ -185,45 +185,118 @@ ExpandPlan plan_expansion(const std::vector<const Instr *> &flat, uint32_t M, ui
         if (phase[a] != phase[b]) return phase[a] < phase[b];
         return owner[a] < owner[b];
     });
-    // Records: consecutive one-term items of one (phase, thread) whose term is the previous item's
-    // result, at a constant record distance, merge into one chain record.
-    E.sched_off.assign((size_t)E.phases * threads + 1, 0);
+    // Records. Consecutive items of one (phase, thread) whose terms include the previous item's
+    // result, at a constant record distance, form a run: each result is the previous one XOR its own
+    // value (its y contributions and its other terms). Where the later items have no other terms the
+    // run is one chain record. Where their other terms (and the first item's terms, when it has
+    // several) are final when the phase starts (free results, results without substitution, results
+    // of earlier phases), those terms are XORed into the run's results by independent records in a
+    // phase of their own just before, spread over the threads, and the run is again one chain record:
+    // its thread then walks the run reading 8 results per LDS round trip instead of one record per
+    // round trip. Otherwise each item is a record of its own.
+    auto final_at = [&](uint32_t t, uint32_t P) { return owner[t] < 0 || phase[t] < P; };
+    struct Rec {
+        uint32_t phase;    // the items' phase
+        bool pre;          // a pre-phase record (XORs final terms into a run's result)
+        uint32_t thread;
+        uint32_t w[EXPAND_RECORD_WORDS];
+        std::vector<uint32_t> ovf;   // terms of a record with more than EXPAND_INLINE_TERMS
+    };
+    std::vector<Rec> recs;
+    std::vector<uint8_t> has_pre(E.phases, 0);
+    auto add_record = [&](uint32_t P, bool pre, uint32_t thread, uint32_t k, const std::vector<uint32_t> &ts) {
+        const uint32_t n = (uint32_t)ts.size();
+        Rec r{P, pre, thread, {k | (n << 16), 0, 0, 0, 0, 0, 0, 0}, {}};
+        if (n == 1) {
+            r.w[1] = ts[0];   // a chain of one
+            r.w[2] = 1;
+        } else if (n <= EXPAND_INLINE_TERMS) {
+            for (uint32_t t = 0; t < n; t++) r.w[1 + t] = ts[t];
+        } else {
+            r.ovf = ts;
+        }
+        recs.push_back(std::move(r));
+    };
     size_t i = 0;
     while (i < item_list.size()) {
         const uint32_t k = item_list[i];
         const Reduced &x = red[(size_t)pivot_of[k]];
-        const uint32_t n = (uint32_t)x.R.size() - 1;
-        if (n >= (1u << 16)) {
+        const std::vector<uint32_t> ts(x.R.begin(), x.R.end() - 1);
+        if (ts.size() >= (1u << 16)) {
             E.why = "a substitution with more than 65,535 terms";
             return E;
         }
-        const size_t group = (size_t)phase[k] * threads + (uint32_t)owner[k];
+        const uint32_t P = phase[k], th = (uint32_t)owner[k];
+        const bool first_final = std::all_of(ts.begin(), ts.end(), [&](uint32_t t) { return final_at(t, P); });
         uint32_t len = 1, stride = 0;
-        if (n == 1) {
-            while (i + len < item_list.size()) {
-                const uint32_t k2 = item_list[i + len], prev = item_list[i + len - 1];
-                const Reduced &x2 = red[(size_t)pivot_of[k2]];
-                if ((size_t)phase[k2] * threads + (uint32_t)owner[k2] != group || x2.R.size() != 2 || x2.R[0] != prev)
-                    break;
-                if (len == 1) stride = k2 - prev;
-                else if (k2 - prev != stride) break;
-                len++;
+        bool side = false;
+        while (i + len < item_list.size()) {
+            const uint32_t k2 = item_list[i + len], prev = item_list[i + len - 1];
+            if (phase[k2] != P || (uint32_t)owner[k2] != th) break;
+            const Reduced &x2 = red[(size_t)pivot_of[k2]];
+            if (!std::binary_search(x2.R.begin(), x2.R.end() - 1, prev)) break;
+            if (len == 1) stride = k2 - prev;
+            else if (k2 - prev != stride) break;
+            if (x2.R.size() > 2) {
+                bool ok = ts.size() == 1 || first_final;
+                for (size_t t = 0; ok && t + 1 < x2.R.size(); t++) ok = x2.R[t] == prev || final_at(x2.R[t], P);
+                if (!ok) break;
+                side = true;
             }
+            len++;
         }
-        uint32_t rec[EXPAND_RECORD_WORDS] = {k | (n << 16), 0, 0, 0, 0, 0, 0, 0};
-        if (n == 1) {
-            rec[1] = x.R[0];
-            rec[2] = len;
-            rec[3] = stride;
-        } else if (n <= EXPAND_INLINE_TERMS) {
-            for (uint32_t t = 0; t < n; t++) rec[1 + t] = x.R[t];
-        } else {
-            rec[1] = (uint32_t)E.terms.size();
-            E.terms.insert(E.terms.end(), x.R.begin(), x.R.end() - 1);
+        if (len > 1 && ts.size() > 1 && !first_final) len = 1;   // side is false here: no pre-record for it
+        if (len == 1) {
+            add_record(P, false, th, k, ts);
+            i++;
+            continue;
         }
-        E.items.insert(E.items.end(), rec, rec + EXPAND_RECORD_WORDS);
-        E.sched_off[group + 1]++;
+        uint32_t y0 = ts[0];
+        if (ts.size() > 1) {
+            add_record(P, true, 0, k, ts);
+            y0 = M;   // a result that is always zero
+            has_pre[P] = 1;
+        }
+        for (uint32_t j = 1; side && j < len; j++) {
+            const uint32_t kj = item_list[i + j], prev = item_list[i + j - 1];
+            const Reduced &xj = red[(size_t)pivot_of[kj]];
+            std::vector<uint32_t> other;
+            for (size_t t = 0; t + 1 < xj.R.size(); t++) {
+                if (xj.R[t] != prev) other.push_back(xj.R[t]);
+            }
+            if (!other.empty()) add_record(P, true, 0, kj, other);
+            has_pre[P] = 1;
+        }
+        Rec c{P, false, th, {k | (1u << 16), y0, len, stride, 0, 0, 0, 0}, {}};
+        recs.push_back(std::move(c));
         i += len;
+    }
+    // Phases with pre-records get a phase of their own just before them; pre-records go round the
+    // threads of that phase.
+    std::vector<uint32_t> newp(E.phases, 0);
+    uint32_t shift = 0;
+    for (uint32_t P = 0; P < E.phases; P++) {
+        shift += has_pre[P];
+        newp[P] = P + shift;
+    }
+    E.phases += shift;
+    std::vector<uint32_t> rr(E.phases, 0);
+    for (Rec &r : recs) {
+        const uint32_t ph = newp[r.phase] - (r.pre ? 1u : 0u);
+        if (r.pre) r.thread = rr[ph]++ % threads;
+        r.phase = ph;
+    }
+    std::stable_sort(recs.begin(), recs.end(), [](const Rec &a, const Rec &b) {
+        return a.phase != b.phase ? a.phase < b.phase : a.thread < b.thread;
+    });
+    E.sched_off.assign((size_t)E.phases * threads + 1, 0);
+    for (Rec &r : recs) {
+        if (!r.ovf.empty()) {
+            r.w[1] = (uint32_t)E.terms.size();
+            E.terms.insert(E.terms.end(), r.ovf.begin(), r.ovf.end());
+        }
+        E.items.insert(E.items.end(), r.w, r.w + EXPAND_RECORD_WORDS);
+        E.sched_off[(size_t)r.phase * threads + r.thread + 1]++;
     }
     for (size_t g = 1; g < E.sched_off.size(); g++) E.sched_off[g] += E.sched_off[g - 1];
     E.complete = true;
@@ -248,7 +321,7 @@ void expand_shot_host(const ExpandPlan &E, const uint8_t *y, const uint8_t *free
         const uint32_t n = r[0] >> 16;
         uint8_t v = m[k];
         if (n == 1) {
-            v ^= m[r[1]];
+            v ^= r[1] < E.M ? m[r[1]] : 0;   // index M: the always-zero result
             m[k] = v;
             for (uint32_t j = 1; j < r[2]; j++) {
                 k += r[3];

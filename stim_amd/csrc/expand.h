@@ -18,8 +18,11 @@
 // workgroup and run in phases separated by barriers. An item joins the thread that owns its latest
 // dependency; its phase is the largest phase of its dependencies, plus one for each dependency owned by
 // another thread. A thread runs its items of a phase in record order, so a chain of dependencies on
-// one thread needs no barrier (a memory circuit's checks repeat round after round: one chain per check,
-// three phases at d=25 instead of 48 substitution levels). ker A = G is
+// one thread needs no barrier (a memory circuit's checks repeat round after round: one chain per check).
+// A run of items on one thread that each use the previous one's result becomes a chain record; the
+// items' other terms, where they are final when the phase starts, are XORed in beforehand by
+// independent records in a phase of their own (at d=25: 5 phases instead of 48 substitution levels,
+// no thread with more than 2 records in a phase). ker A = G is
 // checked by dimension. G lies in ker A (the sampler checks that detectors and observables are
 // deterministic), so dim G <= nullity(A) = the free count. The reference run's random results are
 // random whatever the earlier results were (which results are random does not depend on outcomes in a
@@ -50,8 +53,8 @@ struct ExpandPlan {
     // Substitution records, 8 words each (EXPAND_RECORD_WORDS). With n = word 0 >> 16 terms and
     // k = word 0 & 0xFFFF:
     // n = 1: a chain of (word 2) results k, k + w, k + 2w, ... (w = word 3): the first is XORed with
-    // result word 1, each later one with the result before it in the chain (a check repeated round
-    // after round);
+    // result word 1 (M: a word that is always zero), each later one with the result before it in the
+    // chain (a check repeated round after round);
     // 2 <= n <= 7: result k ^= the XOR of results words 1 .. n (the terms inline);
     // n >= 8: result k ^= the XOR of results terms[word 1 .. word 1 + n).
     // Unused words are 0. Thread t runs records [sched_off[p * threads + t], sched_off[p * threads + t +

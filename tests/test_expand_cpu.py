@@ -60,8 +60,10 @@ def test_plan_applies_to_the_headline_circuit(golden):
     s = _stats(golden.circuit('surface_d25_r25'))
     assert s['complete'] == 1, s
     assert s['M'] == 16225 and s['random'] == 624 and s['pivots'] == 16225 - 624
-    # One chain per check through the 25 rounds: three barrier phases for a dependency depth of 48.
-    assert s['phases'] == 3 and s['depth'] == 48, s
+    # One chain per check through the 25 rounds, and runs along the data qubits whose other terms are
+    # XORed in by a phase of independent records before each: five barrier phases for a dependency
+    # depth of 48.
+    assert s['phases'] == 5 and s['depth'] == 48, s
 
 
 @pytest.mark.parametrize('name', MEMORY)
