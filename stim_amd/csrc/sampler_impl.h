@@ -936,8 +936,8 @@ struct mfs_sampler {
                     uint2 r;
                     if (!spill) {
                         for (uint32_t i = nd; i < 4; i++) dets[i] = 0;
-                        r.x = dets[0] | (dets[1] << 14) | (nd << 28) | ((obs & 1u) << 31);
-                        r.y = dets[2] | (dets[3] << 14) | (((obs >> 1) & 1u) << 28);
+                        r.x = dets[0] | (dets[1] << 14) | (((1u << nd) - 1u) << 28);
+                        r.y = dets[2] | (dets[3] << 14) | (((obs >> 1) & 1u) << 28) | ((obs & 1u) << 29);
                     } else {
                         if (oovf.size() >= (1u << 30)) return;
                         r = make_uint2(0, SYM_OREC_OVERFLOW | (uint32_t)oovf.size());

@@ -148,10 +148,10 @@ struct SymDevParams {
     const uint2 *orec_sites;
     const uint32_t *orec_ovf;      // overflow lists: count, then entries (detector b + m, observable D + k)
 };
-// Outcome record (8 bytes): x = e0 | e1 << 14 | n << 28 (n = 0..4 detector entries) | obs0 << 31,
-// y = e2 | e3 << 14 | obs1 << 28; entries are relative to the site's base detector, unused entries
-// are 0. A symptom with more than four detectors has n = 0, y = SYM_OREC_OVERFLOW | offset of its list
-// in orec_ovf.
+// Outcome record (8 bytes): x = e0 | e1 << 14 | valid << 28, y = e2 | e3 << 14 | obs1 << 28 | obs0 << 29;
+// entries are relative to the site's base detector, valid has bit e set for each of the n = 0..4
+// entries in use (bits 0 .. n - 1), unused entries are 0. A symptom with more than four detectors has
+// x = 0, y = SYM_OREC_OVERFLOW | offset of its list in orec_ovf.
 constexpr uint32_t SYM_OREC_OVERFLOW = 1u << 31;
 
 }  // namespace sa

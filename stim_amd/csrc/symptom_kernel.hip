@@ -317,12 +317,25 @@ __device__ __forceinline__ uint2 block_scan(uint32_t v, uint32_t *scratch) {
 }
 
 // Where a hit's symptom entries land in a tile's LDS image. Entry m < D (a detector) of shot s is bit
-// s * det_bits + det_col0 + m of the image; entry m >= D (observable m - D) is bit
-// obs_base + s * obs_bits + obs_col0 + (m - D), or is dropped when observables are not written. The
-// per-shot parts are computed once per hit (dbase, obase), so an entry costs a select, a shift and
-// one LDS XOR, with no branch.
+// s * det_bits + det_col0 + m; entry m >= D (observable m - D) is bit obs_base + s * obs_bits +
+// obs_col0 + (m - D), or is dropped when observables are not written. Bits are numbered across the
+// workgroup's LDS (det_col0 and obs_base include the image's LDS address times 8, a multiple of 32),
+// so bit b is bit b % 32 of the word at LDS byte address b / 32 * 4, reached without adding the image's
+// base. The per-shot parts are computed once per hit (dbase, obase), so an entry costs a select, a
+// shift and one LDS XOR, with no branch.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+
+__device__ __forceinline__ uint32_t lds_address(uint32_t *p) {
+    return (uint32_t)(uintptr_t)(lds_u32 *)p;
+}
+
+// XORs bit b (numbered across LDS) into its word.
+__device__ __forceinline__ void lds_xor_bit(uint32_t b, uint32_t v) {
+    __hip_atomic_fetch_xor((lds_u32 *)(uintptr_t)((b >> 3) & ~3u), v << (b & 31), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 struct Sink {
-    uint32_t *img;
     uint32_t D;
     uint32_t det_bits, det_col0;
     uint32_t obs_base, obs_bits, obs_col0;
@@ -331,17 +344,17 @@ struct Sink {
 
 __device__ __forceinline__ Sink make_sink(const SymDevParams &P, uint32_t *lds, uint32_t det_words) {
     Sink k;
-    k.img = lds;
+    const uint32_t base = lds_address(lds) * 8;
     k.D = P.D;
     k.det_bits = P.det_row_bytes * 8;
-    k.det_col0 = P.dets.col0;
+    k.det_col0 = base + P.dets.col0;
     k.obs_on = P.obs_mode != 0;
     if (P.obs_mode == 2) {
-        k.obs_base = 0;
+        k.obs_base = base;
         k.obs_bits = k.det_bits;
         k.obs_col0 = P.obs_col0;
     } else {
-        k.obs_base = det_words * 32;
+        k.obs_base = base + det_words * 32;
         k.obs_bits = P.obs_row_bytes * 8;
         k.obs_col0 = 0;
     }
@@ -351,7 +364,7 @@ __device__ __forceinline__ Sink make_sink(const SymDevParams &P, uint32_t *lds, 
 __device__ __forceinline__ void sink_bit(const Sink &k, uint32_t m, uint32_t dbase, uint32_t obase) {
     const bool det = m < k.D;
     const uint32_t bit = (det ? dbase : obase) + m;
-    if (det || k.obs_on) atomicXor(&k.img[bit >> 5], 1u << (bit & 31));
+    if (det || k.obs_on) lds_xor_bit(bit, 1u);
 }
 
 __device__ __forceinline__ void sink_record(const Sink &k, const uint32_t *ovf, uint2 rec, uint32_t dbase,
@@ -390,7 +403,7 @@ __device__ __forceinline__ void sink_hit(const SymDevParams &P, const Sink &k, u
 // Compact records (4 bytes; see symptom.h): detector entries go straight to dbase + m; the observable
 // flags to observable bits obase + D + k (skipped when observables are not written).
 __device__ __forceinline__ void xor_bit(const Sink &k, uint32_t bit) {
-    atomicXor(&k.img[bit >> 5], 1u << (bit & 31));
+    lds_xor_bit(bit, 1u);
 }
 
 __device__ __forceinline__ void sink_record32(const SymDevParams &P, const Sink &k, uint32_t rec, uint32_t dbase,
@@ -574,20 +587,17 @@ __device__ __forceinline__ void apply_outcomes(const SymDevParams &P, const Sink
             // detector is below D, so every entry's bit lies in the shot's row.
             const uint32_t dbase = __umul24(shot[k], sink.det_bits) + sink.det_col0 + se[k].y;
             const uint2 r = rec[k];
-            const uint32_t n = (r.x >> 28) & 7u;
-            // An overflow record holds its list offset where entries 2 and 3 would be (and n = 0): read
-            // those entries as 0 so that every address stays inside the tile's image.
+            // An overflow record holds its list offset where entries 2 and 3 would be (and no valid
+            // bits): read those entries, and its observable flags, as 0 so that every address stays
+            // inside the tile's image.
             const uint32_t ry = (r.y & SYM_OREC_OVERFLOW) ? 0u : r.y;
             const uint32_t f[4] = {r.x & 0x3FFFu, (r.x >> 14) & 0x3FFFu, ry & 0x3FFFu, (ry >> 14) & 0x3FFFu};
 #pragma unroll
-            for (uint32_t e = 0; e < 4; e++) {
-                const uint32_t b = dbase + f[e];
-                atomicXor(&sink.img[b >> 5], e < n ? 1u << (b & 31) : 0u);
-            }
-            if (OBS && ((r.x >> 31) | ((r.y >> 28) & 1u))) {
+            for (uint32_t e = 0; e < 4; e++) lds_xor_bit(dbase + f[e], (r.x >> (28 + e)) & 1u);
+            if (OBS && ((ry >> 28) & 3u)) {
                 const uint32_t obase = sink.obs_base + __umul24(shot[k], sink.obs_bits) + sink.obs_col0;
-                if (r.x >> 31) xor_bit(sink, obase);
-                if ((r.y >> 28) & 1u) xor_bit(sink, obase + 1);
+                if ((ry >> 29) & 1u) xor_bit(sink, obase);
+                if ((ry >> 28) & 1u) xor_bit(sink, obase + 1);
             }
             if (r.y & SYM_OREC_OVERFLOW) {
                 const uint32_t obase = sink.obs_base + __umul24(shot[k], sink.obs_bits) + sink.obs_col0;
