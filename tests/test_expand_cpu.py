@@ -87,6 +87,36 @@ def test_expansion_reproduces_detection_events(golden, name):
     np.testing.assert_array_equal(m, x)
 
 
+GENERATED = [
+    ('surface_code:unrotated_memory_z', 5, 4),
+    ('surface_code:unrotated_memory_x', 5, 4),
+    ('surface_code:rotated_memory_x', 7, 7),
+    ('surface_code:rotated_memory_z', 9, 3),
+    ('surface_code:rotated_memory_z', 11, 11),
+    ('repetition_code:memory', 11, 9),
+]
+
+
+@pytest.mark.parametrize('code,d,r', GENERATED)
+def test_expansion_of_generated_circuits(code, d, r):
+    """Generated memory experiments whose plans have substitution runs with pre-phases (five phases for
+    the surface codes): any record x is reproduced up to its free results."""
+    from stim_amd import gen
+    text = gen.generate(code, distance=d, rounds=r, after_clifford_depolarization=1e-3,
+                        before_measure_flip_probability=1e-3)
+    s = _stats(text)
+    assert s['complete'] == 1 and s['free'] == s['random'], s
+    if code.startswith('surface_code') and d >= 7:
+        assert s['phases'] == 5, s
+    rng = np.random.default_rng(d * 100 + r)
+    M = s['M']
+    x = rng.integers(0, 2, size=(32, M)).astype(bool)
+    zero = np.zeros(M, dtype=bool)
+    y = m2d_oracle.convert(text, x, zero)
+    m = _expand(text, y, rng.integers(0, 2, size=(32, s['free'])).astype(np.uint8))
+    np.testing.assert_array_equal(m2d_oracle.convert(text, m, zero), y)
+
+
 def _free_indices(text, M):
     """Unconstrained results: the record indices that no row pivots on, found by expanding the zero
     record with one free bit set at a time."""
