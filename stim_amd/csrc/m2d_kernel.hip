@@ -10,6 +10,10 @@
 //
 // Work is laid out so that lane-dependent branches are rare: loads and LDS stores whose lanes fall
 // past the tile's shots or the row's end go to clamped addresses or padding that no output reads.
+//
+// Also here: the windowed conversion kernel (a ring of recent measurement words per tile, for records
+// longer than LDS holds; m2d.h) and the measurement sampler's expansion kernel (detection events to
+// measurement records; expand.h).
 #include <hip/hip_runtime.h>
 
 #include "bitops.h"
@@ -88,7 +92,7 @@ __device__ __forceinline__ void flush_words(const uint8_t *img, uint8_t *dst, ui
 // Produces columns [c0, c0 + ncols) of the tile (ncols >= 1), transposed, into shot rows dst (bit 0 =
 // column c0), CHUNK_BLOCKS column blocks at a time: lane l of a half-wave gets shot l's 4 bytes of a
 // 32-column block and puts them in the chunk's LDS image (PITCH bytes per shot); the chunk is then
-// copied to the rows with coalesced byte stores (rows need no alignment). Image slots of lanes past
+// copied to the rows with 4-byte stores at any alignment (flush_words). Image slots of lanes past
 // the tile's shots or blocks past the last are written but never copied; columns past the last
 // read as zero, so the padding bits of each row's last byte are 0.
 __device__ __forceinline__ void emit_columns(const M2dParams &P, const uint32_t *words, uint8_t *img, uint8_t *dst,
