@@ -1160,6 +1160,8 @@ struct mfs_sampler {
         P.thr = d_thr.as<uint32_t>();
         P.nclasses = (uint32_t)plan.classes.size();
         P.otab_words = (uint32_t)plan.otab.size();
+        // DEM samplers have many probability buckets per tile: each wave draws its share of them.
+        P.wave_split = dem_mode ? 1u : 0u;
         P.D = prog.num_detectors;
         P.O = prog.num_observables;
         P.shots_per_tile = S;

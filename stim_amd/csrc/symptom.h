@@ -117,6 +117,7 @@ struct SymDevParams {
     const uint32_t *thr;           // per-site thresholds of thinned groups (8-byte-record path)
     uint32_t nclasses;
     uint32_t otab_words;           // outcome-table words (copied to LDS when small)
+    uint32_t wave_split;           // scan kernel: each wave draws its share of every group (DEM buckets)
     uint32_t D, O;
     uint32_t shots_per_tile;       // power of two
     uint32_t tile_log2;

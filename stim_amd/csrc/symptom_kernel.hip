@@ -453,6 +453,155 @@ __device__ __forceinline__ void record_error(const SymDevParams &P, uint64_t row
     atomicOr(&P.err_words[row * P.err_stride_words + (mech >> 5)], 1u << (mech & 31));
 }
 
+// Saturating prefix sum over the 64 lanes of a wave. Returns (exclusive, total).
+__device__ __forceinline__ uint2 wave_scan(uint32_t v) {
+    const uint32_t lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= (uint32_t)d) x = sat_add(x, y);
+    }
+    const uint32_t total = __shfl(x, 63, 64);
+    uint32_t ex = __shfl_up(x, 1, 64);
+    if (lane == 0) ex = 0;
+    return make_uint2(ex, total);
+}
+
+// Position where draw-space coordinate a falls in a group that starts at draw-space coordinate c and
+// has N positions at probability p (draw space: expected draws, accumulated over the groups in
+// order). Adjacent waves evaluate the same expression for their shared edge, so their ranges meet.
+__device__ __forceinline__ uint32_t draw_position(float a, float c, float p, uint32_t N) {
+    if (a <= c) return 0;
+    const float x = (a - c) / p;
+    return x >= (float)N ? N : (uint32_t)x;
+}
+
+// One tile of the scan sampler with the groups' positions shared among the waves (DEM samplers, whose
+// many probability buckets would each cost a workgroup scan round): the groups' expected draws are
+// laid end to end and cut into NT / 64 equal shares, one per wave; a wave draws every group's range
+// that falls in its share, restarting the process at each range's start (exact, since geometric gaps
+// are memoryless), with prefix scans within the wave and no workgroup barrier between the image's
+// zeroing and its write-out. Streams are keyed by (tile, group, round, lane), as in scan_tile.
+__device__ __forceinline__ void scan_tile_waves(const SymDevParams &P, uint32_t btile, uint32_t *lds) {
+    const uint32_t S = P.shots_per_tile;
+    const uint32_t det_words = (S * P.det_row_bytes + 15) / 16 * 4;
+    const uint32_t obs_words = (S * P.obs_row_bytes + 15) / 16 * 4;
+    uint32_t *const otab_lds = lds + det_words + obs_words + 8;
+    const bool otab_in_lds = P.otab_words <= SYM_OTAB_LDS_WORDS;
+    const uint32_t *otab = otab_in_lds ? otab_lds : P.otab;
+    const uint32_t tid = threadIdx.x, wave = tid >> 6;
+    {
+        uint4 *const v = reinterpret_cast<uint4 *>(lds);
+        for (uint32_t i = tid; i < (det_words + obs_words) / 4; i += NT) v[i] = make_uint4(0, 0, 0, 0);
+    }
+    if (otab_in_lds) {
+        for (uint32_t i = tid; i < P.otab_words; i += NT) otab_lds[i] = P.otab[i];
+    }
+    const Sink sink = make_sink(P, lds, det_words);
+    float total = 0.0f;
+    for (uint32_t gi = 0; gi < P.ngroups; gi++) {
+        const SymGroup G = P.groups[gi];
+        if (G.p > 0.0f) total += (float)(G.nsites << P.tile_log2) * G.p;
+    }
+    const float a0 = total * (float)wave / (float)(NT / 64), a1 = total * (float)(wave + 1) / (float)(NT / 64);
+    __syncthreads();
+
+    const uint64_t s0 = (uint64_t)btile * S;
+    const uint64_t tile64 = P.tile0 + btile;
+    const uint32_t tile = (uint32_t)tile64, lane_word = sym_lane_word(tid, tile64);
+    const uint32_t k0 = (uint32_t)P.seed, k1 = (uint32_t)(P.seed >> 32);
+    float c = 0.0f;
+    for (uint32_t gi = 0; gi < P.ngroups; gi++) {
+        const SymGroup G = P.groups[gi];
+        if (!(G.p > 0.0f)) continue;
+        const uint32_t Ng = G.nsites << P.tile_log2;
+        const float h = (float)Ng * G.p;
+        const uint32_t lo = draw_position(a0, c, G.p, Ng);
+        const uint32_t hi = wave + 1 == NT / 64 ? Ng : draw_position(a1, c, G.p, Ng);
+        c += h;
+        if (hi <= lo) continue;
+        const uint32_t N = hi - lo;
+        const float mu = (float)N * G.p;
+        uint32_t m = (uint32_t)((mu + 3.0f * sqrtf(mu) + 4.0f) / 64) + 1;
+        if (m > SCAN_M_MAX) m = SCAN_M_MAX;
+        uint32_t base = 0;
+        for (uint32_t round = 0; base < N; round++) {
+            uint32_t g[SCAN_M_MAX], oc[SCAN_M_MAX];
+            uint32_t sum = 0;
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX / 2; k++) {
+                if (2 * k < m) {
+                    const uint4 r = philox10(make_uint4(round * (SCAN_M_MAX / 2) + k, lane_word, tile, TAG_SCAN | gi), k0, k1);
+                    g[2 * k] = gap32(r.x, G.inv_log);
+                    oc[2 * k] = r.y;
+                    g[2 * k + 1] = gap32(r.z, G.inv_log);
+                    oc[2 * k + 1] = r.w;
+                } else {
+                    g[2 * k] = g[2 * k + 1] = 0;
+                    oc[2 * k] = oc[2 * k + 1] = 0;
+                }
+            }
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
+                if (k < m) sum = sat_add(sum, g[k] + 1);
+            }
+            const uint2 sc = wave_scan(sum);
+            uint32_t pos = sat_add(base, sc.x);
+            pos = pos < N ? pos : N;
+#pragma unroll
+            for (uint32_t k = 0; k < SCAN_M_MAX; k++) {
+                if (k < m) {
+                    pos += g[k] < N ? g[k] : N;
+                    pos = pos < N ? pos : N;
+                    if (pos < N) {
+                        const uint32_t srel = (lo + pos) >> P.tile_log2;
+                        const uint32_t shot = (lo + pos) & (S - 1);
+                        // Member class of the site (a group holds up to 4 classes in site order).
+                        const uint32_t mk = (srel >= G.bound[1] ? 1u : 0u) + (srel >= G.bound[2] ? 1u : 0u) +
+                                            (srel >= G.bound[3] ? 1u : 0u);
+                        const uint32_t nf = mk == 0 ? G.nout[0] : mk == 1 ? G.nout[1] : mk == 2 ? G.nout[2] : G.nout[3];
+                        uint32_t code;
+                        if (nf & SYM_SEQ) {
+                            code = __umulhi(oc[k], nf & 0xFFFFu) + 1;
+                        } else {
+                            const uint32_t wo =
+                                mk == 0 ? G.woff[0] : mk == 1 ? G.woff[1] : mk == 2 ? G.woff[2] : G.woff[3];
+                            code = outcome(otab, wo, nf, oc[k]);
+                        }
+                        if (G.flags & SYM_DEM_SLOTS) code = SYM_DEM_SLOT_CODE;
+                        const bool thin = (G.flags & SYM_THIN) != 0;
+                        bool fired;
+                        if (P.recs32) {
+                            fired = sink_slot32(P, sink, G.rec0 + srel * G.rstride, G.rstride, code, shot, thin, oc[k]);
+                        } else {
+                            fired = !thin || oc[k] <= P.thr[G.site0 + srel];
+                            if (fired) {
+                                const SymClass cl = P.classes[G.c0];
+                                sink_hit(P, sink, cl.col0 + srel * cl.basis, code, shot, cl.basis);
+                            }
+                        }
+                        if (P.err_words && fired && s0 + shot < P.shots)
+                            record_error(P, s0 + shot, P.site_mech[G.site0 + srel]);
+                    }
+                    pos += 1;
+                }
+            }
+            base = sat_add(base, sc.y);
+            base = base < N ? base : N;
+        }
+    }
+    __syncthreads();
+
+    const uint64_t valid = P.shots > s0 ? (P.shots - s0 < S ? P.shots - s0 : S) : 0;
+    if (P.dets.enabled)
+        write_rows(reinterpret_cast<const uint8_t *>(lds), P.det_row_bytes, (uint32_t)valid,
+                   P.dets.ptr + s0 * P.dets.stride, P.dets.stride);
+    if (P.obs.enabled && P.obs_row_bytes)
+        write_rows(reinterpret_cast<const uint8_t *>(lds + det_words), P.obs_row_bytes, (uint32_t)valid,
+                   P.obs.ptr + s0 * P.obs.stride, P.obs.stride);
+}
+
 // One tile of the scan sampler: tile `btile` of the launch described by P, in the workgroup's LDS.
 __device__ __forceinline__ void scan_tile(const SymDevParams &P, uint32_t btile, uint32_t *lds) {
     const uint32_t S = P.shots_per_tile;
@@ -717,6 +866,12 @@ __global__ __launch_bounds__(NT, 6) void sym_scan_kernel(SymDevParams P) {
     scan_tile(P, blockIdx.x, lds);
 }
 
+// The wave-split form (P.wave_split: DEM samplers) in a kernel of its own.
+__global__ __launch_bounds__(NT, 6) void sym_scan_waves_kernel(SymDevParams P) {
+    extern __shared__ __align__(16) uint32_t lds[];
+    scan_tile_waves(P, blockIdx.x, lds);
+}
+
 // The specialised form holds a round's record loads in registers (16 more VGPRs); at d=25 its
 // 31 KB tile image allows 5 workgroups per CU, so it asks for 5 waves per SIMD.
 template <bool OBS>
@@ -731,6 +886,8 @@ __device__ __forceinline__ void batch_tile(const SymDevParams &P, uint32_t btile
     if (P.orecs) {
         if (P.obs_mode != 0) scan_tile_fast<true>(P, btile, lds);
         else scan_tile_fast<false>(P, btile, lds);
+    } else if (P.wave_split) {
+        scan_tile_waves(P, btile, lds);
     } else {
         scan_tile(P, btile, lds);
     }
@@ -947,6 +1104,13 @@ hipError_t launch_sym_scan(const SymDevParams &P, uint32_t tiles, size_t lds, hi
             if (e != hipSuccess) return e;
             hipLaunchKernelGGL(sym_scan_fast_kernel<false>, dim3(tiles), dim3(NT), lds, stream, P);
         }
+        return hipGetLastError();
+    }
+    if (P.wave_split) {
+        static size_t reserved_waves = 0;
+        const hipError_t e = reserve_lds((const void *)sym_scan_waves_kernel, lds, reserved_waves);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(sym_scan_waves_kernel, dim3(tiles), dim3(NT), lds, stream, P);
         return hipGetLastError();
     }
     static size_t reserved = 0;
