@@ -524,7 +524,7 @@ def dem(args):
                    'lds_bytes_per_tile': engine['lds_bytes_per_tile'], 'probability_buckets': engine['classes'],
                    'parallelism': 'single GPU'},
         'roofline': {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
-                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'kernel': 'sym_scan_kernel',
+                     'frac': achieved / HBM_PEAK_GBS, 'traffic': None, 'kernel': 'sym_scan_waves_kernel',
                      'kernel_ms': kernel_ms, 'algorithmic_bytes_per_launch': b_alg},
         'compile_s': compile_s,
         'cpu_baseline': None if args.no_cpu else dem_cpu_baseline(text),
