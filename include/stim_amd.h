@@ -115,10 +115,11 @@ int mfs_m2d_bench(mfs_m2d *c, uint64_t shots, int warmup, int iters, double out_
  * less than half the LDS (default), 1 = whole-record kernel, 2 = windowed (conversions it does not
  * apply to then fail with MFS_ERR_INVALID_ARGUMENT). */
 int mfs_m2d_set_kernel(mfs_m2d *c, int kernel);
-/* out[0..1] = ring words of the windowed kernel with observables appended / with detectors only in
- * the output rows (0: not applicable), out[2] = LDS bytes per tile of the whole-record kernel,
- * out[3] = of the windowed kernel (appended layout; 0 when not applicable). */
-int mfs_m2d_kernel_info(const mfs_m2d *c, uint64_t out[4]);
+/* out[0..3] = ring words of the windowed kernel per output layout (0: not applicable): observables
+ * appended, observables separate, detectors only, observables appended and separate; out[4] = LDS
+ * bytes per tile of the whole-record kernel, out[5] = of the windowed kernel (appended layout; 0 when
+ * not applicable). */
+int mfs_m2d_kernel_info(const mfs_m2d *c, uint64_t out[6]);
 
 /* Reference sample of a circuit (host only, no GPU): the noiseless run with every random measurement
  * result taken as 0, written bit-packed little-endian into out[0 .. ceil(num_measurements / 8)).

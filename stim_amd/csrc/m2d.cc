@@ -143,14 +143,15 @@ std::vector<uint8_t> sweep_effect_rows(const Block &block, uint32_t K, uint32_t 
 constexpr uint32_t MAX_RING_WORDS = 8192;
 
 // Window of one output layout: part 0 = columns [0, n0), emitted block by block as their terms arrive;
-// columns [n0, cols) (separate observables) after the last chunk. ring_words = 0 when some column's
-// terms lie further apart than MAX_RING_WORDS allows.
+// with `tail`, the observable columns [D, cols) also go to separate rows after the last chunk.
+// ring_words = 0 when some column's terms lie further apart than MAX_RING_WORDS allows.
 struct WinPlan {
     uint32_t ring_words = 0;
     std::vector<uint32_t> chunk_end;   // per chunk: part-0 blocks whose last term it holds, cumulative
 };
 
-WinPlan plan_window(const std::vector<std::vector<uint32_t>> &col_terms, uint32_t n0, uint32_t M) {
+WinPlan plan_window(const std::vector<std::vector<uint32_t>> &col_terms, uint32_t n0, uint32_t D, bool tail,
+                    uint32_t M) {
     WinPlan W;
     const uint32_t nchunks = (M + M2D_WIN_CHUNK - 1) / M2D_WIN_CHUNK;
     uint64_t need = M2D_WIN_CHUNK;   // a ring holds at least one chunk
@@ -168,7 +169,7 @@ WinPlan plan_window(const std::vector<std::vector<uint32_t>> &col_terms, uint32_
             for (uint32_t t : col_terms[j]) need = std::max<uint64_t>(need, (uint64_t)(prev + 1) * M2D_WIN_CHUNK - t);
         }
     }
-    for (size_t j = n0; j < col_terms.size(); j++) {
+    for (size_t j = D; tail && j < col_terms.size(); j++) {
         for (uint32_t t : col_terms[j]) need = std::max<uint64_t>(need, (uint64_t)nchunks * M2D_WIN_CHUNK - t);
     }
     uint32_t ring = M2D_WIN_CHUNK;
@@ -186,12 +187,13 @@ struct mfs_m2d {
     std::vector<uint2> cols;
     std::vector<uint32_t> ovf;
     DevBuf d_cols, d_ovf, d_in, d_out, d_obs, d_tmp, d_sweep;
-    // Windowed kernel (no sweep bits): layout 0 = observables appended to part 0, layout 1 = detectors
-    // only in part 0 (observables separate or absent). kernel_mode: 0 = the whole-record kernel where
-    // the tile's record fits in LDS, else windowed; 1 = whole-record kernel; 2 = windowed (refused
-    // where it does not apply).
-    WinPlan win[2];
-    DevBuf d_chunk_end[2];
+    // Windowed kernel (no sweep bits), one plan per output layout: 0 = observables appended, 1 =
+    // observables separate, 2 = detectors only, 3 = observables appended and separate. kernel_mode:
+    // 0 = the whole-record kernel where the tile's record fits in LDS, else windowed; 1 = whole-record
+    // kernel; 2 = windowed (refused where it does not apply).
+    WinPlan win[4];
+    DevBuf d_chunk_end[4];
+    bool any_window() const { return win[0].ring_words || win[1].ring_words || win[2].ring_words || win[3].ring_words; }
     int kernel_mode = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -236,8 +238,10 @@ struct mfs_m2d {
     // Windowed layout usable for a launch, or -1.
     int window_for(const M2dParams &P) const {
         if (K || !P.in_vec || !M) return -1;
-        const int l = P.n_out == D + O ? 0 : 1;
-        if (l == 1 && P.n_out != D) return -1;
+        int l;
+        if (P.n_out == D + O) l = P.n_obs ? 3 : 0;
+        else if (P.n_out == D) l = P.n_obs ? 1 : 2;
+        else return -1;
         return win[l].ring_words ? l : -1;
     }
     bool whole_fits() const { return m2d_lds_bytes(blocks(), sweep_blocks()) <= MAX_LDS; }
@@ -304,12 +308,14 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
         }
         for (auto &ts : obs_terms) col_terms.push_back(std::move(ts));
         if (!c->K) {
-            c->win[0] = plan_window(col_terms, c->D + c->O, c->M);
-            c->win[1] = plan_window(col_terms, c->D, c->M);
+            c->win[0] = plan_window(col_terms, c->D + c->O, c->D, false, c->M);
+            c->win[1] = plan_window(col_terms, c->D, c->D, true, c->M);
+            c->win[2] = plan_window(col_terms, c->D, c->D, false, c->M);
+            c->win[3] = plan_window(col_terms, c->D + c->O, c->D, true, c->M);
         }
         // The whole-record kernel keeps a tile's measurements and sweep bits in LDS; the windowed one
         // only a ring of recent results.
-        if (!c->whole_fits() && !c->win[0].ring_words && !c->win[1].ring_words) {
+        if (!c->whole_fits() && !c->any_window()) {
             uint32_t max_blocks = 0;
             while (m2d_lds_bytes(max_blocks + 1, c->sweep_blocks()) <= MAX_LDS) max_blocks++;
             throw std::invalid_argument("the converter keeps a tile's measurements and sweep bits in LDS: at most " +
@@ -361,7 +367,7 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
         check(hipStreamCreate(&c->stream), "stream");
         mfs_sampler::upload(c->d_cols, c->cols, "m2d columns");
         mfs_sampler::upload(c->d_ovf, c->ovf, "m2d overflow terms");
-        for (int l = 0; l < 2; l++) {
+        for (int l = 0; l < 4; l++) {
             if (c->win[l].ring_words) mfs_sampler::upload(c->d_chunk_end[l], c->win[l].chunk_end, "m2d window");
         }
         *out = c.release();
@@ -384,7 +390,7 @@ int mfs_m2d_stats(const mfs_m2d *c, uint64_t out[4]) {
 int mfs_m2d_set_kernel(mfs_m2d *c, int kernel) {
     return guarded([&] {
         if (kernel < 0 || kernel > 2) throw std::invalid_argument("kernel must be 0 (auto), 1 (whole record) or 2 (windowed)");
-        if (kernel == 2 && !c->win[0].ring_words && !c->win[1].ring_words)
+        if (kernel == 2 && !c->any_window())
             throw std::invalid_argument("the windowed conversion kernel does not apply to this circuit");
         if (kernel == 1 && !c->whole_fits())
             throw std::invalid_argument("the whole-record conversion kernel does not fit this circuit's record in LDS");
@@ -392,12 +398,11 @@ int mfs_m2d_set_kernel(mfs_m2d *c, int kernel) {
     });
 }
 
-int mfs_m2d_kernel_info(const mfs_m2d *c, uint64_t out[4]) {
+int mfs_m2d_kernel_info(const mfs_m2d *c, uint64_t out[6]) {
     return guarded([&] {
-        out[0] = c->win[0].ring_words;
-        out[1] = c->win[1].ring_words;
-        out[2] = m2d_lds_bytes(c->blocks(), c->sweep_blocks());
-        out[3] = c->win[0].ring_words ? m2d_win_lds_bytes(c->win[0].ring_words, (c->M + M2D_WIN_CHUNK - 1) / M2D_WIN_CHUNK) : 0;
+        for (int l = 0; l < 4; l++) out[l] = c->win[l].ring_words;
+        out[4] = m2d_lds_bytes(c->blocks(), c->sweep_blocks());
+        out[5] = c->win[0].ring_words ? m2d_win_lds_bytes(c->win[0].ring_words, (c->M + M2D_WIN_CHUNK - 1) / M2D_WIN_CHUNK) : 0;
     });
 }
 

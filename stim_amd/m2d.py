@@ -141,10 +141,10 @@ class CompiledMeasurementsToDetectionEventsConverter:
 
     def kernel_info(self) -> dict:
         """Ring words of the windowed kernel per output layout (0: not applicable) and LDS bytes per tile."""
-        out = (ctypes.c_uint64 * 4)()
+        out = (ctypes.c_uint64 * 6)()
         N.check(N.lib().mfs_m2d_kernel_info(self._h, out))
-        return {'ring_words_appended': out[0], 'ring_words_detectors': out[1], 'lds_whole': out[2],
-                'lds_windowed': out[3]}
+        return {'ring_words_appended': out[0], 'ring_words_separate': out[1], 'ring_words_detectors': out[2],
+                'ring_words_appended_and_separate': out[3], 'lds_whole': out[4], 'lds_windowed': out[5]}
 
     def bench_device(self, shots: int, warmup: int = 2, iters: int = 5) -> float:
         """Average kernel time (ms) of converting `shots` device-resident rows (observables appended)."""

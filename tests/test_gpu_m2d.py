@@ -65,7 +65,7 @@ def test_windowed_kernel_matches_reference_conversions(golden):
         text = c['circuit'] or golden.circuit(c['name'])
         conv = _converter(text, c['skip_reference_sample'])
         info = conv.kernel_info()
-        if not info['ring_words_appended'] or not info['ring_words_detectors']:
+        if not info['ring_words_appended'] or not info['ring_words_separate']:
             continue
         conv.set_kernel('windowed')
         M, D, O = conv.num_measurements, conv.num_detectors, conv.num_observables
@@ -76,6 +76,10 @@ def test_windowed_kernel_matches_reference_conversions(golden):
         d, o = conv.convert(measurements=meas, separate_observables=True)
         np.testing.assert_array_equal(d, want[:, :D], err_msg=c['name'])
         np.testing.assert_array_equal(o, want[:, D:], err_msg=c['name'])
+        if info['ring_words_appended_and_separate']:
+            da, oa = conv.convert(measurements=meas, separate_observables=True, append_observables=True)
+            np.testing.assert_array_equal(da, want, err_msg=c['name'])
+            np.testing.assert_array_equal(oa, want[:, D:], err_msg=c['name'])
         n += 1
         saw_d25 |= c['name'] == 'surface_d25_r25'
     assert n >= 20 and saw_d25
@@ -87,7 +91,8 @@ def test_windowed_kernel_equals_whole_record_kernel(golden, name):
     text = golden.circuit(name)
     conv = _converter(text)
     info = conv.kernel_info()
-    assert info['ring_words_appended'] and info['ring_words_detectors'], info
+    assert all(info[k] for k in ('ring_words_appended', 'ring_words_separate', 'ring_words_detectors',
+                                 'ring_words_appended_and_separate')), info
     assert info['lds_windowed'] <= 8192 * 4 + 32 * 260
     M = conv.num_measurements
     shots = 1000 if M > 4000 else 4099
@@ -98,7 +103,9 @@ def test_windowed_kernel_equals_whole_record_kernel(golden, name):
         conv.set_kernel(kernel)
         got[kernel] = (conv.convert(measurements=packed, bit_packed=True, append_observables=True),
                        conv.convert(measurements=packed, bit_packed=True, separate_observables=True),
-                       conv.convert(measurements=packed, bit_packed=True, append_observables=False))
+                       conv.convert(measurements=packed, bit_packed=True, append_observables=False),
+                       conv.convert(measurements=packed, bit_packed=True, append_observables=True,
+                                    separate_observables=True))
     for a, b in zip(got['whole'], got['windowed']):
         for x, y in zip(a if isinstance(a, tuple) else (a,), b if isinstance(b, tuple) else (b,)):
             np.testing.assert_array_equal(x, y)
@@ -116,7 +123,7 @@ def test_records_longer_than_lds_use_the_windowed_kernel():
     M, D = conv.num_measurements, conv.num_detectors
     assert M == 48081 and D == 48000
     info = conv.kernel_info()
-    assert info['lds_whole'] > 160 * 1024 and info['ring_words_appended'] and info['ring_words_detectors'], info
+    assert info['lds_whole'] > 160 * 1024 and info['ring_words_appended'] and info['ring_words_separate'], info
     with pytest.raises(ValueError, match='whole-record'):
         conv.set_kernel('whole')
     rng = np.random.default_rng(21)
@@ -126,6 +133,18 @@ def test_records_longer_than_lds_use_the_windowed_kernel():
     d, o = conv.convert(measurements=meas, separate_observables=True)
     np.testing.assert_array_equal(d, want[:, :D])
     np.testing.assert_array_equal(o, want[:, D:])
+    # An observable on the first result: appended, its column is emitted when its block is ready;
+    # separate, it would be emitted after the last chunk, when that result has left the ring.
+    text2 = text + 'OBSERVABLE_INCLUDE(1) rec[-48081]\n'
+    conv2 = _converter(text2)
+    info2 = conv2.kernel_info()
+    assert info2['ring_words_appended'] == info2['ring_words_separate'] == 0 and info2['ring_words_detectors'], info2
+    with pytest.raises(ValueError):
+        conv2.convert(measurements=meas, separate_observables=True)
+    with pytest.raises(ValueError):
+        conv2.convert(measurements=meas, append_observables=True)
+    want2 = mo.convert(text2, meas, stim_amd.Circuit(text2).reference_sample())
+    np.testing.assert_array_equal(conv2.convert(measurements=meas, append_observables=False), want2[:, :D])
 
 
 def test_windowed_kernel_refused_where_it_does_not_apply(golden):
