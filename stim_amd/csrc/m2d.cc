@@ -350,7 +350,8 @@ int mfs_m2d_compile(const char *stim_text, size_t len, int skip_reference_sample
             const uint8_t f = invert[j];
             uint2 r;
             if (ts.size() <= 2) {
-                r = make_uint2(((uint32_t)ts.size() << 28) | (ts.size() >= 1 ? ts[0] : 0u), ts.size() == 2 ? ts[1] : 0u);
+                r = make_uint2(((uint32_t)ts.size() << 28) | (ts.size() >= 1 ? ts[0] : zero_word),
+                               ts.size() == 2 ? ts[1] : zero_word);
             } else {
                 if (ts.size() > M2D_INDEX_MASK || c->ovf.size() + ts.size() + 3 >= (1ull << 32))
                     throw std::invalid_argument("too many measurement terms for the converter");

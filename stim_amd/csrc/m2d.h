@@ -69,7 +69,7 @@ struct ExpandDev {
 
 // Column record: bits 28..29 of x = term count class c, bit 31 = the column's reference parity
 // (invert). c = 0..2: the column has c terms; x bits 0..27 = first measurement index, y = second
-// (unused fields hold 0). c = 3: x bits 0..27 = term count n, y = offset of the term list in the
+// (unused fields name the always-zero word, m2d_zero_word, so both words are read and XORed). c = 3: x bits 0..27 = term count n, y = offset of the term list in the
 // overflow table (a multiple of 4); the list holds the n indices padded to a multiple of 4 with
 // the index of a word that is always zero (m2d_zero_word), so it is read 16 bytes at a time.
 constexpr uint32_t M2D_INDEX_MASK = 0x0FFFFFFFu;

@@ -43,23 +43,28 @@ __device__ __forceinline__ uint32_t load_word_bytes(const uint8_t *row, uint32_t
     return x;
 }
 
-__device__ __forceinline__ uint32_t column_word(const M2dParams &P, const uint32_t *words, uint2 r) {
-    const uint32_t c = (r.x >> 28) & 3u;
+// Word i of the tile's words, read at its LDS byte address (the words start at LDS address `base`,
+// 0 in these kernels, so no base is added per read).
+typedef __attribute__((address_space(3))) const uint32_t lds_cu32;
+__device__ __forceinline__ uint32_t lds_word(uint32_t base, uint32_t i) {
+    return *(lds_cu32 *)(uintptr_t)(base + 4 * i);
+}
+
+__device__ __forceinline__ uint32_t column_word(const M2dParams &P, uint32_t base, uint2 r) {
     uint32_t v;
-    if (c == 3) {
+    if (((r.x >> 28) & 3u) == 3u) {
         v = 0;
         const uint32_t n = r.x & M2D_INDEX_MASK;
         const uint4 *q = reinterpret_cast<const uint4 *>(P.ovf + r.y);
         for (uint32_t t = 0; t < n; t += 4) {
             const uint4 a = q[t / 4];
-            v ^= words[a.x] ^ words[a.y] ^ words[a.z] ^ words[a.w];
+            v ^= lds_word(base, a.x) ^ lds_word(base, a.y) ^ lds_word(base, a.z) ^ lds_word(base, a.w);
         }
     } else {
-        // Unused index fields hold 0, so both reads stay inside the tile's words.
-        const uint32_t w0 = words[r.x & M2D_INDEX_MASK], w1 = words[r.y];
-        v = (c >= 1 ? w0 : 0u) ^ (c == 2 ? w1 : 0u);
+        // Unused index fields name the zero word, so both words are XORed unconditionally.
+        v = lds_word(base, r.x & M2D_INDEX_MASK) ^ lds_word(base, r.y);
     }
-    return (r.x & M2D_INVERT) ? ~v : v;
+    return v ^ (0u - (r.x >> 31));
 }
 
 // Copies staged rows [0, valid) (pitch P bytes in LDS) to bytes [off, off + nbytes) of shot rows s0 + r
@@ -101,6 +106,7 @@ __device__ __forceinline__ void emit_columns(const M2dParams &P, const uint32_t 
     constexpr uint32_t CB = CHUNK_BLOCKS / HALVES;
     const uint32_t nb = (ncols + 31) / 32, pitch = (ncols + 7) / 8;
     uint32_t *const slot = reinterpret_cast<uint32_t *>(img + lane * PITCH);
+    const uint32_t base = (uint32_t)(uintptr_t)(lds_cu32 *)words;
     for (uint32_t cb0 = 0; cb0 < nb; cb0 += CHUNK_BLOCKS) {
         // Every lane of a half-wave runs the same iterations (the transpose moves data within the half).
         uint2 r[CB];
@@ -112,7 +118,7 @@ __device__ __forceinline__ void emit_columns(const M2dParams &P, const uint32_t 
         }
 #pragma unroll
         for (uint32_t u = 0; u < CB; u++) {
-            slot[half * CB + u] = transpose32(column_word(P, words, r[u]), lane);
+            slot[half * CB + u] = transpose32(column_word(P, base, r[u]), lane);
         }
         __syncthreads();
         const uint32_t off = 4 * cb0;
