@@ -255,6 +255,10 @@ struct mfs_m2d {
         // The kernel addresses a tile's output rows with 32-bit offsets.
         if (P.out_stride >= (1ull << 30) || P.obs_stride >= (1ull << 30))
             throw std::invalid_argument("output row strides must be below 2^30 bytes");
+        if ((kernel_mode == 2 || !whole_fits()) && !P.in_vec && M && !K)
+            throw std::invalid_argument("the windowed conversion kernel reads device measurement rows in place only "
+                                        "when they are 16-byte aligned with a stride of at least 16 * ceil(M / 128) "
+                                        "bytes that is a multiple of 16");
         if ((kernel_mode == 2 || !whole_fits()) && window_for(P) < 0)
             throw std::invalid_argument(kernel_mode == 2
                 ? "the windowed conversion kernel does not apply to this circuit or layout"
